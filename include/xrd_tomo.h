@@ -1,0 +1,229 @@
+/*
+ * xrd_tomo.h -- C ABI of the MI355X (gfx950) XRD-tomography forward projector
+ * and MLEM reconstruction.
+ *
+ * Replaces the MATLAB MEX gateway of airidas-korolkovas/XRD_Tomography:
+ *   void mexFunction(int nlhs, mxArray *plhs[], int nrhs, mxArray const *prhs[])
+ *   (Forward_XRD.cu:76-302), called from main.m:56-64,
+ * and the MLEM loop the reference runs in MATLAB on the CPU (main.m:141-176).
+ *
+ * Conventions
+ *  - Every array argument is a caller-owned DEVICE pointer unless the comment
+ *    says otherwise.  No hidden copies of the inputs are made; outputs are
+ *    fully overwritten (the reference zero-initialises them,
+ *    Forward_XRD.cu:143-166, and then writes every element).
+ *  - All layouts are "first index fastest" (MATLAB column-major), exactly as
+ *    the reference (Forward_XRD.cu:143-166, integrate_diffraction.cu:365,404).
+ *  - Every call returns an int status: XRD_OK (0) or a negative XRD_ERR_*.
+ *    xrd_last_error_string() describes the last failure of the calling thread.
+ *    Nothing aborts.  The reference performs no argument or CUDA error checks.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t passed as void*;
+ *    NULL = default stream) and asynchronous with respect to the host unless
+ *    documented otherwise.
+ *  - Workspace lives in an xrd_context.  A context is not re-entrant: calls
+ *    that use the same context must not overlap in time (one host thread at a
+ *    time, and calls on different streams must be ordered by the caller).
+ *    Different contexts are fully independent.
+ */
+#ifndef XRD_TOMO_H
+#define XRD_TOMO_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define XRD_OK                 0
+#define XRD_ERR_INVALID_ARG   -1   /* a scalar argument is out of range        */
+#define XRD_ERR_NULL_POINTER  -2   /* a required pointer is NULL               */
+#define XRD_ERR_DIMS          -3   /* dimensions <= 0, inconsistent or too big */
+#define XRD_ERR_ALIGN         -4   /* a pointer is not suitably aligned        */
+#define XRD_ERR_HIP           -5   /* a HIP runtime call failed                */
+#define XRD_ERR_ALLOC         -6   /* workspace allocation failed              */
+#define XRD_ERR_UNSUPPORTED   -7   /* option not supported by this build       */
+#define XRD_ERR_NONFINITE     -8   /* a host-side scalar is NaN or Inf         */
+
+/* ---- reference-quirk switches (xrd_discretization.flags) ------------------ */
+/* Compton term uses the (a1,a2) of the previous non-air voxel in the
+ * reference's loop order (nx outer, ny inner); the first non-air voxel uses 0.
+ * Reproduces the read-before-write at integrate_diffraction.cu:181 vs :192-193. */
+#define XRD_QUIRK_STALE_COMPTON_VOXEL  0x1u
+/* Source-leg and detector-leg line integrals swapped, as passed at
+ * integrate_diffraction.cu:259-260.  Only the Compton term (:347) sees it. */
+#define XRD_QUIRK_SWAPPED_LEGS         0x2u
+/* Both quirks: the reference kernel's exact semantics. */
+#define XRD_QUIRKS_REFERENCE (XRD_QUIRK_STALE_COMPTON_VOXEL | XRD_QUIRK_SWAPPED_LEGS)
+
+/* ---- problem descriptors --------------------------------------------------- */
+
+/* Scanner geometry.  All lengths are in voxel units; the voxel grid covers
+ * [0,nx] x [0,ny] and voxel (ix,iy) is centred at (ix+0.5, iy+0.5)
+ * (integrate_diffraction.cu:192-193).  These fields are compile-time literals
+ * in the reference (integrate_diffraction.cu:81-128, Forward_XRD.cu:17-20). */
+typedef struct xrd_geometry {
+    int32_t nx, ny;          /* voxel grid (dims of `segmentation`, Forward_XRD.cu:117-119) */
+    int32_t n_views;         /* SRC: source positions                                  */
+    int32_t n_cols;          /* COL: detector columns                                  */
+    int32_t n_rows;          /* ROW: detector rows; pixels DET = COL*ROW               */
+    int32_t reserved0;
+    double center_x, center_y;   /* rotation centre (Lx/2, Ly/2 in the reference)      */
+    double r_src;                /* source trajectory radius (Rsrc)                    */
+    double r_det;                /* detector-panel distance from the centre (Rdet)     */
+    double view_angle0;          /* phi_v = view_angle0 + v*view_angle_step            */
+    double view_angle_step;      /* reference: 2*pi/SRC (integrate_diffraction.cu:93)  */
+    double col_pitch, row_pitch; /* detector pixel pitches                             */
+    double det_z0;               /* height of detector row 0 above the source plane    */
+    double target_angle;         /* anode target angle, rad                            */
+    double focal_spot;           /* focal-spot size                                    */
+    double wedge_angle;          /* fan-beam wedge opening angle, rad (geomphi)        */
+    double voxel_size;           /* in-plane voxel side (geomdvox), 1 in the reference */
+} xrd_geometry;
+
+/* Energy / momentum-transfer discretisation and physics constants
+ * (Forward_XRD.cu:21-53, integrate_diffraction.cu:138-142, wvt_value.cu:8-23). */
+typedef struct xrd_discretization {
+    int32_t n_energy;        /* NRG: source and detector energy bins                    */
+    int32_t energy_halfwidth;/* DNRG: window is 2*DNRG+1 source bins (needs NRG > 2*DNRG) */
+    int32_t n_q;             /* NWT: reconstruction q-bins (parabolic axis)             */
+    int32_t n_qf;            /* NQF: fine uniform q-bins used for the counts            */
+    int32_t n_mat;           /* MAT: materials (segmentation ids 1..MAT, 0 = air)       */
+    int32_t lut_nq, lut_na;  /* NQ, NA: Compton S2 table size                           */
+    uint32_t flags;          /* XRD_QUIRK_* bits                                        */
+    double e_min;            /* left edge of energy bin 0, keV                          */
+    double e_width;          /* dE, keV                                                 */
+    double q_min, q_max;     /* q-axis range, 1/Angstrom                                */
+    double q_dq0;            /* first coarse bin width (dq0)                            */
+    double ray_step;         /* DQ: line-integral step, voxel units                     */
+    double hbarc;            /* keV*Angstrom                                            */
+    double mc2;              /* electron rest energy, keV                               */
+    double kn0, kn1, kn2;    /* 1/Klein-Nishina quadratic fit (Forward_XRD.cu:51-53)    */
+} xrd_discretization;
+
+/* Fill `g` with the reference's hard-coded scanner (integrate_diffraction.cu:81-128):
+ * mm_over_vox = nx/100, centre (nx,nx), Rsrc 150 mm, Rdet 170 mm, 0.5 mm columns,
+ * 1 mm rows, z0 10 mm, 30 deg target, 0.5 mm focal spot, 0.5 deg wedge.  Host-only. */
+int xrd_geometry_reference(int32_t nx, int32_t ny, int32_t n_views, int32_t n_cols,
+                           int32_t n_rows, xrd_geometry *g);
+
+/* Fill `d` with the reference's discretisation for the given sizes
+ * (Forward_XRD.cu:21-53; energy axis 8..80 keV in n_energy bins). Host-only. */
+int xrd_discretization_reference(int32_t n_energy, int32_t n_q, int32_t n_qf,
+                                 int32_t n_mat, xrd_discretization *d);
+
+/* Number of rows of the model matrix / length of the count vectors:
+ * n_views * n_cols * n_rows * n_energy.  Host-only. */
+int64_t xrd_num_rows(const xrd_geometry *g, const xrd_discretization *d);
+/* Number of columns of the model matrix: n_q * n_mat.  Host-only. */
+int64_t xrd_num_cols(const xrd_discretization *d);
+
+/* ---- context ---------------------------------------------------------------- */
+typedef struct xrd_context xrd_context;
+
+/* Create a context bound to HIP device `device` (makes it current). */
+int xrd_context_create(int device, xrd_context **out);
+int xrd_context_destroy(xrd_context *ctx);
+/* Bytes of device workspace currently held by the context. Host-only. */
+int64_t xrd_context_workspace_bytes(const xrd_context *ctx);
+
+/* Message for the last non-OK status returned on the calling thread. */
+const char *xrd_last_error_string(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int xrd_version(void);
+
+/* ---- forward projector -------------------------------------------------------
+ * Drop-in for Forward_XRD (Forward_XRD.cu:76-302): same inputs in the same order
+ * (prhs[0..9]) and the same outputs (plhs[0..2]) with the same layouts.
+ *
+ *  spectrum     prhs[0]  float [NRG x NRG]   spectrum[e_src + NRG*e_det]
+ *                        (integrate_diffraction.cu:61)
+ *  ifine        prhs[1]  float [NQF x MAT]   ifine[qf + NQF*mat] (integrate_diffraction.cu:52)
+ *  segmentation prhs[2]  int32 [nx x ny]     seg[ix + nx*iy], 1-based material, 0 = air;
+ *                        ids outside [0,MAT] are treated as air
+ *  a1, a2       prhs[3,4] float [nx x ny]    photoelectric / Compton coefficients in
+ *                        1/voxel (main.m:60-61)
+ *  s2_table     prhs[5]  float [NQ x NA]     s2[nq + NQ*na], values 0..255 (floored and
+ *                        clamped to uint8 on load, integrate_diffraction.cu:66-74)
+ *  lut_q_min .. lut_a_max  prhs[6..9]        log2 axes limits of the S2 table
+ *  out_counts   plhs[0]  float [SRC x COL x ROW x NRG]  XRD photon counts
+ *  out_compton  plhs[1]  float [SRC x COL x ROW x NRG]  Compton counts (x255/S2max units)
+ *  out_A        plhs[2]  float [NWT x MAT x SRC x COL x ROW x NRG] model matrix; may be
+ *                        NULL to skip it.
+ * Row index r = src + SRC*(col + COL*(row + ROW*e_det)); column j = nwt + NWT*mat.
+ * Requirements: 16-byte aligned pointers; n_energy > 2*energy_halfwidth.        */
+int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                const float *spectrum, const float *ifine, const int32_t *segmentation,
+                const float *a1, const float *a2, const float *s2_table,
+                double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
+                float *out_counts, float *out_compton, float *out_A, void *stream);
+
+/* ---- ray traversal (the GetIntersection + attenuator sampling, on its own) ----
+ * For `n` segments (x1,y1)->(x2,y2) given in `seg` as 4 doubles each, computes
+ * in FP64 the entry point into [0,nx]x[0,ny] with GetIntersection's rules
+ * (GetIntersection.cu:9-57), and the number of line-integral samples
+ * q = 0,1,.. with q*ray_step <= |P2 - entry| (attenuator.cu:44-69).
+ *  out_side  int32 [n]      side hit first (0: y=0, 1: x=nx, 2: y=ny, 3: x=0), -1 if none
+ *  out_entry double [2n]    entry point (NaN,NaN if none)
+ *  out_count int32 [n]      number of samples (0 if none)
+ *  out_texels (optional) int32 [2*n*max_samples]: (floor(x-0.5), floor(y-0.5)) of each
+ *            sample, -2^31 padding after the last one.  NULL to skip.
+ *  out_integrals (optional) float [n_basis*n]: Kahan-compensated FP32 line integrals
+ *            of the n_basis maps `maps` [n_basis x nx x ny] with exact bilinear weights
+ *            and zero border, times ray_step.  NULL to skip (then maps may be NULL). */
+int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step, int64_t n,
+                   const double *seg, int32_t *out_side, double *out_entry,
+                   int32_t *out_count, int32_t max_samples, int32_t *out_texels,
+                   int32_t n_basis, const float *maps, float *out_integrals,
+                   void *stream);
+
+/* ---- model-matrix operators (explicit A, row-major [rows x cols]) ------------
+ * A is the out_A of xrd_forward viewed as rows (src,col,row,e_det) x cols (nwt,mat),
+ * as the reference's MLEM views it (main.m:140). a_dtype selects the storage type. */
+#define XRD_DTYPE_F32  0
+#define XRD_DTYPE_BF16 1
+#define XRD_DTYPE_F16  2
+
+/* out[r] = sum_j A[r,j] x[j]            (the A*Irecon of main.m:162).  */
+int xrd_project(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows, int32_t n_cols,
+                const float *x, float *out, void *stream);
+/* out[j] = sum_r A[r,j] w[r]            (the AmatrixCPU'*(...) of main.m:162). FP64 sums. */
+int xrd_backproject(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                    int32_t n_cols, const float *w, double *out, void *stream);
+/* out[j] = sum_r A[r,j]                 (the sum(AmatrixCPU,1)' of main.m:162). FP64 sums. */
+int xrd_sensitivity(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                    int32_t n_cols, double *out, void *stream);
+
+/* Convert an FP32 matrix to BF16/F16 storage (round to nearest even). */
+int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, int out_dtype, void *out,
+                       void *stream);
+
+/* ---- MLEM (main.m:141-176) ----------------------------------------------------
+ * x <- x .* A'( y ./ (A x + c + 1e-20) ) ./ (sum(A,1)' + 1e-7); non-finite -> 0; x<0 -> 0.
+ * The plan keeps pointers to A, y, c and x (caller-owned, must outlive the plan),
+ * computes the sensitivity sum(A,1) once (FP64) at creation, and owns the
+ * per-workgroup partial buffers of the fused single-pass forward+adjoint kernel. */
+typedef struct xrd_mlem_plan xrd_mlem_plan;
+
+#define XRD_MLEM_FUSED    0   /* one pass over A per iteration (row dot, ratio, row axpy) */
+#define XRD_MLEM_UNFUSED  1   /* plain: A x, then A' r as two passes over A               */
+
+int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                         int32_t n_cols, const float *y, const float *c, float *x,
+                         int mode, void *stream, xrd_mlem_plan **out);
+/* Run n_iter updates of x in place.  use_graph != 0 replays a captured hipGraph of
+ * one iteration (captured on first use). */
+int xrd_mlem_plan_iterate(xrd_mlem_plan *plan, int32_t n_iter, int use_graph, void *stream);
+/* Poisson log-likelihood sum_r (y log f - f), f = A x + c + 1e-20, FP64, written to
+ * *out_host after synchronising `stream`. */
+int xrd_mlem_plan_loglik(xrd_mlem_plan *plan, double *out_host, void *stream);
+/* Device pointer to the FP64 sensitivity vector (n_cols) held by the plan. */
+const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *plan);
+int xrd_mlem_plan_destroy(xrd_mlem_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XRD_TOMO_H */

@@ -1,0 +1,213 @@
+"""ctypes wrapper of the FP64 oracle (oracle/libxrd_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product path (xrd_tomography_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from xrd_tomography_amd.descriptors import Discretization, Geometry
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libxrd_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, FP64, no contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "libxrd_oracle.so"],
+                       check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.xo_get_intersection.restype = ctypes.c_int
+        L.xo_get_intersection.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [_dp, _dp]
+        L.xo_trace_ray.restype = ctypes.c_int
+        L.xo_trace_ray.argtypes = ([ctypes.c_double] * 4 + [ctypes.c_int] * 2 +
+                                   [ctypes.c_double, _dp, _dp, _ip, ctypes.c_int32, _ip])
+        L.xo_line_integrals.restype = None
+        L.xo_line_integrals.argtypes = ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double] + [ctypes.c_double] * 4 + [_dp])
+        L.xo_bilinear.restype = ctypes.c_double
+        L.xo_bilinear.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double]
+        L.xo_find_src_nrg_range.restype = None
+        L.xo_find_src_nrg_range.argtypes = [ctypes.c_int] * 3 + [_ip, _ip]
+        L.xo_wvt_value.restype = None
+        L.xo_wvt_value.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp]
+        L.xo_scatter_geometry.restype = None
+        L.xo_scatter_geometry.argtypes = [ctypes.POINTER(Geometry)] + [ctypes.c_int] * 5 + [_dp]
+        L.xo_forward.restype = ctypes.c_int
+        L.xo_forward.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
+                                 _dp, _dp, _ip, _dp, _dp, _dp] + [ctypes.c_double] * 4 + \
+                                [_ip, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        for name in ("xo_project", "xo_backproject"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.xo_sensitivity.restype = None
+        L.xo_sensitivity.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int]
+        L.xo_mlem.restype = None
+        L.xo_mlem.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+                              ctypes.c_int]
+        L.xo_loglik.restype = ctypes.c_double
+        L.xo_loglik.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        L.xo_max_threads.restype = ctypes.c_int
+        L.xo_channel_match.restype = ctypes.c_double
+        L.xo_channel_match.argtypes = [ctypes.c_double] * 5
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def get_intersection(x1, y1, x2, y2, nx, ny):
+    rx, ry = ctypes.c_double(), ctypes.c_double()
+    side = lib().xo_get_intersection(x1, y1, x2, y2, nx, ny, ctypes.byref(rx), ctypes.byref(ry))
+    return side, rx.value, ry.value
+
+
+def trace_ray(x1, y1, x2, y2, nx, ny, step, max_samples=0):
+    ex, ey, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    tex = np.zeros(2 * max(max_samples, 1), dtype=np.int32)
+    side = lib().xo_trace_ray(x1, y1, x2, y2, nx, ny, step, ctypes.byref(ex), ctypes.byref(ey),
+                              ctypes.byref(n), max_samples,
+                              tex.ctypes.data_as(_ip) if max_samples > 0 else None)
+    return side, ex.value, ey.value, n.value, tex[:2 * max_samples].reshape(-1, 2)
+
+
+def line_integrals(maps, nx, ny, step, x1, y1, x2, y2):
+    """maps: [nb, nx*ny] (x fastest within a map)."""
+    m, mp = _d(np.asarray(maps).reshape(-1))
+    nb = m.size // (nx * ny)
+    out = np.zeros(nb)
+    lib().xo_line_integrals(mp, nb, nx, ny, step, x1, y1, x2, y2, out.ctypes.data_as(_dp))
+    return out
+
+
+def bilinear(map_, nx, ny, x, y):
+    m, mp = _d(np.asarray(map_).reshape(-1))
+    return lib().xo_bilinear(mp, nx, ny, x, y)
+
+
+def find_src_nrg_range(nrg, n_energy, halfwidth):
+    lo, hi = ctypes.c_int32(), ctypes.c_int32()
+    lib().xo_find_src_nrg_range(nrg, n_energy, halfwidth, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def wvt_value(n, n_q, q_min, q_max, dq0):
+    q, dq = ctypes.c_double(), ctypes.c_double()
+    lib().xo_wvt_value(n, n_q, q_min, q_max, dq0, ctypes.byref(q), ctypes.byref(dq))
+    return q.value, dq.value
+
+
+GEOM_FIELDS = ("src_x", "src_y", "det_x", "det_y", "cos_theta", "sin_half", "anorm2",
+               "bnorm2", "res2", "geom", "rayleigh")
+
+
+def scatter_geometry(g: Geometry, view, col, row, ix, iy):
+    out = np.zeros(len(GEOM_FIELDS))
+    lib().xo_scatter_geometry(ctypes.byref(g), view, col, row, ix, iy, out.ctypes.data_as(_dp))
+    return dict(zip(GEOM_FIELDS, out))
+
+
+def forward(prob, views=None, with_A=True, nthreads=0, disc=None):
+    """Oracle forward model of a synthetic.Problem for the listed views.
+    Returns (N, C, A) float64 with SRC replaced by len(views); A is [rows, cols]."""
+    g = prob.geometry
+    d = disc if disc is not None else prob.disc
+    vl = np.arange(g.n_views, dtype=np.int32) if views is None else np.asarray(views, np.int32)
+    nvl = len(vl)
+    rows = nvl * g.n_cols * g.n_rows * d.n_energy
+    N = np.zeros(rows)
+    C = np.zeros(rows)
+    A = np.zeros(rows * d.n_q * d.n_mat) if with_A else None
+    sp, spp = _d(prob.flat("spectrum"))
+    fi, fip = _d(prob.flat("ifine"))
+    sg, sgp = _i(prob.flat("segmentation"))
+    a1, a1p = _d(prob.flat("a1"))
+    a2, a2p = _d(prob.flat("a2"))
+    s2, s2p = _d(prob.flat("s2_table"))
+    vv, vvp = _i(vl)
+    rc = lib().xo_forward(ctypes.byref(g), ctypes.byref(d), spp, fip, sgp, a1p, a2p, s2p,
+                          *[float(v) for v in prob.lut], vvp, nvl, N.ctypes.data_as(_dp),
+                          C.ctypes.data_as(_dp),
+                          A.ctypes.data_as(_dp) if with_A else None, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"xo_forward failed: {rc}")
+    if with_A:
+        A = A.reshape(rows, d.n_q * d.n_mat)
+    return N, C, A
+
+
+def project(A, x, nthreads=0):
+    A, Ap = _d(A)
+    x, xp = _d(x)
+    out = np.zeros(A.shape[0])
+    lib().xo_project(Ap, A.shape[0], A.shape[1], xp, out.ctypes.data_as(_dp), nthreads)
+    return out
+
+
+def backproject(A, w, nthreads=0):
+    A, Ap = _d(A)
+    w, wp = _d(w)
+    out = np.zeros(A.shape[1])
+    lib().xo_backproject(Ap, A.shape[0], A.shape[1], wp, out.ctypes.data_as(_dp), nthreads)
+    return out
+
+
+def sensitivity(A, nthreads=0):
+    A, Ap = _d(A)
+    out = np.zeros(A.shape[1])
+    lib().xo_sensitivity(Ap, A.shape[0], A.shape[1], out.ctypes.data_as(_dp), nthreads)
+    return out
+
+
+def mlem(A, y, c, x0, n_iter, nthreads=0):
+    A, Ap = _d(A)
+    y, yp = _d(y)
+    c, cp = _d(c)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    lib().xo_mlem(Ap, A.shape[0], A.shape[1], yp, cp, x.ctypes.data_as(_dp), n_iter, nthreads)
+    return x
+
+
+def loglik(A, y, c, x, nthreads=0):
+    A, Ap = _d(A)
+    y, yp = _d(y)
+    c, cp = _d(c)
+    x, xp = _d(x)
+    return lib().xo_loglik(Ap, A.shape[0], A.shape[1], yp, cp, xp, nthreads)
+
+
+def channel_match(es1, es2, eratio, ed1, ed2):
+    return lib().xo_channel_match(es1, es2, eratio, ed1, ed2)
+
+
+def max_threads():
+    return lib().xo_max_threads()

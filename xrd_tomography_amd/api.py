@@ -1,0 +1,279 @@
+"""Host-side mirror of the reference's interface for the forward projector and MLEM.
+
+``Forward_XRD`` keeps the argument order and meaning of the MATLAB MEX gateway
+``[Nphotons, Cphotons, Amatrix] = Forward_XRD(spectrum, Ifine, segmentation,
+a1, a2, S2_grid, q_min, q_max, a_min, a_max)`` (Forward_XRD.cu:76-302,
+called at main.m:56-64) and returns the same three arrays in the same memory
+layout (first index fastest): torch tensors shaped in reverse index order,
+i.e. Nphotons as [NRG, ROW, COL, SRC] and Amatrix as
+[NRG, ROW, COL, SRC, MAT, NWT].  The geometry and discretisation that the
+reference hard-codes are passed as ``geometry`` / ``disc``.
+
+``mlem`` is the iterative solver of main.m:153-176.  torch is used for device
+memory and streams only; all arithmetic runs in libxrd_tomo.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import torch  # loaded before libxrd_tomo.so so both share one HIP runtime
+
+from . import _lib
+from ._lib import DTYPE_BF16, DTYPE_F16, DTYPE_F32, MLEM_FUSED, MLEM_UNFUSED, check, lib
+from .descriptors import Discretization, Geometry
+
+_DTYPES = {torch.float32: DTYPE_F32, torch.bfloat16: DTYPE_BF16, torch.float16: DTYPE_F16}
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _require_gpu(*ts):
+    for t in ts:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError("xrd_tomography_amd operates on GPU tensors only")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+class Context:
+    """Owns an xrd_context (device workspace).  One context per host thread."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("xrd_tomography_amd needs a ROCm GPU (torch.cuda unavailable)")
+        torch.cuda.init()
+        h = ctypes.c_void_p()
+        check(lib().xrd_context_create(device, ctypes.byref(h)), "xrd_context_create")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            torch.cuda.synchronize(self.device)
+            lib().xrd_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(lib().xrd_context_workspace_bytes(self.handle))
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def num_rows(g: Geometry, d: Discretization) -> int:
+    return int(lib().xrd_num_rows(ctypes.byref(g), ctypes.byref(d)))
+
+
+def Forward_XRD(spectrum: torch.Tensor, Ifine: torch.Tensor, segmentation: torch.Tensor,
+                a1: torch.Tensor, a2: torch.Tensor, S2_grid: torch.Tensor,
+                q_min: float, q_max: float, a_min: float, a_max: float, *,
+                geometry: Geometry, disc: Discretization, with_A: bool = True,
+                ctx: Optional[Context] = None, stream=None,
+                out: Optional[Tuple[torch.Tensor, torch.Tensor, Optional[torch.Tensor]]] = None):
+    """Forward projector (Forward_XRD.cu:76-302).  Inputs are flat device tensors in
+    the reference's layouts (spectrum [NRG*NRG] float32, Ifine [NQF*MAT] float32,
+    segmentation [NX*NY] int32, a1/a2 [NX*NY] float32 in 1/voxel, S2_grid [NQ*NA]).
+    Returns (Nphotons, Cphotons, Amatrix); Amatrix is None if with_A is False."""
+    g, d = geometry, disc
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid)
+    for t, dt in ((spectrum, torch.float32), (Ifine, torch.float32),
+                  (segmentation, torch.int32), (a1, torch.float32), (a2, torch.float32),
+                  (S2_grid, torch.float32)):
+        if t.dtype != dt:
+            raise TypeError(f"expected {dt}, got {t.dtype}")
+    shape_n = (d.n_energy, g.n_rows, g.n_cols, g.n_views)
+    if out is None:
+        dev = spectrum.device
+        N = torch.empty(shape_n, dtype=torch.float32, device=dev)
+        C = torch.empty(shape_n, dtype=torch.float32, device=dev)
+        A = torch.empty(shape_n + (d.n_mat, d.n_q), dtype=torch.float32, device=dev) if with_A else None
+    else:
+        N, C, A = out
+    rc = lib().xrd_forward(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                           _ptr(Ifine), _ptr(segmentation), _ptr(a1), _ptr(a2), _ptr(S2_grid),
+                           float(q_min), float(q_max), float(a_min), float(a_max),
+                           _ptr(N), _ptr(C), _ptr(A), _stream(stream))
+    check(rc, "xrd_forward")
+    return N, C, A
+
+
+def forward_problem(prob, device="cuda", with_A=True, ctx=None, disc=None):
+    """Run Forward_XRD on a synthetic.Problem; returns (N, C, A) device tensors."""
+    def dev(name, dt):
+        return torch.from_numpy(prob.flat(name)).to(device=device, dtype=dt)
+    return Forward_XRD(dev("spectrum", torch.float32), dev("ifine", torch.float32),
+                       dev("segmentation", torch.int32), dev("a1", torch.float32),
+                       dev("a2", torch.float32), dev("s2_table", torch.float32), *prob.lut,
+                       geometry=prob.geometry, disc=disc if disc is not None else prob.disc,
+                       with_A=with_A, ctx=ctx)
+
+
+def trace_rays(segments: torch.Tensor, nx: int, ny: int, step: float = 0.25,
+               max_samples: int = 0, maps: Optional[torch.Tensor] = None,
+               ctx: Optional[Context] = None, stream=None):
+    """GetIntersection + attenuator sampling for n segments [n, 4] float64 (x1,y1,x2,y2).
+    Returns (side int32 [n], entry float64 [n,2], count int32 [n], texels int32
+    [n, max_samples, 2] or None, integrals float32 [n_basis, n] or None)."""
+    ctx = ctx or default_context(segments.device.index or 0)
+    _require_gpu(segments, maps)
+    n = segments.shape[0]
+    dev = segments.device
+    side = torch.empty(n, dtype=torch.int32, device=dev)
+    entry = torch.empty((n, 2), dtype=torch.float64, device=dev)
+    count = torch.empty(n, dtype=torch.int32, device=dev)
+    tex = torch.empty((n, max_samples, 2), dtype=torch.int32, device=dev) if max_samples > 0 else None
+    nb = 0 if maps is None else maps.numel() // (nx * ny)
+    ints = torch.empty((nb, n), dtype=torch.float32, device=dev) if maps is not None else None
+    rc = lib().xrd_trace_rays(ctx.handle, nx, ny, float(step), n, _ptr(segments), _ptr(side),
+                              _ptr(entry), _ptr(count), max_samples, _ptr(tex), nb, _ptr(maps),
+                              _ptr(ints), _stream(stream))
+    check(rc, "xrd_trace_rays")
+    return side, entry, count, tex, ints
+
+
+def _mat(A: torch.Tensor):
+    _require_gpu(A)
+    if A.dim() != 2:
+        raise ValueError("A must be 2-D [rows, cols]")
+    if A.dtype not in _DTYPES:
+        raise TypeError(f"unsupported A dtype {A.dtype}")
+    return _DTYPES[A.dtype], A.shape[0], A.shape[1]
+
+
+def project(A: torch.Tensor, x: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
+    """A x (FP32 output)."""
+    ctx = ctx or default_context(A.device.index or 0)
+    dt, nr, nc = _mat(A)
+    out = torch.empty(nr, dtype=torch.float32, device=A.device)
+    check(lib().xrd_project(ctx.handle, _ptr(A), dt, nr, nc, _ptr(x), _ptr(out),
+                            _stream(stream)), "xrd_project")
+    return out
+
+
+def backproject(A: torch.Tensor, w: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
+    """A' w with FP64 column sums."""
+    ctx = ctx or default_context(A.device.index or 0)
+    dt, nr, nc = _mat(A)
+    out = torch.empty(nc, dtype=torch.float64, device=A.device)
+    check(lib().xrd_backproject(ctx.handle, _ptr(A), dt, nr, nc, _ptr(w), _ptr(out),
+                                _stream(stream)), "xrd_backproject")
+    return out
+
+
+def sensitivity(A: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
+    """sum(A, 1)' with FP64 column sums."""
+    ctx = ctx or default_context(A.device.index or 0)
+    dt, nr, nc = _mat(A)
+    out = torch.empty(nc, dtype=torch.float64, device=A.device)
+    check(lib().xrd_sensitivity(ctx.handle, _ptr(A), dt, nr, nc, _ptr(out), _stream(stream)),
+          "xrd_sensitivity")
+    return out
+
+
+def convert_matrix(A: torch.Tensor, dtype: torch.dtype, ctx=None, stream=None) -> torch.Tensor:
+    """FP32 -> BF16/F16 storage (round to nearest even) with the library's kernel."""
+    ctx = ctx or default_context(A.device.index or 0)
+    _require_gpu(A)
+    out = torch.empty(A.shape, dtype=dtype, device=A.device)
+    check(lib().xrd_convert_matrix(ctx.handle, _ptr(A), A.numel(), _DTYPES[dtype], _ptr(out),
+                                   _stream(stream)), "xrd_convert_matrix")
+    return out
+
+
+class MlemPlan:
+    """MLEM of main.m:153-176 on an explicit model matrix resident on the GPU.
+    x is updated in place; y, c and A must outlive the plan."""
+
+    def __init__(self, A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, x: torch.Tensor,
+                 fused: bool = True, ctx=None, stream=None):
+        self.ctx = ctx or default_context(A.device.index or 0)
+        dt, nr, nc = _mat(A)
+        _require_gpu(y, c, x)
+        if y.numel() != nr or c.numel() != nr or x.numel() != nc:
+            raise ValueError("y, c must have n_rows elements and x n_cols elements")
+        for t in (y, c, x):
+            if t.dtype != torch.float32:
+                raise TypeError("y, c and x must be float32")
+        self._keep = (A, y, c, x)
+        self.x = x
+        h = ctypes.c_void_p()
+        check(lib().xrd_mlem_plan_create(self.ctx.handle, _ptr(A), dt, nr, nc, _ptr(y), _ptr(c),
+                                         _ptr(x), MLEM_FUSED if fused else MLEM_UNFUSED,
+                                         _stream(stream), ctypes.byref(h)),
+              "xrd_mlem_plan_create")
+        self.handle = h
+        self.n_cols = nc
+
+    def iterate(self, n_iter: int, use_graph: bool = False, stream=None):
+        check(lib().xrd_mlem_plan_iterate(self.handle, int(n_iter), int(bool(use_graph)),
+                                          _stream(stream)), "xrd_mlem_plan_iterate")
+        return self.x
+
+    def loglik(self, stream=None) -> float:
+        v = ctypes.c_double()
+        check(lib().xrd_mlem_plan_loglik(self.handle, ctypes.byref(v), _stream(stream)),
+              "xrd_mlem_plan_loglik")
+        return v.value
+
+    def sensitivity(self) -> torch.Tensor:
+        """Copy of the FP64 sensitivity vector computed at plan creation."""
+        p = lib().xrd_mlem_plan_sensitivity(self.handle)
+        return _wrap_device_ptr(p, self.n_cols, torch.float64, self.x.device).clone()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().xrd_mlem_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _wrap_device_ptr(ptr: int, n: int, dtype, device):
+    """Non-owning torch view of n elements at a device pointer owned by the library."""
+    class _Holder:
+        pass
+    h = _Holder()
+    h.__cuda_array_interface__ = {
+        "shape": (n,), "typestr": "<f8" if dtype == torch.float64 else "<f4",
+        "data": (ptr, False), "version": 2,
+    }
+    return torch.as_tensor(h, device=device)
+
+
+def mlem(A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, n_iter: int, x0: float = 10.0,
+         fused: bool = True, use_graph: bool = False, ctx=None) -> torch.Tensor:
+    """n_iter MLEM updates from x = x0 (main.m:154) on device; returns x (float32)."""
+    x = torch.full((A.shape[1],), float(x0), dtype=torch.float32, device=A.device)
+    plan = MlemPlan(A, y, c, x, fused=fused, ctx=ctx)
+    plan.iterate(n_iter, use_graph=use_graph)
+    torch.cuda.current_stream().synchronize()
+    plan.close()
+    return x

@@ -1,0 +1,660 @@
+// forward.hip -- gfx950 kernels of the XRD forward projector.
+//
+// Replaces integrate_diffraction (integrate_diffraction.cu:9-409) and its
+// helpers GetIntersection, attenuator, find_src_nrg_range, wvt_value and the
+// Copy2Vector/texture set-up of Forward_XRD.cu:208-237.  The reference runs
+// one 256-thread block per (view, pixel, detector energy) and loops serially
+// over all voxels, recomputing both attenuation line integrals for every
+// energy block.  Here:
+//   * prepare_*      : quantise the Compton table, build a zero-bordered
+//                      interleaved (a1,a2) map, and a material-major list of
+//                      the non-air voxels in the reference's loop order;
+//   * source_legs    : one FP64-traced, Kahan-summed line integral per
+//                      (view, voxel) for the source leg (it does not depend on
+//                      the pixel or the energy);
+//   * forward_kernel : one workgroup per (view, pixel) that produces all NRG
+//                      detector-energy rows at once.  Per chunk of voxels it
+//                      traces the detector legs (split across the workgroup),
+//                      builds a per-(voxel, source energy) table in LDS (FP64
+//                      geometry and exponentials), then every lane accumulates
+//                      its (q-bin, energy) Gaussian terms in FP32 registers,
+//                      flushed per chunk into compensated totals.  The
+//                      detector-response window (find_src_nrg_range) is applied
+//                      once per material at the end, which removes the
+//                      reference's 11x recomputation of every Gaussian term.
+#include "xrd_device.hpp"
+#include "xrd_internal.hpp"
+
+namespace xrd {
+
+constexpr double kLog2e = 1.4426950408889634074;
+constexpr double kRs2pi = 0.39894228040143267794;  // 1/sqrt(2 pi)
+
+// ---------------------------------------------------------------------------
+// preparation
+
+__global__ void prepare_maps_kernel(PrepArgs a) {
+    const int pw = a.nx + 2, ph = a.ny + 2;
+    const int n = pw * ph;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int px = i % pw, py = i / pw;
+        float2 v = make_float2(0.f, 0.f);
+        if (px >= 1 && px <= a.nx && py >= 1 && py <= a.ny) {
+            const int pix = (px - 1) + a.nx * (py - 1);
+            v.x = a.a1[pix];
+            v.y = a.a2[pix];
+        }
+        a.mu_padded[i] = v;
+    }
+    // integrate_diffraction.cu:66-74: floor, clamp to [0,255], uint8
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.s2_count;
+         i += gridDim.x * blockDim.x) {
+        const float f = floorf(a.s2_table[i]);
+        int v = (f >= 0.f) ? (f > 255.f ? 255 : (int)f) : 0;
+        a.s2u8[i] = (uint8_t)v;
+    }
+}
+
+// Block-wide exclusive scan of 0/1 flags for a 1024-thread block.
+__device__ __forceinline__ int block_rank(bool flag, int *wsum, int &total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const unsigned long long m = __ballot(flag);
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; i++) {
+        const int s = wsum[i];
+        if (i < w) base += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return base + below;
+}
+
+// Non-air voxel list.  Pass 1 walks the reference's loop order (nx outer,
+// ny inner; integrate_diffraction.cu:173-178) to find, for the stale-voxel
+// quirk, the previous non-air voxel of each one.  Passes 2..M+1 emit the
+// voxels of each material in that same order.
+__global__ __launch_bounds__(1024) void prepare_list_kernel(PrepArgs a) {
+    __shared__ int wsum[16];
+    const int npix = a.nx * a.ny;
+    const bool stale = (a.flags & XRD_QUIRK_STALE_COMPTON_VOXEL) != 0;
+    int base = 0;
+    for (int c = 0; c < npix; c += blockDim.x) {
+        const int ro = c + threadIdx.x;
+        bool f = false;
+        int pix = 0;
+        if (ro < npix) {
+            const int ix = ro / a.ny, iy = ro % a.ny;
+            pix = ix + a.nx * iy;
+            const int s = a.seg[pix];
+            f = (s >= 1 && s <= a.n_mat);
+        }
+        int tot;
+        const int r = block_rank(f, wsum, tot);
+        if (f) a.ref_list[base + r] = pix;
+        base += tot;
+    }
+    __syncthreads();
+    const int nv = base;
+    for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+        const int pix = a.ref_list[j];
+        if (stale) {
+            const int prev = j > 0 ? a.ref_list[j - 1] : -1;
+            a.cpx1[pix] = prev >= 0 ? a.a1[prev] : 0.f;
+            a.cpx2[pix] = prev >= 0 ? a.a2[prev] : 0.f;
+        } else {
+            a.cpx1[pix] = a.a1[pix];
+            a.cpx2[pix] = a.a2[pix];
+        }
+    }
+    __syncthreads();
+    int off = 0;
+    for (int m = 1; m <= a.n_mat; m++) {
+        if (threadIdx.x == 0) a.mat_offsets[m - 1] = off;
+        for (int c = 0; c < npix; c += blockDim.x) {
+            const int ro = c + threadIdx.x;
+            bool f = false;
+            int pix = 0;
+            if (ro < npix) {
+                const int ix = ro / a.ny, iy = ro % a.ny;
+                pix = ix + a.nx * iy;
+                f = (a.seg[pix] == m);
+            }
+            int tot;
+            const int r = block_rank(f, wsum, tot);
+            if (f) {
+                VoxelEntry v;
+                v.pix = pix;
+                v.mat = m - 1;
+                v.a1c = a.cpx1[pix];
+                v.a2c = a.cpx2[pix];
+                a.voxels[off + r] = v;
+            }
+            off += tot;
+        }
+    }
+    if (threadIdx.x == 0) a.mat_offsets[a.n_mat] = off;
+}
+
+hipError_t launch_prepare(const PrepArgs &a, hipStream_t s) {
+    const int n = (a.nx + 2) * (a.ny + 2);
+    int blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(prepare_maps_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(prepare_list_kernel, dim3(1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// source legs: one thread per (view, voxel)
+
+__global__ __launch_bounds__(256) void source_legs_kernel(ForwardArgs a, float2 *src_int) {
+    const int npix = a.nx * a.ny;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    const int nv = a.mat_offsets[a.n_mat];
+    if (k >= nv) return;
+    const VoxelEntry ve = a.voxels[k];
+    const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
+    const ViewConst vc = a.views[v];
+    const Ray r = make_ray(vc.src_x, vc.src_y, (double)ix + 0.5, (double)iy + 0.5, a.nx, a.ny,
+                           a.ray_step);
+    Kahan s1, s2;
+    for (int q = 0; q < r.n; q++) {
+        double px, py;
+        ray_sample(r, a.ray_step, q, px, py);
+        const float2 t = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
+        s1.add(t.x);
+        s2.add(t.y);
+    }
+    src_int[(size_t)v * npix + k] =
+        make_float2((float)((double)s1.value() * a.ray_step), (float)((double)s2.value() * a.ray_step));
+}
+
+hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStream_t s) {
+    const int npix = a.nx * a.ny;
+    dim3 grid((npix + 255) / 256, a.n_views);
+    hipLaunchKernelGGL(source_legs_kernel, grid, dim3(256), 0, s, a, src_int);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// forward accumulation
+
+struct VoxShared {
+    double cth, s, As, Bs, gR, gp, a2c;
+    double lin0, lin1, lout0, lout1;
+    int na, pad;
+};
+
+// Per-(view, pixel, voxel) scattering geometry in FP64
+// (integrate_diffraction.cu:192-255), same operation order as the FP64 oracle.
+struct PixelGeom {
+    double sp, cp, sx, sy, dx, dy, dz, ndx, ndy, nsx, nsy, nsz;
+};
+
+__device__ __forceinline__ void voxel_geometry(const ForwardArgs &a, const PixelGeom &pg, int ix,
+                                               int iy, VoxShared &o) {
+#pragma clang fp contract(off)
+    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
+    const double ax = vx - pg.sx, ay = vy - pg.sy;
+    const double proj = (ax * pg.nsx + ay * pg.nsy) / sqrt(pg.nsx * pg.nsx + pg.nsy * pg.nsy);
+    const double vz = -proj * a.tan_w2;
+    const double hvox = proj * a.tan_w;
+    const double az = vz - 0.0;
+    const double bx = pg.dx - vx, by = pg.dy - vy, bz = pg.dz - vz;
+    const double ab = ax * bx + ay * by + az * bz;
+    const double an2 = ax * ax + ay * ay + az * az;
+    const double bn2 = bx * bx + by * by + bz * bz;
+    const double cth = ab * (1.0 / sqrt(an2)) * (1.0 / sqrt(bn2));
+    const double sth = sqrt(fabs(1.0 - cth) / 2.0);
+    const double Sx = bx - ab * ax / an2, Sy = by - ab * ay / an2, Sz = bz - ab * az / an2;
+    const double Dx = -ax + ab * bx / bn2, Dy = -ay + ab * by / bn2, Dz = -az + ab * bz / bn2;
+    const double Vx = -(Sx + Dx), Vy = -(Sy + Dy), Vz = -(Sz + Dz);
+    const double Dn = Dx * pg.ndx + Dy * pg.ndy + Dz * 0.0;
+    const double Sn = Sx * pg.nsx + Sy * pg.nsy + Sz * pg.nsz;
+    const double dD2 = (Dx * Dx + Dy * Dy + Dz * Dz - Dn * Dn) * a.adet / 12.0;
+    const double vV2 =
+        (a.voxel_size * a.voxel_size * (Vx * Vx + Vy * Vy) + hvox * hvox * Vz * Vz) / 12.0;
+    const double sS2 = (Sx * Sx + Sy * Sy + Sz * Sz - Sn * Sn) * a.focal_spot * a.focal_spot / 12.0;
+    const double rdotn = -pg.sp * ax + pg.cp * ay;
+    const double cospsi = rdotn * (1.0 / sqrt(ax * ax + ay * ay));
+    const double g_sv = cospsi * (1.0 / sqrt(an2)) * a.tan_w;
+    const double g_vd = -(pg.ndx * bx + pg.ndy * by) * (1.0 / sqrt(bn2 * bn2 * bn2));
+    const double res2 = sS2 + vV2 + dD2;
+    const double k1 = 2.0 * sth / a.hbarc;
+    o.cth = cth;
+    o.s = sth;
+    o.As = k1 * k1 * a.e_width * a.e_width / 12.0;
+    o.Bs = res2 / (2.0 * sth) / (2.0 * sth) / an2 / bn2;
+    o.gp = g_sv * g_vd;
+    o.gR = o.gp * ((1.0 + cth * cth) / 2.0);
+}
+
+__device__ __forceinline__ double f2_kn(const ForwardArgs &a, double e) {
+    return a.kn0 + a.kn1 * e + a.kn2 * e * e;
+}
+
+// Block of G*TQ lanes: lane (tq, h) owns coarse bin tq, fine bins tq*FS..tq*FS+FS-1
+// and the source energies e = h + G*i, i < EPT.
+template <int TQ, int G, int EPT, int FS, bool WITH_A>
+__global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) {
+    constexpr int NT = G * TQ;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int t = threadIdx.x;
+    const int tq = t % TQ, h = t / TQ;
+    const int V = a.n_views, E = a.n_energy, Q = a.n_q, QF = a.n_qf, M = a.n_mat;
+    const int v = blockIdx.x % V, p = blockIdx.x / V;
+    const int col = p % a.n_cols, row = p / a.n_cols;
+    const int npix = a.nx * a.ny;
+    const int nsplit = NT / kc;
+    const int Gc = NT / E;  // Compton lane groups
+
+    // ---- LDS carve-up
+    VoxShared *vox = reinterpret_cast<VoxShared *>(smem);
+    unsigned char *r2 = smem + ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
+    float4 *part = reinterpret_cast<float4 *>(r2);                 // [kc][nsplit]
+    float4 *tw = part + NT;                                          // [kc][E]
+    float *tb = reinterpret_cast<float *>(tw + kc * E);              // [kc][E] band
+    float *tk = tb + kc * E;                                         // [kc][E] Compton K
+    float *hst = reinterpret_cast<float *>(r2);                      // [E][Q] staging (aliases)
+    double *red = reinterpret_cast<double *>(r2);                    // final reductions (aliases)
+
+    // ---- per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64
+    PixelGeom pg;
+    {
+#pragma clang fp contract(off)
+        const ViewConst vc = a.views[v];
+        pg.sp = vc.sp;
+        pg.cp = vc.cp;
+        pg.sx = vc.src_x;
+        pg.sy = vc.src_y;
+        const double rdet = ((double)col - (double)a.n_cols / 2.0 + 0.5) * a.col_pitch;
+        pg.dx = a.center_x - a.r_det * vc.sp + rdet * vc.cp;
+        pg.dy = a.center_y + a.r_det * vc.cp + rdet * vc.sp;
+        pg.dz = a.det_z0 + (double)row * a.row_pitch;
+        pg.ndx = vc.sp;
+        pg.ndy = -vc.cp;
+        pg.nsx = -vc.sp * a.sta;
+        pg.nsy = vc.cp * a.sta;
+        pg.nsz = -a.cta;
+    }
+
+    // ---- per-lane q constants
+    float qch = 0.f, qcl = 0.f, dqc = 0.f;
+    if (WITH_A && tq < Q) {
+        const double q = a.q_coarse[tq], dq = a.q_coarse[Q + tq];
+        qch = (float)q;
+        qcl = (float)(q - (double)qch);
+        dqc = (float)dq;
+    } else {
+        qch = 1e30f;  // never in band; dqc = 0 keeps the term at zero
+    }
+    // fine centres q_min + dq_fine*(0.5 + n) (integrate_diffraction.cu:302);
+    // q_coarse holds [Q centres][Q widths][q_min].
+    float qfh[FS], qfl[FS];
+    float wlo_c = 1e30f, whi_c = -1e30f, wlo_f = 1e30f, whi_f = -1e30f;
+    {
+        const double qmin_axis = a.q_coarse[2 * Q];
+#pragma unroll
+        for (int s = 0; s < FS; s++) {
+            const int n = tq * FS + s;
+            qfh[s] = 1e30f;
+            qfl[s] = 0.f;
+            if (n < QF) {
+                const double q = qmin_axis + a.dq_fine * (0.5 + (double)n);
+                qfh[s] = (float)q;
+                qfl[s] = (float)(q - (double)qfh[s]);
+            }
+        }
+        float lo = 1e30f, hi = -1e30f;
+#pragma unroll
+        for (int s = 0; s < FS; s++)
+            if (tq * FS + s < QF) {
+                lo = fminf(lo, qfh[s]);
+                hi = fmaxf(hi, qfh[s]);
+            }
+        float clo = (WITH_A && tq < Q) ? qch : 1e30f, chi = (WITH_A && tq < Q) ? qch : -1e30f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, o, kWave));
+            hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
+            clo = fminf(clo, __shfl_xor(clo, o, kWave));
+            chi = fmaxf(chi, __shfl_xor(chi, o, kWave));
+        }
+        wlo_f = lo;
+        whi_f = hi;
+        wlo_c = clo;
+        whi_c = chi;
+    }
+
+    // ---- accumulators
+    // per-chunk partials (at most kc terms each), per-material coarse totals
+    // (FP32, one add per chunk) and FP64 fine totals
+    float pc[EPT], pf[EPT], tc[EPT];
+    double ft[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        pc[i] = 0.f;
+        pf[i] = 0.f;
+        tc[i] = 0.f;
+        ft[i] = 0.0;
+    }
+    double cacc = 0.0;
+    const int ced = t % E, cgrp = t / E;
+    const bool compton_lane = t < Gc * E;
+
+    const bool swap = (a.flags & XRD_QUIRK_SWAPPED_LEGS) != 0;
+
+    for (int m = 0; m < M; m++) {
+        const int lo = a.mat_offsets[m], hi = a.mat_offsets[m + 1];
+        float wf[FS];
+#pragma unroll
+        for (int s = 0; s < FS; s++) {
+            const int n = tq * FS + s;
+            wf[s] = (n < QF) ? (float)(a.dq_fine * (double)a.ifine[n + (size_t)QF * m]) : 0.f;
+        }
+        for (int k0 = lo; k0 < hi; k0 += kc) {
+            const int nk = min(kc, hi - k0);
+            // ---- phase 1: detector legs (split over nsplit lanes per voxel) + geometry
+            {
+                const int j = t % kc, r = t / kc;
+                if (j < nk) {
+                    const VoxelEntry ve = a.voxels[k0 + j];
+                    const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
+                    const Ray ray = make_ray(pg.dx, pg.dy, (double)ix + 0.5, (double)iy + 0.5,
+                                             a.nx, a.ny, a.ray_step);
+                    Kahan s1, s2;
+                    for (int q = r; q < ray.n; q += nsplit) {
+                        double px, py;
+                        ray_sample(ray, a.ray_step, q, px, py);
+                        const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
+                        s1.add(tv.x);
+                        s2.add(tv.y);
+                    }
+                    part[j * nsplit + r] = make_float4(s1.s, s1.c, s2.s, s2.c);
+                    if (r == 0) {
+                        VoxShared vs;
+                        voxel_geometry(a, pg, ix, iy, vs);
+                        vs.a2c = (double)ve.a2c;
+                        vs.na = lut_index((log2(fabs((double)ve.a1c / (double)ve.a2c)) -
+                                           a.lut_a_min) / a.da_lut,
+                                          a.lut_na);
+                        const float2 ls = a.src_int[(size_t)v * npix + (k0 + j)];
+                        vs.lin0 = ls.x;  // source leg; swapped below if requested
+                        vs.lin1 = ls.y;
+                        vox[j] = vs;
+                    }
+                }
+            }
+            __syncthreads();
+            if (t < nk) {
+                double d1 = 0.0, d2 = 0.0;
+                for (int r = 0; r < nsplit; r++) {
+                    const float4 pv = part[t * nsplit + r];
+                    d1 += (double)(pv.x - pv.y);
+                    d2 += (double)(pv.z - pv.w);
+                }
+                d1 *= a.ray_step;
+                d2 *= a.ray_step;
+                VoxShared &vs = vox[t];
+                const double s1 = vs.lin0, s2 = vs.lin1;
+                // integrate_diffraction.cu:259-260 passes the detector leg where the
+                // source leg belongs; reproduced only under XRD_QUIRK_SWAPPED_LEGS.
+                if (swap) {
+                    vs.lin0 = d1; vs.lin1 = d2; vs.lout0 = s1; vs.lout1 = s2;
+                } else {
+                    vs.lin0 = s1; vs.lin1 = s2; vs.lout0 = d1; vs.lout1 = d2;
+                }
+            }
+            __syncthreads();
+            // ---- phase 2: per-(voxel, source energy) table, FP64
+            for (int idx = t; idx < nk * E; idx += NT) {
+                const int j = idx / E, e = idx % E;
+                const VoxShared &vs = vox[j];
+                const EnergyConst ec = a.energies[e];
+                const double arg = (vs.lin0 + vs.lout0) * ec.inv_f1 + (vs.lin1 + vs.lout1) * ec.inv_f2;
+                const double pinpout = exp(-arg);
+                const double sig2 = vs.As + vs.Bs * ec.k2sq;
+                const double w = ec.wcoef * vs.s;
+                const float wh = (float)w;
+                const float wl = (float)(w - (double)wh);
+                const double c2 = -kLog2e / (2.0 * sig2);
+                const double P = pinpout * vs.gR * kRs2pi / sqrt(sig2);
+                tw[j * E + e] = make_float4(wh, wl, (float)c2, (float)P);
+                tb[j * E + e] = (float)sqrt(64.0 / -c2);
+                // Compton pieces (integrate_diffraction.cu:318-356) without RF and channel_match
+                const double ein = ec.ein;
+                const double er = 1.0 + ein * (1.0 - vs.cth);
+                const double eout = ein / er;
+                const double wc = ein * vs.s;
+                const double l2w = log2(wc * sqrt(er + wc * wc) / er);
+                const int nq = lut_index((l2w - a.lut_q_min) / a.dq_lut, a.lut_nq);
+                const double g1 = eout * eout * eout, g2 = f2_kn(a, eout);
+                const double pin = exp(-(vs.lin0 * ec.inv_f1 + vs.lin1 * ec.inv_f2));
+                const double pc_ = exp(-(vs.lout0 / g1 + vs.lout1 / g2));
+                const double cf = er * er * (er + 1.0 / er - 1.0 + vs.cth * vs.cth) / 2.0;
+                const double K = pin * pc_ * vs.gp * cf * vs.a2c *
+                                 (double)a.s2u8[nq + a.lut_nq * vs.na];
+                tk[j * E + e] = (float)K;
+            }
+            __syncthreads();
+            // ---- Compton accumulation over the detector-energy rows
+            if (compton_lane) {
+                int wlo, whi;
+                src_window(ced, E, a.dn, wlo, whi);
+                const double Ed1 = a.e_min + (double)ced * a.e_width;
+                const double Ed2 = a.e_min + (double)(ced + 1) * a.e_width;
+                for (int j = cgrp; j < nk; j += Gc) {
+                    const double cth = vox[j].cth;
+                    for (int e = wlo; e <= whi; e++) {
+                        const EnergyConst ec = a.energies[e];
+                        const double er = 1.0 + ec.ein * (1.0 - cth);
+                        const double Eo1 = ec.es1 / er, Eo2 = ec.es2 / er;
+                        const double lo_e = Eo1 > Ed1 ? Eo1 : Ed1;
+                        const double hi_e = Eo2 < Ed2 ? Eo2 : Ed2;
+                        double cm = hi_e - lo_e;
+                        if (!(cm > 0.0)) continue;
+                        cm = cm / (Eo2 - Eo1);
+                        cacc += cm * (double)a.spectrum[e + (size_t)E * ced] * (double)tk[j * E + e];
+                    }
+                }
+            }
+            // ---- phase 3: Gaussian resolution terms (integrate_diffraction.cu:298-315)
+            for (int j = 0; j < nk; j++) {
+#pragma unroll
+                for (int i = 0; i < EPT; i++) {
+                    const int e = h + G * i;
+                    if (e >= E) continue;  // wave-uniform
+                    const float4 tv = tw[j * E + e];
+                    const float band = tb[j * E + e];
+                    if (WITH_A && tv.x + band >= wlo_c && tv.x - band <= whi_c) {
+                        const float d = (tv.x - qch) + (tv.y - qcl);
+                        const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
+                        pc[i] = fmaf(tv.w * dqc, g, pc[i]);
+                    }
+                    if (tv.x + band >= wlo_f && tv.x - band <= whi_f) {
+                        float acc = 0.f;
+#pragma unroll
+                        for (int s = 0; s < FS; s++) {
+                            const float d = (tv.x - qfh[s]) + (tv.y - qfl[s]);
+                            acc = fmaf(wf[s], __builtin_amdgcn_exp2f(d * d * tv.z), acc);
+                        }
+                        pf[i] = fmaf(tv.w, acc, pf[i]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                if (WITH_A) tc[i] += pc[i];
+                ft[i] += (double)pf[i];
+                pc[i] = 0.f;
+                pf[i] = 0.f;
+            }
+            __syncthreads();
+        }
+        if (WITH_A) {
+            // ---- material flush: apply the detector-response window
+            // A[nwt, mat, e_det] = sum_{e in window(e_det)} spectrum[e, e_det] * H[e, nwt]
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = h + G * i;
+                if (e < E && tq < Q) hst[e * Q + tq] = tc[i];
+                tc[i] = 0.f;
+            }
+            __syncthreads();
+            for (int ed = h; ed < E; ed += G) {
+                if (tq < Q) {
+                    int wlo, whi;
+                    src_window(ed, E, a.dn, wlo, whi);
+                    double acc = 0.0;
+                    for (int e = wlo; e <= whi; e++)
+                        acc += (double)a.spectrum[e + (size_t)E * ed] * (double)hst[e * Q + tq];
+                    const size_t rowi = (size_t)v + (size_t)V * ((size_t)p + (size_t)a.n_det * ed);
+                    a.out_a[(size_t)tq + (size_t)Q * ((size_t)m + (size_t)M * rowi)] = (float)acc;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- counts: reduce fine totals over the q lanes of each energy group
+    const int lane = t & 63, wid = t >> 6;
+    const int nw = NT / kWave;  // waves; waves [h*TQ/64, (h+1)*TQ/64) hold group h
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const double sv = wave_sum(ft[i]);
+        if (lane == 0) red[wid * EPT + i] = sv;
+    }
+    __syncthreads();
+    double *hf = red + nw * EPT;  // [E]
+    double *cr = hf + E;          // [Gc][E]
+    if (t < E) {
+        const int hh = t % G, i = t / G;
+        double sacc = 0.0;
+        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++) sacc += red[w * EPT + i];
+        hf[t] = sacc;
+    }
+    if (compton_lane) cr[cgrp * E + ced] = cacc;
+    __syncthreads();
+    if (t < E) {
+        int wlo, whi;
+        src_window(t, E, a.dn, wlo, whi);
+        double nacc = 0.0;
+        for (int e = wlo; e <= whi; e++) nacc += (double)a.spectrum[e + (size_t)E * t] * hf[e];
+        double cs = 0.0;
+        for (int g = 0; g < Gc; g++) cs += cr[g * E + t];
+        const size_t rowi = (size_t)v + (size_t)V * ((size_t)p + (size_t)a.n_det * t);
+        a.out_n[rowi] = (float)nacc;
+        a.out_c[rowi] = (float)cs;
+    }
+}
+
+template <int TQ, int G, int EPT, int FS, bool WITH_A>
+static hipError_t launch_fwd_t(const ForwardArgs &a, hipStream_t s) {
+    constexpr int NT = G * TQ;
+    const int kc = 16;
+    const int E = a.n_energy;
+    const int Gc = NT / E;
+    const size_t vox_bytes = ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
+    size_t r2 = (size_t)NT * 16 + (size_t)kc * E * (16 + 4 + 4);
+    r2 = r2 > (size_t)E * a.n_q * 4 ? r2 : (size_t)E * a.n_q * 4;
+    const size_t redb = ((size_t)(NT / kWave) * EPT + E + (size_t)Gc * E) * 8;
+    r2 = r2 > redb ? r2 : redb;
+    const size_t shmem = vox_bytes + r2;
+    auto kern = forward_kernel<TQ, G, EPT, FS, WITH_A>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    const unsigned nblk = (unsigned)a.n_views * (unsigned)a.n_det;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(NT), shmem, s, a, kc);
+    return hipGetLastError();
+}
+
+template <bool WITH_A>
+static hipError_t dispatch_fwd(const ForwardArgs &a, hipStream_t s) {
+    const int E = a.n_energy, Q = a.n_q, QF = a.n_qf;
+    if (Q <= 128 && QF <= 384) {
+        if (E <= 16) return launch_fwd_t<128, 4, 4, 3, WITH_A>(a, s);
+        if (E <= 64) return launch_fwd_t<128, 4, 16, 3, WITH_A>(a, s);
+    }
+    if (Q <= 256 && QF <= 768 && E <= 64) return launch_fwd_t<256, 4, 16, 3, WITH_A>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_forward(const ForwardArgs &a, hipStream_t s) {
+    return a.out_a ? dispatch_fwd<true>(a, s) : dispatch_fwd<false>(a, s);
+}
+
+// ---------------------------------------------------------------------------
+// ray traversal on its own (for the intersection-list contract)
+
+__device__ __forceinline__ float texel_planar(const float *map, int nx, int ny, int i, int j) {
+    return (i < 0 || j < 0 || i >= nx || j >= ny) ? 0.f : map[i + nx * j];
+}
+
+__global__ void trace_rays_kernel(int nx, int ny, double step, int64_t n, const double *seg,
+                                  int32_t *side, double *entry, int32_t *count,
+                                  int32_t max_samples, int32_t *texels, int32_t n_basis,
+                                  const float *maps, float *ints) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Ray r = make_ray(seg[4 * i], seg[4 * i + 1], seg[4 * i + 2], seg[4 * i + 3], nx, ny,
+                           step);
+    side[i] = r.side;
+    entry[2 * i] = r.side >= 0 ? r.ex : __longlong_as_double(0x7ff8000000000000ll);
+    entry[2 * i + 1] = r.side >= 0 ? r.ey : __longlong_as_double(0x7ff8000000000000ll);
+    count[i] = r.n;
+    if (texels) {
+        for (int q = 0; q < max_samples; q++) {
+            int32_t tx = INT32_MIN, ty = INT32_MIN;
+            if (q < r.n) {
+                double px, py;
+                ray_sample(r, step, q, px, py);
+                tx = (int32_t)floor(px - 0.5);
+                ty = (int32_t)floor(py - 0.5);
+            }
+            texels[2 * (i * max_samples + q)] = tx;
+            texels[2 * (i * max_samples + q) + 1] = ty;
+        }
+    }
+    if (ints) {
+        const size_t plane = (size_t)nx * ny;
+        for (int b = 0; b < n_basis; b++) {
+            const float *mp = maps + b * plane;
+            Kahan acc;
+            for (int q = 0; q < r.n; q++) {
+                double px, py;
+                ray_sample(r, step, q, px, py);
+                const double xt = px - 0.5, yt = py - 0.5;
+                const double fx = floor(xt), fy = floor(yt);
+                const float ax = (float)(xt - fx), ay = (float)(yt - fy);
+                const int i0 = (int)fx, j0 = (int)fy;
+                const float bx = 1.f - ax, by = 1.f - ay;
+                const float val = by * (bx * texel_planar(mp, nx, ny, i0, j0) +
+                                        ax * texel_planar(mp, nx, ny, i0 + 1, j0)) +
+                                  ay * (bx * texel_planar(mp, nx, ny, i0, j0 + 1) +
+                                        ax * texel_planar(mp, nx, ny, i0 + 1, j0 + 1));
+                acc.add(val);
+            }
+            ints[(size_t)b * n + i] = (float)((double)acc.value() * step);
+        }
+    }
+}
+
+hipError_t launch_trace_rays(int nx, int ny, double step, int64_t n, const double *seg,
+                             int32_t *side, double *entry, int32_t *count, int32_t max_samples,
+                             int32_t *texels, int32_t n_basis, const float *maps, float *ints,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    hipLaunchKernelGGL(trace_rays_kernel, dim3(blocks), dim3(128), 0, s, nx, ny, step, n, seg,
+                       side, entry, count, max_samples, texels, n_basis, maps, ints);
+    return hipGetLastError();
+}
+
+}  // namespace xrd

@@ -1,0 +1,454 @@
+// mlem.hip -- explicit model-matrix operators and the MLEM update on gfx950.
+//
+// Replaces the MATLAB loop of main.m:159-176:
+//   x <- x .* A'( y ./ (A x + c + 1e-20) ) ./ (sum(A,1)' + 1e-7)
+//   x(~isfinite(x)) = 0; x(x<0) = 0
+// A is row-major [rows x cols] (the reference's Amatrix viewed as in
+// main.m:140).  The iteration is HBM-bound (2 flop per A element read), so the
+// fused kernel streams each row of A exactly once: one wave64 owns a row at a
+// time, lanes hold 4 consecutive columns per 256-column group, the row dot
+// product is a wave reduction, the ratio y/(Ax+c) is formed in FP64 and the
+// row is accumulated into per-lane FP64 column sums.  Per-workgroup FP64
+// partials are reduced in a fixed order by the update kernel, so every result
+// is deterministic.
+#include <hip/hip_fp16.h>
+
+#include "xrd_device.hpp"
+#include "xrd_internal.hpp"
+
+namespace xrd {
+
+constexpr int kMatThreads = 256;   // 4 waves per workgroup
+constexpr int kMatWaves = kMatThreads / kWave;
+
+// 4 consecutive elements of A as floats.
+template <typename T> struct Quad;
+template <> struct Quad<float> {
+    static __device__ __forceinline__ float4 load(const float *p) {
+        return *reinterpret_cast<const float4 *>(p);
+    }
+};
+struct bf16x4 { uint32_t lo, hi; };
+struct f16x4 { uint32_t lo, hi; };
+template <> struct Quad<bf16x4> {
+    static __device__ __forceinline__ float4 load(const bf16x4 *p) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(p);
+        return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                           __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+    }
+};
+template <> struct Quad<f16x4> {
+    static __device__ __forceinline__ float4 load(const f16x4 *p) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(p);
+        const __half2 a = *reinterpret_cast<const __half2 *>(&u.x);
+        const __half2 b = *reinterpret_cast<const __half2 *>(&u.y);
+        const float2 fa = __half22float2(a), fb = __half22float2(b);
+        return make_float4(fa.x, fa.y, fb.x, fb.y);
+    }
+};
+
+// element pointer of column group (row, quad index)
+template <typename T> __device__ __forceinline__ const T *quad_ptr(const void *A, int64_t r,
+                                                                    int n_cols, int qi);
+template <> __device__ __forceinline__ const float *quad_ptr<float>(const void *A, int64_t r,
+                                                                    int n_cols, int qi) {
+    return reinterpret_cast<const float *>(A) + r * n_cols + 4 * qi;
+}
+template <> __device__ __forceinline__ const bf16x4 *quad_ptr<bf16x4>(const void *A, int64_t r,
+                                                                      int n_cols, int qi) {
+    return reinterpret_cast<const bf16x4 *>(reinterpret_cast<const uint16_t *>(A) + r * n_cols +
+                                            4 * qi);
+}
+template <> __device__ __forceinline__ const f16x4 *quad_ptr<f16x4>(const void *A, int64_t r,
+                                                                    int n_cols, int qi) {
+    return reinterpret_cast<const f16x4 *>(reinterpret_cast<const uint16_t *>(A) + r * n_cols +
+                                           4 * qi);
+}
+
+enum { kColsOnes = 0, kColsWeighted = 1, kColsMlem = 2 };
+
+// Workgroup b handles rows [b*rpb, min((b+1)*rpb, n_rows)).  Each step the 4
+// waves take R consecutive rows each.  Lane l holds columns 4*(l + 64 g) + i.
+template <typename T, int NG, int R, int MODE>
+__global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int64_t n_rows,
+                                                             int n_cols, int64_t rpb,
+                                                             const float *x, const float *y,
+                                                             const float *c, const float *w,
+                                                             double *partial) {
+    __shared__ double red[kMatWaves][NG * 256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    int64_t r1 = r0 + rpb;
+    if (r1 > n_rows) r1 = n_rows;
+    const int nq = n_cols >> 2;
+
+    float4 xv[NG];
+    bool gv[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int qi = lane + 64 * g;
+        gv[g] = qi < nq;
+        xv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MODE == kColsMlem && gv[g]) xv[g] = *reinterpret_cast<const float4 *>(x + 4 * qi);
+    }
+    double acc[NG][4];
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[g][i] = 0.0;
+
+    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
+        float4 av[R][NG];
+#pragma unroll
+        for (int k = 0; k < R; k++)
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                const int64_t r = base + k;
+                av[k][g] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (r < r1 && gv[g]) av[k][g] = Quad<T>::load(quad_ptr<T>(A, r, n_cols, lane + 64 * g));
+            }
+        double ratio[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t r = base + k;
+            if (MODE == kColsMlem) {
+                float d = 0.f;
+#pragma unroll
+                for (int g = 0; g < NG; g++) {
+                    d = fmaf(av[k][g].x, xv[g].x, d);
+                    d = fmaf(av[k][g].y, xv[g].y, d);
+                    d = fmaf(av[k][g].z, xv[g].z, d);
+                    d = fmaf(av[k][g].w, xv[g].w, d);
+                }
+                d = wave_sum(d);
+                double rt = 0.0;
+                if (r < r1) {
+                    const double cr = (double)c[r];
+                    rt = ((double)y[r] - 0.0 * cr) / ((double)d + cr + 1e-20);
+                }
+                ratio[k] = rt;
+            } else if (MODE == kColsWeighted) {
+                ratio[k] = (r < r1) ? (double)w[r] : 0.0;
+            } else {
+                ratio[k] = (r < r1) ? 1.0 : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++)
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                acc[g][0] = fma((double)av[k][g].x, ratio[k], acc[g][0]);
+                acc[g][1] = fma((double)av[k][g].y, ratio[k], acc[g][1]);
+                acc[g][2] = fma((double)av[k][g].z, ratio[k], acc[g][2]);
+                acc[g][3] = fma((double)av[k][g].w, ratio[k], acc[g][3]);
+            }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) red[wid][4 * (lane + 64 * g) + i] = acc[g][i];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n_cols; j += kMatThreads) {
+        double s = red[0][j];
+#pragma unroll
+        for (int q = 1; q < kMatWaves; q++) s += red[q][j];
+        partial[(size_t)blockIdx.x * n_cols + j] = s;
+    }
+}
+
+// out[r] = A_r . x  (FP32, one wave per row group)
+template <typename T, int NG, int R>
+__global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int64_t n_rows,
+                                                              int n_cols, int64_t rpb,
+                                                              const float *x, float *out) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    int64_t r1 = r0 + rpb;
+    if (r1 > n_rows) r1 = n_rows;
+    const int nq = n_cols >> 2;
+    float4 xv[NG];
+    bool gv[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int qi = lane + 64 * g;
+        gv[g] = qi < nq;
+        xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
+        float4 av[R][NG];
+#pragma unroll
+        for (int k = 0; k < R; k++)
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                const int64_t r = base + k;
+                av[k][g] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (r < r1 && gv[g]) av[k][g] = Quad<T>::load(quad_ptr<T>(A, r, n_cols, lane + 64 * g));
+            }
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            float d = 0.f;
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                d = fmaf(av[k][g].x, xv[g].x, d);
+                d = fmaf(av[k][g].y, xv[g].y, d);
+                d = fmaf(av[k][g].z, xv[g].z, d);
+                d = fmaf(av[k][g].w, xv[g].w, d);
+            }
+            d = wave_sum(d);
+            const int64_t r = base + k;
+            if (lane == 0 && r < r1) out[r] = d;
+        }
+    }
+}
+
+// Sum per-block partials in a fixed order: 16 waves each take every 16th
+// block for 64 columns; the 16 wave sums are then added in wave order.
+__global__ __launch_bounds__(1024) void reduce_partials_kernel(const double *partial,
+                                                               int nblocks, int n_cols,
+                                                               double *out, const double *sens,
+                                                               float *x) {
+    __shared__ double red[16][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (j < n_cols)
+        for (int b = wid; b < nblocks; b += 16) s += partial[(size_t)b * n_cols + j];
+    red[wid][lane] = s;
+    __syncthreads();
+    if (wid == 0 && j < n_cols) {
+        double t = red[0][lane];
+        for (int q = 1; q < 16; q++) t += red[q][lane];
+        if (x) {
+            // main.m:162-163,176
+            double v = (double)x[j] * t / (sens[j] + 1e-7);
+            float f = (float)v;
+            if (!isfinite(f)) f = 0.f;
+            if (f < 0.f) f = 0.f;
+            x[j] = f;
+        } else {
+            out[j] = t;
+        }
+    }
+}
+
+__global__ void ratio_kernel(int64_t n, const float *f, const float *y, const float *c,
+                             float *r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double cr = (double)c[i];
+        r[i] = (float)(((double)y[i] - 0.0 * cr) / ((double)f[i] + cr + 1e-20));
+    }
+}
+
+template <typename T, int NG, int R>
+__global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int64_t n_rows,
+                                                             int n_cols, int64_t rpb,
+                                                             const float *x, const float *y,
+                                                             const float *c, double *partial) {
+    __shared__ double red[kMatWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    int64_t r1 = r0 + rpb;
+    if (r1 > n_rows) r1 = n_rows;
+    const int nq = n_cols >> 2;
+    float4 xv[NG];
+    bool gv[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int qi = lane + 64 * g;
+        gv[g] = qi < nq;
+        xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    double ll = 0.0;
+    for (int64_t r = r0 + wid; r < r1; r += kMatWaves) {
+        float d = 0.f;
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            if (!gv[g]) continue;
+            const float4 av = Quad<T>::load(quad_ptr<T>(A, r, n_cols, lane + 64 * g));
+            d = fmaf(av.x, xv[g].x, d);
+            d = fmaf(av.y, xv[g].y, d);
+            d = fmaf(av.z, xv[g].z, d);
+            d = fmaf(av.w, xv[g].w, d);
+        }
+        d = wave_sum(d);
+        const double f = (double)d + (double)c[r] + 1e-20;
+        const double yr = (double)y[r];
+        ll += (yr > 0.0 ? yr * log(f) : 0.0) - f;
+    }
+    if (lane == 0) red[wid] = ll;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < kMatWaves; q++) s += red[q];
+        partial[blockIdx.x] = s;
+    }
+}
+
+__global__ void convert_kernel(const float *A, int64_t n, int dtype, void *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = A[i];
+        if (dtype == XRD_DTYPE_BF16) {
+            uint32_t u = __float_as_uint(v);
+            uint16_t b;
+            if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) {
+                b = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+            } else {
+                u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+                b = (uint16_t)(u >> 16);
+            }
+            reinterpret_cast<uint16_t *>(out)[i] = b;
+        } else {
+            reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+static int g_num_cus = 0;
+static int num_cus() {
+    if (g_num_cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            n > 0)
+            g_num_cus = n;
+        else
+            g_num_cus = 256;
+    }
+    return g_num_cus;
+}
+
+int mat_num_blocks(const MatArgs &m) {
+    // two resident workgroups per CU, never more blocks than 16-row groups
+    int64_t nb = (int64_t)num_cus() * 2;
+    const int64_t cap = (m.n_rows + 15) / 16;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb - 1) / nb; }
+
+#define XRD_DISPATCH_NG(NGV, ...)                                  \
+    do {                                                           \
+        const int ngr = (m.n_cols + 255) / 256;                    \
+        if (ngr <= 1) { constexpr int NGV = 1; __VA_ARGS__; }      \
+        else if (ngr <= 2) { constexpr int NGV = 2; __VA_ARGS__; } \
+        else if (ngr <= 4) { constexpr int NGV = 4; __VA_ARGS__; } \
+        else if (ngr <= 8) { constexpr int NGV = 8; __VA_ARGS__; } \
+        else return hipErrorInvalidValue;                          \
+    } while (0)
+
+template <typename T, int MODE>
+static hipError_t colsum_t(const MatArgs &m, const float *x, const float *y, const float *c,
+                           const float *w, double *partial, hipStream_t s) {
+    const int nb = mat_num_blocks(m);
+    const int64_t rpb = rows_per_block(m, nb);
+    XRD_DISPATCH_NG(NG, {
+        constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
+        hipLaunchKernelGGL((colsum_kernel<T, NG, R, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, partial);
+    });
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t colsum_dtype(const MatArgs &m, const float *x, const float *y, const float *c,
+                               const float *w, double *partial, hipStream_t s) {
+    switch (m.dtype) {
+        case XRD_DTYPE_F32: return colsum_t<float, MODE>(m, x, y, c, w, partial, s);
+        case XRD_DTYPE_BF16: return colsum_t<bf16x4, MODE>(m, x, y, c, w, partial, s);
+        case XRD_DTYPE_F16: return colsum_t<f16x4, MODE>(m, x, y, c, w, partial, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_colsum_partials(const MatArgs &m, const float *w, double *partial,
+                                  hipStream_t s) {
+    if (w) return colsum_dtype<kColsWeighted>(m, nullptr, nullptr, nullptr, w, partial, s);
+    return colsum_dtype<kColsOnes>(m, nullptr, nullptr, nullptr, nullptr, partial, s);
+}
+
+hipError_t launch_mlem_fused(const MatArgs &m, const float *x, const float *y, const float *c,
+                             double *partial, hipStream_t s) {
+    return colsum_dtype<kColsMlem>(m, x, y, c, nullptr, partial, s);
+}
+
+template <typename T>
+static hipError_t project_t(const MatArgs &m, const float *x, float *out, hipStream_t s) {
+    const int nb = mat_num_blocks(m);
+    const int64_t rpb = rows_per_block(m, nb);
+    XRD_DISPATCH_NG(NG, {
+        constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
+        hipLaunchKernelGGL((project_kernel<T, NG, R>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
+                           m.n_rows, m.n_cols, rpb, x, out);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_project(const MatArgs &m, const float *x, float *out, hipStream_t s) {
+    switch (m.dtype) {
+        case XRD_DTYPE_F32: return project_t<float>(m, x, out, s);
+        case XRD_DTYPE_BF16: return project_t<bf16x4>(m, x, out, s);
+        case XRD_DTYPE_F16: return project_t<f16x4>(m, x, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t loglik_t(const MatArgs &m, const float *x, const float *y, const float *c,
+                           double *partial, hipStream_t s) {
+    const int nb = mat_num_blocks(m);
+    const int64_t rpb = rows_per_block(m, nb);
+    XRD_DISPATCH_NG(NG, {
+        hipLaunchKernelGGL((loglik_kernel<T, NG, 1>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
+                           m.n_rows, m.n_cols, rpb, x, y, c, partial);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float *y,
+                                  const float *c, double *partial, hipStream_t s) {
+    switch (m.dtype) {
+        case XRD_DTYPE_F32: return loglik_t<float>(m, x, y, c, partial, s);
+        case XRD_DTYPE_BF16: return loglik_t<bf16x4>(m, x, y, c, partial, s);
+        case XRD_DTYPE_F16: return loglik_t<f16x4>(m, x, y, c, partial, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_reduce_partials(const double *partial, int nblocks, int n_cols, double *out,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((n_cols + 63) / 64), dim3(1024), 0, s,
+                       partial, nblocks, n_cols, out, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlem_update(const double *partial, int nblocks, int n_cols,
+                              const double *sens, float *x, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((n_cols + 63) / 64), dim3(1024), 0, s,
+                       partial, nblocks, n_cols, nullptr, sens, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_ratio(int64_t n, const float *f, const float *y, const float *c, float *r,
+                        hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ratio_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, f,
+                       y, c, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const float *A, int64_t n, int dtype, void *out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(convert_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, n, dtype, out);
+    return hipGetLastError();
+}
+
+}  // namespace xrd

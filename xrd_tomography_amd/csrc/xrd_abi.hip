@@ -1,0 +1,655 @@
+// xrd_abi.hip -- the C ABI of include/xrd_tomo.h: argument validation,
+// workspace management and kernel dispatch.
+//
+// Mirrors the MEX gateway Forward_XRD (Forward_XRD.cu:76-302) with caller-owned
+// device buffers, explicit dimensions, a stream and status codes instead of
+// mxGPUArray copies, hidden allocations and no error checking.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/xrd_tomo.h"
+#include "xrd_internal.hpp"
+
+using namespace xrd;
+
+namespace {
+
+thread_local char g_err[512] = "no error";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(XRD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                     \
+    do {                                        \
+        hipError_t e_ = (expr);                 \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+bool aligned(const void *p, size_t a) { return ((uintptr_t)p % a) == 0; }
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace
+
+struct xrd_context {
+    int device = 0;
+    // forward workspace
+    DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables;
+    // explicit-matrix partials
+    DevBuf partial, small;
+    // pinned staging for host-computed per-view / per-energy tables
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
+};
+
+struct xrd_mlem_plan {
+    xrd_context *ctx = nullptr;
+    MatArgs m{};
+    const float *y = nullptr, *c = nullptr;
+    float *x = nullptr;
+    int mode = XRD_MLEM_FUSED;
+    int nblocks = 0;
+    DevBuf sens, partial, f, r, llp;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+extern "C" {
+
+const char *xrd_last_error_string(void) { return g_err; }
+
+int xrd_version(void) { return 100; }
+
+int xrd_geometry_reference(int32_t nx, int32_t ny, int32_t n_views, int32_t n_cols,
+                           int32_t n_rows, xrd_geometry *g) {
+    if (!g) return fail(XRD_ERR_NULL_POINTER, "xrd_geometry_reference: g is NULL");
+    if (nx <= 0 || ny <= 0 || n_views <= 0 || n_cols <= 0 || n_rows <= 0)
+        return fail(XRD_ERR_DIMS, "xrd_geometry_reference: non-positive size");
+    // integrate_diffraction.cu:81-128
+    const double mmv = (double)nx / 100.0;
+    memset(g, 0, sizeof(*g));
+    g->nx = nx;
+    g->ny = ny;
+    g->n_views = n_views;
+    g->n_cols = n_cols;
+    g->n_rows = n_rows;
+    g->center_x = 100.0 * mmv;
+    g->center_y = 100.0 * mmv;
+    g->r_src = 150.0 * mmv;
+    g->r_det = 170.0 * mmv;
+    g->view_angle0 = 0.0;
+    g->view_angle_step = 2.0 * M_PI / n_views;
+    g->col_pitch = 0.5 * mmv;
+    g->row_pitch = 1.0 * mmv;
+    g->det_z0 = 10.0 * mmv;
+    g->target_angle = M_PI / 6.0;
+    g->focal_spot = 0.5 * mmv;
+    g->wedge_angle = 0.5 / 180.0 * M_PI;
+    g->voxel_size = 1.0;
+    return XRD_OK;
+}
+
+int xrd_discretization_reference(int32_t n_energy, int32_t n_q, int32_t n_qf, int32_t n_mat,
+                                 xrd_discretization *d) {
+    if (!d) return fail(XRD_ERR_NULL_POINTER, "xrd_discretization_reference: d is NULL");
+    if (n_energy <= 10 || n_q <= 0 || n_qf <= 0 || n_mat <= 0)
+        return fail(XRD_ERR_DIMS, "xrd_discretization_reference: bad size");
+    // Forward_XRD.cu:21-53, integrate_diffraction.cu:138-140
+    memset(d, 0, sizeof(*d));
+    d->n_energy = n_energy;
+    d->energy_halfwidth = 5;
+    d->n_q = n_q;
+    d->n_qf = n_qf;
+    d->n_mat = n_mat;
+    d->lut_nq = 64;
+    d->lut_na = 64;
+    d->flags = 0;
+    d->e_min = 8.0;
+    d->e_width = 72.0 / n_energy;
+    d->q_min = 0.5;
+    d->q_max = 6.0;
+    d->q_dq0 = 0.01;
+    d->ray_step = 0.25;
+    d->hbarc = 1.973269804;
+    d->mc2 = 510.99895;
+    d->kn0 = 0.751691286166726;
+    d->kn1 = 1.4599683207703;
+    d->kn2 = -0.64597532690936;
+    return XRD_OK;
+}
+
+int64_t xrd_num_rows(const xrd_geometry *g, const xrd_discretization *d) {
+    if (!g || !d) return -1;
+    return (int64_t)g->n_views * g->n_cols * g->n_rows * d->n_energy;
+}
+
+int64_t xrd_num_cols(const xrd_discretization *d) {
+    if (!d) return -1;
+    return (int64_t)d->n_q * d->n_mat;
+}
+
+int xrd_context_create(int device, xrd_context **out) {
+    if (!out) return fail(XRD_ERR_NULL_POINTER, "xrd_context_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+        return fail(XRD_ERR_INVALID_ARG, "xrd_context_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    xrd_context *c = new xrd_context();
+    c->device = device;
+    hipError_t e = hipEventCreateWithFlags(&c->staged, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipEventCreate");
+    }
+    *out = c;
+    return XRD_OK;
+}
+
+int xrd_context_destroy(xrd_context *c) {
+    if (!c) return XRD_OK;
+    hipSetDevice(c->device);
+    if (c->staged_pending) hipEventSynchronize(c->staged);
+    for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
+                      &c->src_int, &c->tables, &c->partial, &c->small})
+        b->release();
+    if (c->pinned) hipHostFree(c->pinned);
+    if (c->staged) hipEventDestroy(c->staged);
+    delete c;
+    return XRD_OK;
+}
+
+int64_t xrd_context_workspace_bytes(const xrd_context *c) {
+    if (!c) return 0;
+    int64_t s = 0;
+    for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
+                            &c->offsets, &c->src_int, &c->tables, &c->partial, &c->small})
+        s += (int64_t)b->bytes;
+    return s;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_geometry(const xrd_geometry *g, const xrd_discretization *d, const char *fn) {
+    if (!g || !d) return fail(XRD_ERR_NULL_POINTER, "%s: geometry/discretization is NULL", fn);
+    if (g->nx <= 0 || g->ny <= 0 || g->n_views <= 0 || g->n_cols <= 0 || g->n_rows <= 0)
+        return fail(XRD_ERR_DIMS, "%s: non-positive geometry size", fn);
+    if ((int64_t)g->nx * g->ny > (1ll << 28))
+        return fail(XRD_ERR_DIMS, "%s: voxel grid too large", fn);
+    if (d->n_energy <= 0 || d->n_q <= 0 || d->n_qf <= 0 || d->n_mat <= 0 || d->lut_nq <= 0 ||
+        d->lut_na <= 0 || d->energy_halfwidth < 0)
+        return fail(XRD_ERR_DIMS, "%s: non-positive discretization size", fn);
+    if (d->n_energy <= 2 * d->energy_halfwidth)
+        return fail(XRD_ERR_DIMS, "%s: n_energy (%d) must exceed 2*energy_halfwidth (%d)", fn,
+                    d->n_energy, 2 * d->energy_halfwidth);
+    const double vals[] = {g->center_x, g->center_y, g->r_src, g->r_det, g->view_angle0,
+                           g->view_angle_step, g->col_pitch, g->row_pitch, g->det_z0,
+                           g->target_angle, g->focal_spot, g->wedge_angle, g->voxel_size,
+                           d->e_min, d->e_width, d->q_min, d->q_max, d->q_dq0, d->ray_step,
+                           d->hbarc, d->mc2, d->kn0, d->kn1, d->kn2};
+    for (double v : vals)
+        if (!std::isfinite(v)) return fail(XRD_ERR_NONFINITE, "%s: non-finite descriptor field", fn);
+    if (!(d->ray_step > 0.0) || !(d->e_width > 0.0) || !(d->q_max > d->q_min) ||
+        !(d->hbarc > 0.0) || !(d->mc2 > 0.0))
+        return fail(XRD_ERR_INVALID_ARG, "%s: ray_step, e_width, hbarc, mc2 must be > 0 and "
+                    "q_max > q_min", fn);
+    if (d->e_min <= 0.0) return fail(XRD_ERR_INVALID_ARG, "%s: e_min must be > 0", fn);
+    return XRD_OK;
+}
+
+hipError_t stage_to_device(xrd_context *c, const void *host, size_t bytes, void *dev,
+                           hipStream_t s) {
+    if (c->staged_pending) {
+        hipError_t e = hipEventSynchronize(c->staged);
+        if (e != hipSuccess) return e;
+        c->staged_pending = false;
+    }
+    if (bytes > c->pinned_bytes) {
+        if (c->pinned) hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+        hipError_t e = hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        c->pinned_bytes = bytes;
+    }
+    memcpy(c->pinned, host, bytes);
+    hipError_t e = hipMemcpyAsync(dev, c->pinned, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(c->staged, s);
+    if (e == hipSuccess) c->staged_pending = true;
+    return e;
+}
+
+}  // namespace
+
+extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                           const float *spectrum, const float *ifine,
+                           const int32_t *segmentation, const float *a1, const float *a2,
+                           const float *s2_table, double lut_q_min, double lut_q_max,
+                           double lut_a_min, double lut_a_max, float *out_counts,
+                           float *out_compton, float *out_A, void *stream) {
+    const char *fn = "xrd_forward";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_geometry(g, d, fn);
+    if (rc) return rc;
+    if (!spectrum || !ifine || !segmentation || !a1 || !a2 || !s2_table || !out_counts ||
+        !out_compton)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required input/output pointer is NULL", fn);
+    const void *ptrs[] = {spectrum, ifine, segmentation, a1, a2, s2_table, out_counts,
+                          out_compton, out_A};
+    for (const void *p : ptrs)
+        if (p && !aligned(p, 16)) return fail(XRD_ERR_ALIGN, "%s: pointers must be 16-byte aligned", fn);
+    const double lims[] = {lut_q_min, lut_q_max, lut_a_min, lut_a_max};
+    for (double v : lims)
+        if (!std::isfinite(v)) return fail(XRD_ERR_NONFINITE, "%s: non-finite LUT limit", fn);
+    if (!(lut_q_max > lut_q_min) || !(lut_a_max > lut_a_min))
+        return fail(XRD_ERR_INVALID_ARG, "%s: LUT limits must satisfy max > min", fn);
+    if ((int64_t)d->n_q * d->n_mat * (int64_t)g->n_views * g->n_cols * g->n_rows * d->n_energy >
+        (int64_t)1 << 40)
+        return fail(XRD_ERR_DIMS, "%s: model matrix too large", fn);
+    if (d->n_energy > 64 || d->n_q > 256 || d->n_qf > 768)
+        return fail(XRD_ERR_UNSUPPORTED,
+                    "%s: this build supports n_energy <= 64, n_q <= 256, n_qf <= 768", fn);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+
+    const int nx = g->nx, ny = g->ny, V = g->n_views, E = d->n_energy, Q = d->n_q;
+    const size_t npix = (size_t)nx * ny;
+    const size_t npad = (size_t)(nx + 2) * (ny + 2);
+    HIP_TRY(ctx->mu_padded.ensure(npad * sizeof(float2)), "workspace alloc");
+    HIP_TRY(ctx->s2u8.ensure((size_t)d->lut_nq * d->lut_na), "workspace alloc");
+    HIP_TRY(ctx->cpx.ensure(2 * npix * sizeof(float)), "workspace alloc");
+    HIP_TRY(ctx->ref_list.ensure(npix * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->voxels.ensure(npix * sizeof(VoxelEntry)), "workspace alloc");
+    HIP_TRY(ctx->offsets.ensure((size_t)(d->n_mat + 1) * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->src_int.ensure((size_t)V * npix * sizeof(float2)), "workspace alloc");
+
+    // host-computed tables: views, energies, coarse q axis (FP64, host libm)
+    const size_t vbytes = (size_t)V * sizeof(ViewConst);
+    const size_t ebytes = (size_t)E * sizeof(EnergyConst);
+    const size_t qbytes = (size_t)(2 * Q + 1) * sizeof(double);
+    std::vector<unsigned char> host(vbytes + ebytes + qbytes);
+    ViewConst *vc = reinterpret_cast<ViewConst *>(host.data());
+    EnergyConst *ec = reinterpret_cast<EnergyConst *>(host.data() + vbytes);
+    double *qc = reinterpret_cast<double *>(host.data() + vbytes + ebytes);
+    for (int v = 0; v < V; v++) {
+        // integrate_diffraction.cu:93-95
+        const double phi = g->view_angle0 + (double)v * g->view_angle_step;
+        vc[v].sp = sin(phi);
+        vc[v].cp = cos(phi);
+        vc[v].src_x = g->center_x + g->r_src * vc[v].sp;
+        vc[v].src_y = g->center_y - g->r_src * vc[v].cp;
+    }
+    for (int e = 0; e < E; e++) {
+        // integrate_diffraction.cu:269-292
+        EnergyConst &k = ec[e];
+        k.es1 = d->e_min + (double)e * d->e_width;
+        k.es2 = d->e_min + (double)(e + 1) * d->e_width;
+        k.esm = 0.5 * (k.es1 + k.es2);
+        k.ein = k.esm / d->mc2;
+        k.inv_f1 = 1.0 / (k.ein * k.ein * k.ein);
+        k.inv_f2 = 1.0 / (d->kn0 + d->kn1 * k.ein + d->kn2 * k.ein * k.ein);
+        k.wcoef = 2.0 * k.esm / d->hbarc;
+        k.k2sq = (k.esm / d->hbarc) * (k.esm / d->hbarc);
+    }
+    {
+        // wvt_value.cu:8-23
+        const double NW = (double)Q;
+        const double cc = (d->q_max - d->q_min - d->q_dq0 * NW) / (NW * NW);
+        for (int n = 0; n < Q; n++) {
+            const double e1 = d->q_min + (double)n * d->q_dq0 + cc * (double)n * (double)n;
+            const double e2 = d->q_min + (double)(n + 1) * d->q_dq0 + cc * (double)(n + 1) * (double)(n + 1);
+            qc[n] = 0.5 * (e1 + e2);
+            qc[Q + n] = e2 - e1;
+        }
+        qc[2 * Q] = d->q_min;
+    }
+    HIP_TRY(ctx->tables.ensure(host.size()), "workspace alloc");
+    HIP_TRY(stage_to_device(ctx, host.data(), host.size(), ctx->tables.ptr, s), "stage tables");
+
+    PrepArgs pa{};
+    pa.nx = nx;
+    pa.ny = ny;
+    pa.n_mat = d->n_mat;
+    pa.flags = d->flags;
+    pa.seg = segmentation;
+    pa.a1 = a1;
+    pa.a2 = a2;
+    pa.s2_table = s2_table;
+    pa.s2_count = d->lut_nq * d->lut_na;
+    pa.s2u8 = (uint8_t *)ctx->s2u8.ptr;
+    pa.mu_padded = (float2 *)ctx->mu_padded.ptr;
+    pa.cpx1 = (float *)ctx->cpx.ptr;
+    pa.cpx2 = (float *)ctx->cpx.ptr + npix;
+    pa.ref_list = (int *)ctx->ref_list.ptr;
+    pa.voxels = (VoxelEntry *)ctx->voxels.ptr;
+    pa.mat_offsets = (int *)ctx->offsets.ptr;
+    HIP_TRY(launch_prepare(pa, s), "prepare kernels");
+
+    ForwardArgs fa{};
+    fa.nx = nx;
+    fa.ny = ny;
+    fa.n_views = V;
+    fa.n_cols = g->n_cols;
+    fa.n_rows = g->n_rows;
+    fa.n_det = g->n_cols * g->n_rows;
+    fa.n_energy = E;
+    fa.dn = d->energy_halfwidth;
+    fa.n_q = Q;
+    fa.n_qf = d->n_qf;
+    fa.n_mat = d->n_mat;
+    fa.lut_nq = d->lut_nq;
+    fa.lut_na = d->lut_na;
+    fa.flags = d->flags;
+    fa.center_x = g->center_x;
+    fa.center_y = g->center_y;
+    fa.r_src = g->r_src;
+    fa.r_det = g->r_det;
+    fa.col_pitch = g->col_pitch;
+    fa.row_pitch = g->row_pitch;
+    fa.det_z0 = g->det_z0;
+    fa.sta = sin(g->target_angle);
+    fa.cta = cos(g->target_angle);
+    fa.tan_w = tan(g->wedge_angle);
+    fa.tan_w2 = tan(g->wedge_angle / 2.0);
+    fa.focal_spot = g->focal_spot;
+    fa.voxel_size = g->voxel_size;
+    fa.adet = g->col_pitch * g->row_pitch;
+    fa.e_min = d->e_min;
+    fa.e_width = d->e_width;
+    fa.hbarc = d->hbarc;
+    fa.mc2 = d->mc2;
+    fa.kn0 = d->kn0;
+    fa.kn1 = d->kn1;
+    fa.kn2 = d->kn2;
+    fa.ray_step = d->ray_step;
+    fa.dq_fine = (d->q_max - d->q_min) / (double)d->n_qf;
+    fa.lut_q_min = lut_q_min;
+    fa.lut_a_min = lut_a_min;
+    fa.dq_lut = (lut_q_max - lut_q_min) / (double)d->lut_nq;
+    fa.da_lut = (lut_a_max - lut_a_min) / (double)d->lut_na;
+    fa.spectrum = spectrum;
+    fa.ifine = ifine;
+    fa.s2u8 = (const uint8_t *)ctx->s2u8.ptr;
+    fa.mu_padded = (const float2 *)ctx->mu_padded.ptr;
+    fa.voxels = (const VoxelEntry *)ctx->voxels.ptr;
+    fa.mat_offsets = (const int *)ctx->offsets.ptr;
+    fa.src_int = (const float2 *)ctx->src_int.ptr;
+    fa.views = (const ViewConst *)ctx->tables.ptr;
+    fa.energies = (const EnergyConst *)((unsigned char *)ctx->tables.ptr + vbytes);
+    fa.q_coarse = (const double *)((unsigned char *)ctx->tables.ptr + vbytes + ebytes);
+    fa.out_n = out_counts;
+    fa.out_c = out_compton;
+    fa.out_a = out_A;
+    HIP_TRY(launch_source_integrals(fa, (float2 *)ctx->src_int.ptr, s), "source-leg kernel");
+    hipError_t e = launch_forward(fa, s);
+    if (e == hipErrorInvalidValue)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: no kernel instantiation for these sizes", fn);
+    HIP_TRY(e, "forward kernel");
+    return XRD_OK;
+}
+
+extern "C" int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step,
+                              int64_t n, const double *seg, int32_t *out_side,
+                              double *out_entry, int32_t *out_count, int32_t max_samples,
+                              int32_t *out_texels, int32_t n_basis, const float *maps,
+                              float *out_integrals, void *stream) {
+    const char *fn = "xrd_trace_rays";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (nx <= 0 || ny <= 0 || n < 0) return fail(XRD_ERR_DIMS, "%s: bad sizes", fn);
+    if (!std::isfinite(ray_step) || !(ray_step > 0.0))
+        return fail(XRD_ERR_INVALID_ARG, "%s: ray_step must be finite and > 0", fn);
+    if (n == 0) return XRD_OK;
+    if (!seg || !out_side || !out_entry || !out_count)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required pointer is NULL", fn);
+    if (out_texels && max_samples <= 0)
+        return fail(XRD_ERR_INVALID_ARG, "%s: max_samples must be > 0 with out_texels", fn);
+    if (out_integrals && (n_basis <= 0 || !maps))
+        return fail(XRD_ERR_INVALID_ARG, "%s: out_integrals needs n_basis > 0 and maps", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(launch_trace_rays(nx, ny, ray_step, n, seg, out_side, out_entry, out_count,
+                              out_texels ? max_samples : 0, out_texels,
+                              out_integrals ? n_basis : 0, maps, out_integrals,
+                              (hipStream_t)stream),
+            "trace kernel");
+    return XRD_OK;
+}
+
+namespace {
+
+int check_matrix(const void *A, int dtype, int64_t n_rows, int32_t n_cols, const char *fn) {
+    if (!A) return fail(XRD_ERR_NULL_POINTER, "%s: A is NULL", fn);
+    if (dtype != XRD_DTYPE_F32 && dtype != XRD_DTYPE_BF16 && dtype != XRD_DTYPE_F16)
+        return fail(XRD_ERR_INVALID_ARG, "%s: unknown dtype %d", fn, dtype);
+    if (n_rows <= 0 || n_cols <= 0) return fail(XRD_ERR_DIMS, "%s: non-positive size", fn);
+    if (n_cols % 4 != 0) return fail(XRD_ERR_DIMS, "%s: n_cols must be a multiple of 4", fn);
+    if (n_cols > 2048) return fail(XRD_ERR_UNSUPPORTED, "%s: n_cols > 2048", fn);
+    if (!aligned(A, dtype == XRD_DTYPE_F32 ? 16 : 8))
+        return fail(XRD_ERR_ALIGN, "%s: A misaligned", fn);
+    return XRD_OK;
+}
+
+}  // namespace
+
+extern "C" int xrd_project(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                           int32_t n_cols, const float *x, float *out, void *stream) {
+    const char *fn = "xrd_project";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_matrix(A, a_dtype, n_rows, n_cols, fn);
+    if (rc) return rc;
+    if (!x || !out) return fail(XRD_ERR_NULL_POINTER, "%s: x/out is NULL", fn);
+    if (!aligned(x, 16)) return fail(XRD_ERR_ALIGN, "%s: x must be 16-byte aligned", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    MatArgs m{A, a_dtype, n_rows, n_cols};
+    HIP_TRY(launch_project(m, x, out, (hipStream_t)stream), "project kernel");
+    return XRD_OK;
+}
+
+static int colsum_common(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                         int32_t n_cols, const float *w, double *out, void *stream,
+                         const char *fn) {
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_matrix(A, a_dtype, n_rows, n_cols, fn);
+    if (rc) return rc;
+    if (!out) return fail(XRD_ERR_NULL_POINTER, "%s: out is NULL", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    MatArgs m{A, a_dtype, n_rows, n_cols};
+    const int nb = mat_num_blocks(m);
+    HIP_TRY(ctx->partial.ensure((size_t)nb * n_cols * sizeof(double)), "workspace alloc");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(launch_colsum_partials(m, w, (double *)ctx->partial.ptr, s), "colsum kernel");
+    HIP_TRY(launch_reduce_partials((const double *)ctx->partial.ptr, nb, n_cols, out, s),
+            "reduce kernel");
+    return XRD_OK;
+}
+
+extern "C" int xrd_backproject(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                               int32_t n_cols, const float *w, double *out, void *stream) {
+    if (!w) return fail(XRD_ERR_NULL_POINTER, "xrd_backproject: w is NULL");
+    return colsum_common(ctx, A, a_dtype, n_rows, n_cols, w, out, stream, "xrd_backproject");
+}
+
+extern "C" int xrd_sensitivity(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                               int32_t n_cols, double *out, void *stream) {
+    return colsum_common(ctx, A, a_dtype, n_rows, n_cols, nullptr, out, stream,
+                         "xrd_sensitivity");
+}
+
+extern "C" int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, int out_dtype,
+                                  void *out, void *stream) {
+    const char *fn = "xrd_convert_matrix";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!A || !out) return fail(XRD_ERR_NULL_POINTER, "%s: A/out is NULL", fn);
+    if (n < 0) return fail(XRD_ERR_DIMS, "%s: n < 0", fn);
+    if (out_dtype != XRD_DTYPE_BF16 && out_dtype != XRD_DTYPE_F16)
+        return fail(XRD_ERR_INVALID_ARG, "%s: out_dtype must be BF16 or F16", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(launch_convert(A, n, out_dtype, out, (hipStream_t)stream), "convert kernel");
+    return XRD_OK;
+}
+
+// ---- MLEM -------------------------------------------------------------------------
+
+namespace {
+
+hipError_t mlem_one_iteration(xrd_mlem_plan *p, hipStream_t s) {
+    hipError_t e;
+    if (p->mode == XRD_MLEM_FUSED) {
+        e = launch_mlem_fused(p->m, p->x, p->y, p->c, (double *)p->partial.ptr, s);
+        if (e != hipSuccess) return e;
+    } else {
+        e = launch_project(p->m, p->x, (float *)p->f.ptr, s);
+        if (e != hipSuccess) return e;
+        e = launch_ratio(p->m.n_rows, (const float *)p->f.ptr, p->y, p->c, (float *)p->r.ptr, s);
+        if (e != hipSuccess) return e;
+        e = launch_colsum_partials(p->m, (const float *)p->r.ptr, (double *)p->partial.ptr, s);
+        if (e != hipSuccess) return e;
+    }
+    return launch_mlem_update((const double *)p->partial.ptr, p->nblocks, p->m.n_cols,
+                              (const double *)p->sens.ptr, p->x, s);
+}
+
+}  // namespace
+
+extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                                    int32_t n_cols, const float *y, const float *c, float *x,
+                                    int mode, void *stream, xrd_mlem_plan **out) {
+    const char *fn = "xrd_mlem_plan_create";
+    if (!out) return fail(XRD_ERR_NULL_POINTER, "%s: out is NULL", fn);
+    *out = nullptr;
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_matrix(A, a_dtype, n_rows, n_cols, fn);
+    if (rc) return rc;
+    if (!y || !c || !x) return fail(XRD_ERR_NULL_POINTER, "%s: y/c/x is NULL", fn);
+    if (!aligned(x, 16)) return fail(XRD_ERR_ALIGN, "%s: x must be 16-byte aligned", fn);
+    if (mode != XRD_MLEM_FUSED && mode != XRD_MLEM_UNFUSED)
+        return fail(XRD_ERR_INVALID_ARG, "%s: unknown mode %d", fn, mode);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    xrd_mlem_plan *p = new xrd_mlem_plan();
+    p->ctx = ctx;
+    p->m = MatArgs{A, a_dtype, n_rows, n_cols};
+    p->y = y;
+    p->c = c;
+    p->x = x;
+    p->mode = mode;
+    p->nblocks = mat_num_blocks(p->m);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = p->sens.ensure((size_t)n_cols * sizeof(double));
+    if (e == hipSuccess) e = p->partial.ensure((size_t)p->nblocks * n_cols * sizeof(double));
+    if (e == hipSuccess) e = p->llp.ensure((size_t)p->nblocks * sizeof(double) + 8);
+    if (e == hipSuccess && mode == XRD_MLEM_UNFUSED) {
+        e = p->f.ensure((size_t)n_rows * sizeof(float));
+        if (e == hipSuccess) e = p->r.ensure((size_t)n_rows * sizeof(float));
+    }
+    // sensitivity sum(A,1)' once, FP64 (main.m:162)
+    if (e == hipSuccess) e = launch_colsum_partials(p->m, nullptr, (double *)p->partial.ptr, s);
+    if (e == hipSuccess)
+        e = launch_reduce_partials((const double *)p->partial.ptr, p->nblocks, n_cols,
+                                   (double *)p->sens.ptr, s);
+    if (e != hipSuccess) {
+        xrd_mlem_plan_destroy(p);
+        return hip_fail(e, "xrd_mlem_plan_create");
+    }
+    *out = p;
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_plan_iterate(xrd_mlem_plan *p, int32_t n_iter, int use_graph,
+                                     void *stream) {
+    const char *fn = "xrd_mlem_plan_iterate";
+    if (!p) return fail(XRD_ERR_NULL_POINTER, "%s: plan is NULL", fn);
+    if (n_iter < 0) return fail(XRD_ERR_INVALID_ARG, "%s: n_iter < 0", fn);
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) {
+        for (int i = 0; i < n_iter; i++) HIP_TRY(mlem_one_iteration(p, s), "mlem iteration");
+        return XRD_OK;
+    }
+    if (!p->exec) {
+        // capture one iteration on a private stream, replay it n_iter times
+        hipStream_t cs;
+        HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t e2 = mlem_one_iteration(p, cs);
+            e = hipStreamEndCapture(cs, &p->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        hipStreamDestroy(cs);
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+    }
+    for (int i = 0; i < n_iter; i++) HIP_TRY(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_plan_loglik(xrd_mlem_plan *p, double *out_host, void *stream) {
+    if (!p || !out_host) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_plan_loglik: NULL argument");
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(launch_loglik_partials(p->m, p->x, p->y, p->c, (double *)p->llp.ptr, s), "loglik kernel");
+    std::vector<double> h(p->nblocks);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->llp.ptr, h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double t = 0.0;
+    for (double v : h) t += v;
+    *out_host = t;
+    return XRD_OK;
+}
+
+extern "C" const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *p) {
+    return p ? (const double *)p->sens.ptr : nullptr;
+}
+
+extern "C" int xrd_mlem_plan_destroy(xrd_mlem_plan *p) {
+    if (!p) return XRD_OK;
+    hipSetDevice(p->ctx->device);
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->sens.release();
+    p->partial.release();
+    p->f.release();
+    p->r.release();
+    p->llp.release();
+    delete p;
+    return XRD_OK;
+}

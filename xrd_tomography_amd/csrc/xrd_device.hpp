@@ -1,0 +1,167 @@
+// xrd_device.hpp -- device helpers shared by the gfx950 kernels.
+//
+// Ray traversal (GetIntersection.cu:9-57, attenuator.cu:19-69) is done in
+// FP64 with contraction disabled, so its results (entry point, sample count,
+// texel indices) are the same bits as any IEEE double restatement that
+// performs the same operations in the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xrd {
+
+constexpr int kWave = 64;
+
+// ---- FP64 ray traversal ------------------------------------------------------
+
+// GetIntersection.cu:9-57: first side (y=0, x=nx, y=ny, x=0) crossed by the
+// segment (x1,y1)->(x2,y2) with |den| > 1e-5 and parameters in (-eps, 1+eps).
+__device__ __forceinline__ int get_intersection(double x1, double y1, double x2, double y2,
+                                                int nx, int ny, double &rx, double &ry) {
+#pragma clang fp contract(off)
+    const double X = (double)nx, Y = (double)ny;
+    const double eps = 1e-5;
+    const double x2_x1 = x2 - x1;
+    const double y2_y1 = y2 - y1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double x3 = (i == 1 || i == 2) ? X : 0.0;
+        const double x4 = (i == 0 || i == 1) ? X : 0.0;
+        const double y3 = (i >= 2) ? Y : 0.0;
+        const double y4 = (i == 1 || i == 2) ? Y : 0.0;
+        const double x4_x3 = x4 - x3;
+        const double y3_y1 = y3 - y1;
+        const double y4_y3 = y4 - y3;
+        const double x3_x1 = x3 - x1;
+        const double den = x4_x3 * y2_y1 - x2_x1 * y4_y3;
+        if (den > eps || den < -eps) {
+            const double ua = (x4_x3 * y3_y1 - x3_x1 * y4_y3) / den;
+            if (ua > -eps && ua < 1.0 + eps) {
+                const double ub = (x2_x1 * y3_y1 - x3_x1 * y2_y1) / den;
+                if (ub > -eps && ub < 1.0 + eps) {
+                    rx = x1 + x2_x1 * ua;
+                    ry = y1 + y2_y1 * ua;
+                    return i;
+                }
+            }
+        }
+    }
+    rx = __longlong_as_double(0x7ff8000000000000ll);
+    ry = rx;
+    return -1;
+}
+
+// One attenuator ray: entry point e, unit direction u, and sample count n
+// (samples q = 0..n-1 with q*step <= |P2 - e|; none if there is no entry).
+struct Ray {
+    double ex, ey, ux, uy;
+    int n;
+    int side;
+};
+
+__device__ __forceinline__ Ray make_ray(double x1, double y1, double x2, double y2, int nx,
+                                        int ny, double step) {
+#pragma clang fp contract(off)
+    Ray r;
+    r.side = get_intersection(x1, y1, x2, y2, nx, ny, r.ex, r.ey);
+    r.ux = r.uy = 0.0;
+    r.n = 0;
+    if (r.side < 0) return r;
+    const double lx = x2 - r.ex, ly = y2 - r.ey;
+    const double L = sqrt(lx * lx + ly * ly);
+    r.ux = lx / L;
+    r.uy = ly / L;
+    if (!(L >= 0.0)) return r;
+    int n = (int)floor(L / step) + 1;
+    while ((double)n * step <= L) n++;
+    while (n > 0 && (double)(n - 1) * step > L) n--;
+    r.n = n;
+    return r;
+}
+
+// Position of sample q (attenuator.cu:49-55), FP64, no contraction.
+__device__ __forceinline__ void ray_sample(const Ray &r, double step, int q, double &px,
+                                           double &py) {
+#pragma clang fp contract(off)
+    const double qr = step * (double)q;
+    px = r.ex + r.ux * qr;
+    py = r.ey + r.uy * qr;
+}
+
+// ---- compensated FP32 accumulation --------------------------------------------
+
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ Kahan() : s(0.f), c(0.f) {}
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s - c; }
+};
+
+// Exact-weight bilinear sample of an interleaved (a1,a2) map stored with a one
+// texel zero border: padded[(i+1) + (nx+2)*(j+1)] holds texel (i,j).  Corner
+// indices are clamped into the border independently, which reproduces the
+// texture unit's border-zero addressing (Forward_XRD.cu:229-233) for any
+// position.  Texel (i,j) is centred at (i+0.5, j+0.5).
+__device__ __forceinline__ float2 bilinear2(const float2 *__restrict__ padded, int nx, int ny,
+                                            double px, double py) {
+    const double xt = px - 0.5, yt = py - 0.5;
+    const double fx = floor(xt), fy = floor(yt);
+    const float ax = (float)(xt - fx), ay = (float)(yt - fy);
+    int i0 = (int)fx, j0 = (int)fy;
+    const int pw = nx + 2;
+    const int i0c = min(max(i0, -1), nx) + 1, i1c = min(max(i0 + 1, -1), nx) + 1;
+    const int j0c = min(max(j0, -1), ny) + 1, j1c = min(max(j0 + 1, -1), ny) + 1;
+    const float2 t00 = padded[i0c + pw * j0c];
+    const float2 t10 = padded[i1c + pw * j0c];
+    const float2 t01 = padded[i0c + pw * j1c];
+    const float2 t11 = padded[i1c + pw * j1c];
+    const float bx = 1.f - ax, by = 1.f - ay;
+    float2 v;
+    v.x = by * (bx * t00.x + ax * t10.x) + ay * (bx * t01.x + ax * t11.x);
+    v.y = by * (bx * t00.y + ax * t10.y) + ay * (bx * t01.y + ax * t11.y);
+    return v;
+}
+
+// ---- wave reductions (wave64) ---------------------------------------------------
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// find_src_nrg_range.cu:8-25 (the window is shifted, never truncated).
+__host__ __device__ __forceinline__ void src_window(int nrg, int n_energy, int hw, int &lo,
+                                                    int &hi) {
+    int mn = nrg - hw;
+    if (mn < 0) mn = 0;
+    int mx = mn + 2 * hw;
+    if (mx >= n_energy) {
+        mx = n_energy - 1;
+        mn = mx - 2 * hw;
+    }
+    lo = mn;
+    hi = mx;
+}
+
+// floor(t) clamped to [0,n-1], NaN -> 0 (integrate_diffraction.cu:187-189, :337-339).
+__device__ __forceinline__ int lut_index(double t, int n) {
+    const double f = floor(t);
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)n) return n - 1;
+    return (int)f;
+}
+
+}  // namespace xrd

@@ -1,0 +1,117 @@
+// xrd_internal.hpp -- argument blocks and launchers shared by the kernel files
+// and the C-ABI layer.  Not part of the public interface (include/xrd_tomo.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/xrd_tomo.h"
+
+namespace xrd {
+
+// One entry of the non-air voxel list (reference loop order within a material).
+struct VoxelEntry {
+    int32_t pix;     // ix + nx*iy
+    int32_t mat;     // 0-based material
+    float a1c, a2c;  // (a1,a2) used by the Compton term
+};
+
+// Per-energy-bin constants, host-computed in FP64.
+struct EnergyConst {
+    double es1, es2, esm;  // source-bin edges and centre, keV
+    double ein;            // esm / mc2
+    double inv_f1, inv_f2; // 1/e^3 and 1/f2(e) at ein
+    double wcoef;          // 2*esm/hbarc : wvt = wcoef * sin(theta/2)
+    double k2sq;           // (esm/hbarc)^2
+};
+
+// Per-view constants, host-computed in FP64 with the host libm.
+struct ViewConst {
+    double sp, cp;        // sin/cos of the view angle
+    double src_x, src_y;  // source position
+};
+
+struct ForwardArgs {
+    // sizes
+    int nx, ny, n_views, n_cols, n_rows, n_det;
+    int n_energy, dn, n_q, n_qf, n_mat, lut_nq, lut_na;
+    unsigned flags;
+    // geometry scalars
+    double center_x, center_y, r_src, r_det, col_pitch, row_pitch, det_z0;
+    double sta, cta;          // sin/cos target angle
+    double tan_w, tan_w2;     // tan(wedge), tan(wedge/2)
+    double focal_spot, voxel_size, adet;
+    // discretisation scalars
+    double e_min, e_width, hbarc, mc2, kn0, kn1, kn2, ray_step;
+    double dq_fine, lut_q_min, lut_a_min, dq_lut, da_lut;
+    // inputs
+    const float *spectrum;      // [E x E]
+    const float *ifine;         // [QF x M]
+    const uint8_t *s2u8;        // [NQ x NA] quantised Compton table
+    const float2 *mu_padded;    // [(nx+2) x (ny+2)] (a1,a2), zero border
+    const VoxelEntry *voxels;   // material-major voxel list
+    const int *mat_offsets;     // [M+1] list offsets per material
+    const float2 *src_int;      // [V x nx*ny] source-leg integrals (a1,a2)
+    const ViewConst *views;     // [V]
+    const EnergyConst *energies;// [E]
+    const double *q_coarse;     // [2*Q]: centres then widths (FP64)
+    // outputs
+    float *out_n, *out_c, *out_a;
+};
+
+struct PrepArgs {
+    int nx, ny, n_mat;
+    unsigned flags;
+    const int32_t *seg;
+    const float *a1, *a2;
+    const float *s2_table;
+    int s2_count;
+    uint8_t *s2u8;
+    float2 *mu_padded;
+    float *cpx1, *cpx2;     // per-pixel Compton (a1,a2) images
+    int *ref_list;          // scratch [nx*ny]
+    VoxelEntry *voxels;
+    int *mat_offsets;
+};
+
+hipError_t launch_prepare(const PrepArgs &a, hipStream_t s);
+hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStream_t s);
+// Returns hipErrorInvalidValue when no kernel instantiation fits the sizes.
+hipError_t launch_forward(const ForwardArgs &a, hipStream_t s);
+hipError_t launch_trace_rays(int nx, int ny, double step, int64_t n, const double *seg,
+                             int32_t *side, double *entry, int32_t *count, int32_t max_samples,
+                             int32_t *texels, int32_t n_basis, const float *maps, float *ints,
+                             hipStream_t s);
+
+// ---- explicit-matrix kernels -------------------------------------------------------
+struct MatArgs {
+    const void *A;
+    int dtype;
+    int64_t n_rows;
+    int n_cols;
+};
+
+int mat_num_blocks(const MatArgs &m);
+hipError_t launch_project(const MatArgs &m, const float *x, float *out, hipStream_t s);
+// Writes per-block FP64 partial column sums of diag(w) A (w = NULL: ones)
+// into partial[nblocks x n_cols].
+hipError_t launch_colsum_partials(const MatArgs &m, const float *w, double *partial,
+                                  hipStream_t s);
+// out[j] = sum_b partial[b, j]  (fixed order)
+hipError_t launch_reduce_partials(const double *partial, int nblocks, int n_cols, double *out,
+                                  hipStream_t s);
+// fused MLEM forward+adjoint pass: partial[b, j] = sum_{r in block b} A[r,j] y_r/(A_r x + c_r + 1e-20)
+hipError_t launch_mlem_fused(const MatArgs &m, const float *x, const float *y, const float *c,
+                             double *partial, hipStream_t s);
+// ratio r = y / (f + c + 1e-20) given f = A x
+hipError_t launch_ratio(int64_t n, const float *f, const float *y, const float *c, float *r,
+                        hipStream_t s);
+// x <- x * (sum_b partial[b,j]) / (sens[j] + 1e-7), non-finite -> 0, < 0 -> 0
+hipError_t launch_mlem_update(const double *partial, int nblocks, int n_cols,
+                              const double *sens, float *x, hipStream_t s);
+// per-block FP64 partial Poisson log-likelihood
+hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float *y,
+                                  const float *c, double *partial, hipStream_t s);
+hipError_t launch_convert(const float *A, int64_t n, int dtype, void *out, hipStream_t s);
+
+}  // namespace xrd

@@ -1,0 +1,305 @@
+"""Seeded synthetic inputs for the forward projector and MLEM.
+
+The reference builds its inputs in MATLAB from measured data
+(spectrum_builder.m, XSection_builder.m, phantom_generator.m, XRD_builder.m).
+Those builders are replaced here by seeded generators with the same output
+semantics, so every workload is reproducible without MATLAB or HDF5:
+
+* attenuation: the reference's two-basis model mu(E) = a1/e^3 + a2/f2(e),
+  e = E/mc^2, f2 = 1/KN quadratic (integrate_diffraction.cu:278,291-296), with
+  a1 = rho*a, a2 = rho*b per material (phantom_generator.m:31-36 uses the same
+  split), in 1/voxel as the kernel expects (main.m:60-61);
+* spectrum: Kramers (80 - E)/E behind 2.5 mm Al, smeared with the detector
+  response sigma(E) = 0.5*(1.61 + 0.025 E) keV into the NRG x NRG matrix
+  Ndet(e_src, e_det) that Forward_XRD takes as its first input
+  (spectrum_builder.m:65-95);
+* XRD form factors: 3-6 Gaussian Bragg peaks plus an amorphous hump per
+  material on the fine q axis (XRD_builder.m builds the same object from
+  measured patterns);
+* Compton table: a smooth incoherent-scattering surface on the
+  (log2 q, log2 a1/a2) grid, values 0..255 (XSection_builder.m:93-128).
+
+Geometry of the scaled workloads: fan beam, source-isocentre 600 mm,
+source-detector 1000 mm, isocentre at the centre of the voxel grid, one
+detector row at a height and column pitch chosen so that isocentre rays
+scatter between 2 and 8 degrees.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+from .descriptors import Discretization, Geometry, q_axis
+
+_DATA = os.path.join(os.path.dirname(__file__), "data")
+
+MC2 = 510.99895
+RHO_AL = 2.699
+
+
+@dataclasses.dataclass
+class Problem:
+    """Everything xrd_forward takes, as host numpy arrays (float32/int32)."""
+    name: str
+    geometry: Geometry
+    disc: Discretization
+    spectrum: np.ndarray      # [E x E] float32, spectrum[e_src + E*e_det]
+    ifine: np.ndarray         # [QF x M] float32 (q fastest), flattened order
+    segmentation: np.ndarray  # [NX x NY] int32, x fastest
+    a1: np.ndarray            # [NX x NY] float32, 1/voxel
+    a2: np.ndarray            # [NX x NY] float32, 1/voxel
+    s2_table: np.ndarray      # [NQ x NA] float32, 0..255
+    lut: tuple                # (q_min, q_max, a_min, a_max)
+    x_true: np.ndarray        # [Q x M] float64 form factor sampled at coarse centres
+    photons_per_view: float
+    mlem_iterations: int
+    voxel_mm: float
+
+    @property
+    def n_rows(self) -> int:
+        g, d = self.geometry, self.disc
+        return g.n_views * g.n_cols * g.n_rows * d.n_energy
+
+    @property
+    def n_cols(self) -> int:
+        return self.disc.n_q * self.disc.n_mat
+
+    def flat(self, name: str) -> np.ndarray:
+        """Array in the ABI's memory order (first index fastest), 1-D."""
+        a = getattr(self, name)
+        return np.ascontiguousarray(a.reshape(-1, order="F") if a.ndim > 1 else a)
+
+
+# ---------------------------------------------------------------------------
+# physics helpers
+
+def f2_kn(e, d: Discretization):
+    """1/KN quadratic of Forward_XRD.cu:51-53 at reduced energy e = E/mc^2."""
+    return d.kn0 + d.kn1 * e + d.kn2 * e * e
+
+
+def mu_total(a1, a2, energy_kev, d: Discretization):
+    """mu(E) = a1/e^3 + a2/f2(e) (integrate_diffraction.cu:291-296)."""
+    e = energy_kev / d.mc2
+    return a1 / e ** 3 + a2 / f2_kn(e, d)
+
+
+def al_mass_attenuation(energy_kev):
+    """Log-log interpolation of the Al mu/rho table (cm^2/g)."""
+    tab = np.loadtxt(os.path.join(_DATA, "al_mass_attenuation.csv"), delimiter=",",
+                     comments="#")
+    e_mev = np.asarray(energy_kev, dtype=np.float64) / 1000.0
+    return np.exp(np.interp(np.log(e_mev), np.log(tab[:, 0]), np.log(tab[:, 1])))
+
+
+def _norm_cdf(x):
+    from math import erf
+    return 0.5 * (1.0 + np.vectorize(erf)(x / math.sqrt(2.0)))
+
+
+def kramers_response_matrix(d: Discretization, kvp: float = 80.0, al_mm: float = 2.5):
+    """Ndet(e_src, e_det) [E x E], src fastest (spectrum_builder.m:65-95 semantics):
+    photons emitted in source bin e_src times the probability that the detector
+    response sigma(E) = 0.5*(1.61 + 0.025 E) puts them in detector bin e_det."""
+    E = d.n_energy
+    edges = d.e_min + d.e_width * np.arange(E + 1)
+    centres = 0.5 * (edges[:-1] + edges[1:])
+    shape = np.clip(kvp - centres, 0.0, None) / centres
+    shape *= np.exp(-al_mass_attenuation(centres) * RHO_AL * al_mm / 10.0)
+    shape *= d.e_width
+    sigma = 0.5 * (1.61 + 0.025 * centres)
+    # m[s, t]: probability that a photon of source bin s lands in detector bin t
+    z2 = (edges[None, 1:] - centres[:, None]) / sigma[:, None]
+    z1 = (edges[None, :-1] - centres[:, None]) / sigma[:, None]
+    prob = _norm_cdf(z2) - _norm_cdf(z1)
+    return (shape[:, None] * prob).astype(np.float64)  # [s, t]
+
+
+def form_factor(q, rng, n_peaks_range=(3, 6)):
+    """Gaussian Bragg peaks + amorphous hump, evaluated at q (1/Angstrom)."""
+    npk = int(rng.integers(n_peaks_range[0], n_peaks_range[1] + 1))
+    centres = rng.uniform(0.6, 3.4, npk)
+    widths = rng.uniform(0.02, 0.05, npk)
+    heights = rng.lognormal(0.0, 1.0, npk)
+    hc, hw = rng.uniform(1.0, 2.0), rng.uniform(0.3, 0.6)
+    hh = rng.lognormal(0.0, 0.5)
+    q = np.asarray(q, dtype=np.float64)
+    f = hh * np.exp(-0.5 * ((q - hc) / hw) ** 2)
+    for c, w, h in zip(centres, widths, heights):
+        f = f + h * np.exp(-0.5 * ((q - c) / w) ** 2)
+    return f, dict(centres=centres, widths=widths, heights=heights, hump=(hc, hw, hh))
+
+
+def compton_table(nq: int, na: int, lut):
+    """Smooth incoherent-scattering surface S2(log2 q, log2 a1/a2) in 0..255.
+    S rises from 0 to 1 with q (scaled by an a-dependent onset), times 255."""
+    qmin, qmax, amin, amax = lut
+    lq = qmin + (np.arange(nq) + 0.5) * (qmax - qmin) / nq
+    la = amin + (np.arange(na) + 0.5) * (amax - amin) / na
+    Lq, La = np.meshgrid(lq, la, indexing="ij")  # [nq, na]
+    onset = -8.0 + 0.25 * (La - 0.5 * (amin + amax))
+    s = 1.0 / (1.0 + np.exp(-(Lq - onset) * 1.3))
+    return (255.0 * s).astype(np.float64)  # [nq, na]
+
+
+# ---------------------------------------------------------------------------
+# geometry presets
+
+def fan_geometry(nx: int, ny: int, voxel_mm: float, n_views: int, n_cols: int,
+                 sid_mm: float = 600.0, sdd_mm: float = 1000.0,
+                 theta_min_deg: float = 2.0, theta_max_deg: float = 8.0) -> Geometry:
+    """Fan-beam scanner with the XRD panel opposite the source (the reference's
+    arrangement, integrate_diffraction.cu:94-121) at source-isocentre sid_mm and
+    source-detector sdd_mm; one row at height z0 and COL columns spanning the
+    lateral offsets for which isocentre rays scatter in [theta_min, theta_max]."""
+    iso_det = sdd_mm - sid_mm
+    z0 = iso_det * math.tan(math.radians(theta_min_deg))
+    rho_max = iso_det * math.tan(math.radians(theta_max_deg))
+    r_max = math.sqrt(rho_max ** 2 - z0 ** 2)
+    col_pitch = 2.0 * r_max / n_cols
+    v = voxel_mm
+    return Geometry(nx=nx, ny=ny, n_views=n_views, n_cols=n_cols, n_rows=1,
+                    center_x=nx / 2.0, center_y=ny / 2.0,
+                    r_src=sid_mm / v, r_det=iso_det / v,
+                    view_angle0=0.0, view_angle_step=2.0 * math.pi / n_views,
+                    col_pitch=col_pitch / v, row_pitch=1.0 / v, det_z0=z0 / v,
+                    target_angle=math.pi / 6.0, focal_spot=0.5 / v,
+                    wedge_angle=math.radians(0.5), voxel_size=1.0)
+
+
+def scaled_discretization(n_mat: int, n_energy: int = 61, e_lo: float = 20.0,
+                          e_hi: float = 80.0, n_q: int = 128, n_qf: int = 384,
+                          q_min: float = 0.5, q_max: float = 3.5,
+                          flags: int = 0) -> Discretization:
+    """Energy bins centred on e_lo..e_hi (1 keV for 61 bins), parabolic q axis."""
+    d = Discretization.reference(n_energy=n_energy, n_q=n_q, n_qf=n_qf, n_mat=n_mat,
+                                 flags=flags)
+    d.e_width = (e_hi - e_lo) / (n_energy - 1)
+    d.e_min = e_lo - 0.5 * d.e_width
+    d.q_min, d.q_max = q_min, q_max
+    return d
+
+
+# ---------------------------------------------------------------------------
+# phantoms
+
+def _paint_phantom(nx, ny, voxel_mm, n_mat, rng, disc_radius_mm, axis_range_mm,
+                   allow_rect):
+    seg = np.zeros((nx, ny), dtype=np.int32)  # seg[ix, iy]
+    x = (np.arange(nx) + 0.5 - nx / 2.0) * voxel_mm
+    y = (np.arange(ny) + 0.5 - ny / 2.0) * voxel_mm
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    for m in range(1, n_mat + 1):
+        a = rng.uniform(*axis_range_mm)
+        b = rng.uniform(*axis_range_mm)
+        r = rng.uniform(0.0, max(disc_radius_mm - max(a, b), 0.0))
+        t = rng.uniform(0.0, 2.0 * math.pi)
+        cx, cy = r * math.cos(t), r * math.sin(t)
+        ang = rng.uniform(0.0, math.pi)
+        ca, sa = math.cos(ang), math.sin(ang)
+        u = (X - cx) * ca + (Y - cy) * sa
+        w = -(X - cx) * sa + (Y - cy) * ca
+        if allow_rect and rng.uniform() < 0.5:
+            mask = (np.abs(u) <= a) & (np.abs(w) <= b)
+        else:
+            mask = (u / a) ** 2 + (w / b) ** 2 <= 1.0
+        mask &= X ** 2 + Y ** 2 <= disc_radius_mm ** 2
+        seg[mask] = m  # later materials win (painted in order)
+    return seg
+
+
+def _materials(n_mat, rng):
+    rho = rng.uniform(0.9, 2.7, n_mat)
+    a = np.exp(rng.uniform(math.log(2e-5), math.log(2e-4), n_mat))
+    b = rng.uniform(0.13, 0.17, n_mat)
+    return rho, a, b
+
+
+def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -> Problem:
+    """Build one of the named workloads.
+
+    ``fan64``  : 64x64 voxels at 4 mm, 4 ellipse materials, 60 views, 64 pixels,
+                 61 energies, 128 q-bins; 1e4 photons/view, 20 MLEM iterations.
+    ``fan256`` : 256x256 voxels at 2 mm, 8 materials in ellipses and rectangles,
+                 360 views, 256 pixels, 61 energies, 128 q-bins; 1e5 photons/view,
+                 50 MLEM iterations.
+    ``small``  : 24x24 voxels, 4 views, 6 pixels, 16 energies, 24 q-bins, 2 materials
+                 (tests).
+    ``reference`` : the reference's own scanner literals (integrate_diffraction.cu:81-140)
+                 on a 40x40 grid with 3 materials, 4 views, 16 pixels, 16 energies.
+    Keyword overrides replace sizes (nx, n_views, n_cols, n_energy, n_q, n_qf, n_mat,
+    flags, voxel_mm, photons_per_view, mlem_iterations).
+    """
+    presets = {
+        "fan64": dict(nx=64, voxel_mm=4.0, n_mat=4, n_views=60, n_cols=64, n_energy=61,
+                      n_q=128, n_qf=384, photons_per_view=1e4, mlem_iterations=20,
+                      allow_rect=False, axis=(12.0, 40.0), disc=110.0, seed=0),
+        "fan256": dict(nx=256, voxel_mm=2.0, n_mat=8, n_views=360, n_cols=256, n_energy=61,
+                       n_q=128, n_qf=384, photons_per_view=1e5, mlem_iterations=50,
+                       allow_rect=True, axis=(20.0, 80.0), disc=240.0, seed=1),
+        "small": dict(nx=24, voxel_mm=8.0, n_mat=2, n_views=4, n_cols=6, n_energy=16,
+                      n_q=24, n_qf=72, photons_per_view=1e4, mlem_iterations=10,
+                      allow_rect=True, axis=(20.0, 60.0), disc=90.0, seed=2),
+        "reference": dict(nx=40, voxel_mm=2.0, n_mat=3, n_views=4, n_cols=16, n_energy=16,
+                          n_q=32, n_qf=96, photons_per_view=1e4, mlem_iterations=10,
+                          allow_rect=False, axis=(8.0, 20.0), disc=35.0, seed=3),
+    }
+    if name not in presets:
+        raise ValueError(f"unknown workload {name!r}; choose from {sorted(presets)}")
+    p = dict(presets[name])
+    p.update({k: v for k, v in overrides.items() if k not in ("flags",)})
+    flags = int(overrides.get("flags", 0))
+    rng = np.random.Generator(np.random.PCG64(p["seed"] if seed is None else seed))
+    nx = ny = int(p["nx"])
+    vmm = float(p["voxel_mm"])
+    M = int(p["n_mat"])
+
+    if name == "reference":
+        g = Geometry.reference(nx, ny, n_views=int(p["n_views"]), n_cols=int(p["n_cols"]))
+        d = Discretization.reference(n_energy=int(p["n_energy"]), n_q=int(p["n_q"]),
+                                     n_qf=int(p["n_qf"]), n_mat=M, flags=flags)
+        # the reference phantom occupies [0,NX]^2 while its rotation centre is at
+        # (NX,NY) (integrate_diffraction.cu:88-95); keep inclusions inside the grid.
+    else:
+        g = fan_geometry(nx, ny, vmm, int(p["n_views"]), int(p["n_cols"]))
+        d = scaled_discretization(M, n_energy=int(p["n_energy"]), n_q=int(p["n_q"]),
+                                  n_qf=int(p["n_qf"]), flags=flags)
+
+    seg = _paint_phantom(nx, ny, vmm, M, rng, float(p["disc"]), p["axis"], p["allow_rect"])
+    rho, a, b = _materials(M, rng)
+    vox_cm = vmm / 10.0
+    a1 = np.zeros((nx, ny)); a2 = np.zeros((nx, ny))
+    for m in range(M):
+        a1[seg == m + 1] = rho[m] * a[m] * vox_cm
+        a2[seg == m + 1] = rho[m] * b[m] * vox_cm
+
+    q, dq, qf, dqf = q_axis(d)
+    ifine = np.zeros((d.n_qf, M))
+    x_true = np.zeros((d.n_q, M))
+    for m in range(M):
+        f_fine, _ = form_factor(qf, rng)
+        # reference magnitudes: 1e-3 .. 1 per cm (XRD_builder.m:52,101,152)
+        scale = math.exp(rng.uniform(math.log(1e-2), math.log(1.0)))
+        ifine[:, m] = scale * f_fine / f_fine.max()
+        x_true[:, m] = np.interp(q, qf, ifine[:, m])
+
+    spec = kramers_response_matrix(d)
+    lut = (-12.0, -4.0, -14.0, -8.0)
+    s2 = compton_table(d.lut_nq, d.lut_na, lut)
+
+    return Problem(name=name, geometry=g, disc=d,
+                   spectrum=spec.astype(np.float32), ifine=ifine.astype(np.float32),
+                   segmentation=seg, a1=a1.astype(np.float32), a2=a2.astype(np.float32),
+                   s2_table=s2.astype(np.float32), lut=lut, x_true=x_true,
+                   photons_per_view=float(p["photons_per_view"]),
+                   mlem_iterations=int(p["mlem_iterations"]), voxel_mm=vmm)
+
+
+def poisson_counts(expected: np.ndarray, seed: int) -> np.ndarray:
+    """Poisson noise (the poissrnd of main.m:141), seeded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.poisson(np.clip(expected, 0.0, None)).astype(np.float32)
