@@ -178,6 +178,38 @@ int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step, in
                    int32_t n_basis, const float *maps, float *out_integrals,
                    void *stream);
 
+/* ---- differentiable coefficient model and its VJP -----------------------------
+ * No reference counterpart as an entry point: the differentiable form of the
+ * Rayleigh/XRD accumulation (integrate_diffraction.cu:266-315) with
+ *   - per-voxel form factors F_vox(q_n) = sum_k coef[vox*K + k] basis[k*NWT + n]
+ *     on the coarse q axis (every voxel with segmentation > 0 scatters), and
+ *   - attenuation mu_e(pix) = sum_b beta[b*NRG + e] maps[b*nx*ny + pix]
+ *     (the reference is B = 2, maps = (a1,a2), beta = (1/e^3, 1/f2(e)) of
+ *     integrate_diffraction.cu:291-296; a per-energy map is B = NRG, beta = I).
+ * out_counts has the Nphotons layout (Compton excluded).
+ * Limits: K <= 32, B <= 64, NRG <= 64, NWT <= 128. */
+int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                       const float *spectrum, const int32_t *segmentation, const float *coef,
+                       int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,
+                       const float *beta, float *out_counts, void *stream);
+
+/* Vector-Jacobian product of xrd_forward_coeffs for the upstream gradient
+ * grad_counts [rows] (e.g. 1 - y/(counts + background) for the Poisson NLL).
+ * Line integrals are recomputed, not stored.  grad_coef [nx*ny x K] (FP64,
+ * deterministic) and grad_maps [B x nx*ny] (FP64, accumulated with atomics) are
+ * overwritten; either may be NULL. */
+int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                   const float *spectrum, const int32_t *segmentation, const float *coef,
+                   int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,
+                   const float *beta, const float *grad_counts, double *grad_coef,
+                   double *grad_maps, void *stream);
+
+/* Poisson negative log-likelihood sum(f - y log f), f = counts + background
+ * (background may be NULL), written to *out_loss_host (synchronises the stream);
+ * grad_counts (optional) receives 1 - y/f. */
+int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts, const float *background,
+                    const float *y, float *grad_counts, double *out_loss_host, void *stream);
+
 /* ---- model-matrix operators (explicit A, row-major [rows x cols]) ------------
  * A is the out_A of xrd_forward viewed as rows (src,col,row,e_det) x cols (nwt,mat),
  * as the reference's MLEM views it (main.m:140). a_dtype selects the storage type. */

@@ -211,3 +211,58 @@ def channel_match(es1, es2, eratio, ed1, ed2):
 
 def max_threads():
     return lib().xo_max_threads()
+
+
+def _cm_args(cp):
+    p = cp.base
+    sp = _d(p.flat("spectrum"))
+    sg = _i(p.flat("segmentation"))
+    co = _d(cp.coef.reshape(-1))
+    ba = _d(cp.basis.reshape(-1))
+    mp = _d(cp.maps.reshape(-1))
+    be = _d(cp.beta.reshape(-1))
+    return (sp, sg, co, ba, mp, be)
+
+
+def forward_coeffs(cp, nthreads=0, coef=None, maps=None):
+    """Counts of the coefficient model (no background), float64 [rows]."""
+    L = lib()
+    L.xo_forward_coeffs.restype = ctypes.c_int
+    L.xo_forward_coeffs.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
+                                    _dp, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                    _dp, ctypes.c_int]
+    p = cp.base
+    sp, sg, co, ba, mp, be = _cm_args(cp)
+    if coef is not None:
+        co = _d(np.asarray(coef).reshape(-1))
+    if maps is not None:
+        mp = _d(np.asarray(maps).reshape(-1))
+    out = np.zeros(p.n_rows)
+    rc = L.xo_forward_coeffs(ctypes.byref(p.geometry), ctypes.byref(p.disc), sp[1], sg[1], co[1],
+                             cp.K, ba[1], cp.B, mp[1], be[1], out.ctypes.data_as(_dp), nthreads)
+    if rc:
+        raise RuntimeError(f"xo_forward_coeffs failed: {rc}")
+    return out
+
+
+def vjp_coeffs(cp, w, nthreads=0, want_coef=True, want_maps=True):
+    """(grad_coef [npix, K], grad_maps [B, npix]) for upstream gradient w [rows]."""
+    L = lib()
+    L.xo_vjp_coeffs.restype = ctypes.c_int
+    L.xo_vjp_coeffs.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
+                                _dp, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                _dp, _dp, ctypes.c_int]
+    p = cp.base
+    sp, sg, co, ba, mp, be = _cm_args(cp)
+    ww = _d(w)
+    npix = p.geometry.nx * p.geometry.ny
+    gc = np.zeros(npix * cp.K) if want_coef else None
+    gm = np.zeros(cp.B * npix) if want_maps else None
+    rc = L.xo_vjp_coeffs(ctypes.byref(p.geometry), ctypes.byref(p.disc), sp[1], sg[1], co[1],
+                         cp.K, ba[1], cp.B, mp[1], be[1], ww[1],
+                         gc.ctypes.data_as(_dp) if want_coef else None,
+                         gm.ctypes.data_as(_dp) if want_maps else None, nthreads)
+    if rc:
+        raise RuntimeError(f"xo_vjp_coeffs failed: {rc}")
+    return (gc.reshape(npix, cp.K) if want_coef else None,
+            gm.reshape(cp.B, npix) if want_maps else None)

@@ -576,3 +576,259 @@ double xo_loglik(const double *A, int64_t n_rows, int n_cols, const double *y,
     free(f);
     return ll;
 }
+
+/* ---- coefficient model and its vector-Jacobian product ------------------------
+ * The differentiable form of integrate_diffraction.cu:266-315 used by the VJP:
+ * each non-air voxel carries its own form factor F_vox(q_n) = sum_k coef[vox,k]
+ * basis[k,n] on the coarse axis, and the attenuation is mu_e = sum_b beta[b,e] m_b
+ * (the reference is B = 2 with beta = (1/e^3, 1/f2(e)), integrate_diffraction.cu:291-296;
+ * a full per-energy map is B = NRG with beta = identity).
+ *   yhat[v,p,ed] = sum_vox sum_{e in win(ed)} RF[e,ed] P(e) S(e)
+ *   P(e) = exp(-sum_b beta[b,e] (Ls_b + Ld_b)) g R / sqrt(2 pi sigma_e^2)
+ *   S(e) = sum_n exp(-(wvt_e - q_n)^2 / (2 sigma_e^2)) dq_n F_vox(q_n)
+ * The Compton term is a constant background here. */
+
+/* Adds gamma[b]*step*(bilinear weight) of every sample of the ray to grad[b][pix]. */
+static void xo_line_scatter(double *grad, int nb, int nx, int ny, double step, double x1,
+                            double y1, double x2, double y2, const double *gamma) {
+    double ex, ey;
+    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) return;
+    double ux, uy, L;
+    xo_ray_dir(ex, ey, x2, y2, &ux, &uy, &L);
+    const int32_t n = xo_sample_count(L, step);
+    const long plane = (long)nx * ny;
+    for (int32_t q = 0; q < n; q++) {
+        const double qr = step * (double)q;
+        const double px = ex + ux * qr, py = ey + uy * qr;
+        const double xt = px - 0.5, yt = py - 0.5;
+        const double fi = floor(xt), fj = floor(yt);
+        const double ax = xt - fi, ay = yt - fj;
+        const long i = (long)fi, j = (long)fj;
+        const long ci[4] = {i, i + 1, i, i + 1}, cj[4] = {j, j, j + 1, j + 1};
+        const double w[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
+        for (int c = 0; c < 4; c++) {
+            if (ci[c] < 0 || cj[c] < 0 || ci[c] >= nx || cj[c] >= ny) continue;
+            const long pix = ci[c] + (long)nx * cj[c];
+            for (int b = 0; b < nb; b++) grad[b * plane + pix] += gamma[b] * step * w[c];
+        }
+    }
+}
+
+typedef struct {
+    int E, Q, K, B, nvox;
+    double *qc, *dqc, *Esm;
+    int32_t *vix, *viy;
+} xo_cm_setup;
+
+static int xo_cm_prepare(const xrd_geometry *g, const xrd_discretization *d,
+                         const int32_t *seg, int K, int B, xo_cm_setup *s) {
+    const int NX = g->nx, NY = g->ny, E = d->n_energy, Q = d->n_q;
+    if (NX <= 0 || NY <= 0 || E <= 2 * d->energy_halfwidth || Q <= 0 || K <= 0 || B <= 0)
+        return XRD_ERR_DIMS;
+    s->E = E;
+    s->Q = Q;
+    s->K = K;
+    s->B = B;
+    s->qc = (double *)malloc(sizeof(double) * (2 * Q + E));
+    s->dqc = s->qc + Q;
+    s->Esm = s->qc + 2 * Q;
+    for (int n = 0; n < Q; n++) xo_wvt_value(n, Q, d->q_min, d->q_max, d->q_dq0, &s->qc[n], &s->dqc[n]);
+    for (int e = 0; e < E; e++) {
+        const double e1 = d->e_min + (double)e * d->e_width;
+        const double e2 = d->e_min + (double)(e + 1) * d->e_width;
+        s->Esm[e] = 0.5 * (e1 + e2);
+    }
+    s->vix = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)NX * NY);
+    s->viy = s->vix + (size_t)NX * NY;
+    s->nvox = 0;
+    for (int ix = 0; ix < NX; ix++)
+        for (int iy = 0; iy < NY; iy++)
+            if (seg[ix + NX * iy] > 0) {
+                s->vix[s->nvox] = ix;
+                s->viy[s->nvox] = iy;
+                s->nvox++;
+            }
+    return XRD_OK;
+}
+
+static void xo_cm_free(xo_cm_setup *s) {
+    free(s->qc);
+    free(s->vix);
+}
+
+/* Per-(view, pixel, voxel) pieces: P[e], sigma2[e], wvt[e], line integrals. */
+static void xo_cm_triple(const xrd_geometry *g, const xrd_discretization *d,
+                         const xo_cm_setup *s, const double *maps, const double *beta, int view,
+                         int col, int row, int k, double *P, double *sig2, double *wvt,
+                         double *Ls, double *Ld, xo_scatter_geom *sg) {
+    const int NX = g->nx, NY = g->ny, B = s->B, E = s->E;
+    const int ix = s->vix[k], iy = s->viy[k];
+    xo_scatter_geometry(g, view, col, row, ix, iy, sg);
+    xo_line_integrals(maps, B, NX, NY, d->ray_step, sg->src_x, sg->src_y, ix + 0.5, iy + 0.5, Ls);
+    xo_line_integrals(maps, B, NX, NY, d->ray_step, sg->det_x, sg->det_y, ix + 0.5, iy + 0.5, Ld);
+    const double rs2pi = 1.0 / sqrt(2.0 * XO_PI);
+    for (int e = 0; e < E; e++) {
+        double mu = 0.0;
+        for (int b = 0; b < B; b++) mu += beta[(long)b * E + e] * (Ls[b] + Ld[b]);
+        const double Es = s->Esm[e];
+        const double k1 = 2.0 * sg->sin_half / d->hbarc, k2 = Es / d->hbarc;
+        sig2[e] = k1 * k1 * d->e_width * d->e_width / 12.0 +
+                  k2 * k2 * sg->res2 / (2.0 * sg->sin_half) / (2.0 * sg->sin_half) / sg->anorm2 /
+                      sg->bnorm2;
+        wvt[e] = 2.0 * Es / d->hbarc * sg->sin_half;
+        P[e] = exp(-mu) * sg->geom * sg->rayleigh * rs2pi / sqrt(sig2[e]);
+    }
+}
+
+int xo_forward_coeffs(const xrd_geometry *g, const xrd_discretization *d,
+                      const double *spectrum, const int32_t *seg, const double *coef, int K,
+                      const double *basis, int B, const double *maps, const double *beta,
+                      double *out_yhat, int nthreads) {
+    xo_cm_setup s;
+    int rc = xo_cm_prepare(g, d, seg, K, B, &s);
+    if (rc) return rc;
+    const int V = g->n_views, DET = g->n_cols * g->n_rows, E = s.E, Q = s.Q, DN = d->energy_halfwidth;
+    const long npix = (long)g->nx * g->ny;
+    const int nt = xo_threads(nthreads);
+#pragma omp parallel num_threads(nt)
+    {
+        double *P = (double *)malloc(sizeof(double) * (5 * E + 2 * B + Q));
+        double *sig2 = P + E, *wvt = sig2 + E, *S = wvt + E, *acc = S + E;
+        double *Ls = acc + E, *Ld = Ls + B, *F = Ld + B;
+#pragma omp for schedule(dynamic, 1)
+        for (long vp = 0; vp < (long)V * DET; vp++) {
+            const int v = (int)(vp % V), det = (int)(vp / V);
+            const int col = det % g->n_cols, row = det / g->n_cols;
+            for (int e = 0; e < E; e++) acc[e] = 0.0;
+            for (int k = 0; k < s.nvox; k++) {
+                xo_scatter_geom sg;
+                xo_cm_triple(g, d, &s, maps, beta, v, col, row, k, P, sig2, wvt, Ls, Ld, &sg);
+                const long pix = s.vix[k] + (long)g->nx * s.viy[k];
+                for (int n = 0; n < Q; n++) {
+                    double f = 0.0;
+                    for (int kk = 0; kk < K; kk++) f += coef[pix * K + kk] * basis[(long)kk * Q + n];
+                    F[n] = f;
+                }
+                for (int e = 0; e < E; e++) {
+                    double sv = 0.0;
+                    for (int n = 0; n < Q; n++) {
+                        const double dd = wvt[e] - s.qc[n];
+                        sv += exp(-dd * dd / (2.0 * sig2[e])) * s.dqc[n] * F[n];
+                    }
+                    acc[e] += P[e] * sv;
+                }
+            }
+            for (int ed = 0; ed < E; ed++) {
+                int lo, hi;
+                xo_find_src_nrg_range(ed, E, DN, &lo, &hi);
+                double y = 0.0;
+                for (int e = lo; e <= hi; e++) y += spectrum[e + (long)E * ed] * acc[e];
+                out_yhat[v + (long)V * (det + (long)DET * ed)] = y;
+            }
+        }
+        free(P);
+    }
+    (void)npix;
+    xo_cm_free(&s);
+    return XRD_OK;
+}
+
+int xo_vjp_coeffs(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                  const int32_t *seg, const double *coef, int K, const double *basis, int B,
+                  const double *maps, const double *beta, const double *w, double *grad_coef,
+                  double *grad_maps, int nthreads) {
+    xo_cm_setup s;
+    int rc = xo_cm_prepare(g, d, seg, K, B, &s);
+    if (rc) return rc;
+    const int V = g->n_views, DET = g->n_cols * g->n_rows, E = s.E, Q = s.Q, DN = d->energy_halfwidth;
+    const int NX = g->nx, NY = g->ny;
+    const long npix = (long)NX * NY;
+    const int nt = xo_threads(nthreads);
+    double *hpart = (double *)calloc((size_t)nt * s.nvox * Q, sizeof(double));
+    double *gpart = grad_maps ? (double *)calloc((size_t)nt * B * npix, sizeof(double)) : NULL;
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *H = hpart + (size_t)tid * s.nvox * Q;
+        double *GM = gpart ? gpart + (size_t)tid * B * npix : NULL;
+        double *P = (double *)malloc(sizeof(double) * (5 * E + 3 * B + Q));
+        double *sig2 = P + E, *wvt = sig2 + E, *S = wvt + E, *Wt = S + E;
+        double *Ls = Wt + E, *Ld = Ls + B, *gam = Ld + B, *F = gam + B;
+#pragma omp for schedule(dynamic, 1)
+        for (long vp = 0; vp < (long)V * DET; vp++) {
+            const int v = (int)(vp % V), det = (int)(vp / V);
+            const int col = det % g->n_cols, row = det / g->n_cols;
+            /* transpose of the detector-response window: Wt[e] = sum_ed RF[e,ed] w[ed] */
+            for (int e = 0; e < E; e++) Wt[e] = 0.0;
+            for (int ed = 0; ed < E; ed++) {
+                int lo, hi;
+                xo_find_src_nrg_range(ed, E, DN, &lo, &hi);
+                const double wr = w[v + (long)V * (det + (long)DET * ed)];
+                for (int e = lo; e <= hi; e++) Wt[e] += spectrum[e + (long)E * ed] * wr;
+            }
+            for (int k = 0; k < s.nvox; k++) {
+                xo_scatter_geom sg;
+                xo_cm_triple(g, d, &s, maps, beta, v, col, row, k, P, sig2, wvt, Ls, Ld, &sg);
+                const long pix = s.vix[k] + (long)NX * s.viy[k];
+                for (int n = 0; n < Q; n++) {
+                    double f = 0.0;
+                    for (int kk = 0; kk < K; kk++) f += coef[pix * K + kk] * basis[(long)kk * Q + n];
+                    F[n] = f;
+                }
+                double *h = H + (size_t)k * Q;
+                for (int e = 0; e < E; e++) {
+                    double sv = 0.0;
+                    const double we = Wt[e] * P[e];
+                    for (int n = 0; n < Q; n++) {
+                        const double dd = wvt[e] - s.qc[n];
+                        const double gq = exp(-dd * dd / (2.0 * sig2[e])) * s.dqc[n];
+                        sv += gq * F[n];
+                        h[n] += we * gq;
+                    }
+                    S[e] = sv;
+                }
+                if (GM) {
+                    for (int b = 0; b < B; b++) {
+                        double gb = 0.0;
+                        for (int e = 0; e < E; e++) gb -= Wt[e] * beta[(long)b * E + e] * P[e] * S[e];
+                        gam[b] = gb;
+                    }
+                    xo_line_scatter(GM, B, NX, NY, d->ray_step, sg.src_x, sg.src_y, s.vix[k] + 0.5,
+                                    s.viy[k] + 0.5, gam);
+                    xo_line_scatter(GM, B, NX, NY, d->ray_step, sg.det_x, sg.det_y, s.vix[k] + 0.5,
+                                    s.viy[k] + 0.5, gam);
+                }
+            }
+        }
+        free(P);
+    }
+    if (grad_coef) {
+        memset(grad_coef, 0, sizeof(double) * npix * K);
+        for (int k = 0; k < s.nvox; k++) {
+            const long pix = s.vix[k] + (long)NX * s.viy[k];
+            for (int kk = 0; kk < K; kk++) {
+                double acc = 0.0;
+                for (int n = 0; n < Q; n++) {
+                    double h = 0.0;
+                    for (int t = 0; t < nt; t++) h += hpart[((size_t)t * s.nvox + k) * Q + n];
+                    acc += basis[(long)kk * Q + n] * h;
+                }
+                grad_coef[pix * K + kk] = acc;
+            }
+        }
+    }
+    if (grad_maps) {
+        for (long i = 0; i < (long)B * npix; i++) {
+            double acc = 0.0;
+            for (int t = 0; t < nt; t++) acc += gpart[(size_t)t * B * npix + i];
+            grad_maps[i] = acc;
+        }
+    }
+    free(hpart);
+    free(gpart);
+    xo_cm_free(&s);
+    return XRD_OK;
+}

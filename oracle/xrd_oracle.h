@@ -98,6 +98,21 @@ void xo_mlem(const double *A, int64_t n_rows, int n_cols, const double *y, const
 double xo_loglik(const double *A, int64_t n_rows, int n_cols, const double *y,
                  const double *c, const double *x, int nthreads);
 
+/* Coefficient model (differentiable form of integrate_diffraction.cu:266-315):
+ * per-voxel form factors coef [nx*ny x K] (k fastest) on a basis [K x NWT] of the
+ * coarse q axis, attenuation mu_e = sum_b beta[b*NRG + e] maps[b*nx*ny + pix].
+ * Non-air voxels (seg > 0) scatter.  out_yhat: counts in the N layout (no Compton). */
+int xo_forward_coeffs(const xrd_geometry *g, const xrd_discretization *d,
+                      const double *spectrum, const int32_t *seg, const double *coef, int K,
+                      const double *basis, int B, const double *maps, const double *beta,
+                      double *out_yhat, int nthreads);
+/* Vector-Jacobian product of the coefficient model with upstream gradient w [rows]:
+ * grad_coef [nx*ny x K] and grad_maps [B x nx*ny] (either may be NULL). */
+int xo_vjp_coeffs(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                  const int32_t *seg, const double *coef, int K, const double *basis, int B,
+                  const double *maps, const double *beta, const double *w, double *grad_coef,
+                  double *grad_maps, int nthreads);
+
 /* Number of OpenMP threads the oracle would use for nthreads <= 0. */
 int xo_max_threads(void);
 

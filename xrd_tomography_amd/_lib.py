@@ -29,7 +29,7 @@ EXPORTS = (
     "xrd_last_error_string", "xrd_version", "xrd_forward", "xrd_trace_rays", "xrd_project",
     "xrd_backproject", "xrd_sensitivity", "xrd_convert_matrix", "xrd_mlem_plan_create",
     "xrd_mlem_plan_iterate", "xrd_mlem_plan_loglik", "xrd_mlem_plan_sensitivity",
-    "xrd_mlem_plan_destroy",
+    "xrd_mlem_plan_destroy", "xrd_forward_coeffs", "xrd_vjp_coeffs", "xrd_poisson_nll",
 )
 
 
@@ -70,6 +70,12 @@ def _declare(L):
         "xrd_mlem_plan_loglik": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl), _vp]),
         "xrd_mlem_plan_sensitivity": (_vp, [_vp]),
         "xrd_mlem_plan_destroy": (ctypes.c_int, [_vp]),
+        "xrd_forward_coeffs": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _i32, _vp, _i32, _vp,
+                                              _vp, _vp, _vp]),
+        "xrd_vjp_coeffs": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
+                                          _vp, _vp, _vp, _vp]),
+        "xrd_poisson_nll": (ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp,
+                                           ctypes.POINTER(_dbl), _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

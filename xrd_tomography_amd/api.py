@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes
 from typing import Optional, Tuple
 
+import numpy as np
 import torch  # loaded before libxrd_tomo.so so both share one HIP runtime
 
 from . import _lib
@@ -277,3 +278,61 @@ def mlem(A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, n_iter: int, x0: flo
     torch.cuda.current_stream().synchronize()
     plan.close()
     return x
+
+
+# ---- differentiable coefficient model ------------------------------------------------
+
+def forward_coeffs(spectrum, segmentation, coef, basis, maps, beta, *, geometry: Geometry,
+                   disc: Discretization, ctx=None, stream=None) -> torch.Tensor:
+    """Counts of the coefficient model (xrd_forward_coeffs): coef [nx*ny, K],
+    basis [K, NWT], maps [B, nx*ny], beta [B, NRG], float32 device tensors."""
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, segmentation, coef, basis, maps, beta)
+    g, d = geometry, disc
+    out = torch.empty((d.n_energy, g.n_rows, g.n_cols, g.n_views), dtype=torch.float32,
+                      device=spectrum.device)
+    check(lib().xrd_forward_coeffs(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                                   _ptr(segmentation), _ptr(coef), coef.shape[1], _ptr(basis),
+                                   maps.shape[0], _ptr(maps), _ptr(beta), _ptr(out),
+                                   _stream(stream)), "xrd_forward_coeffs")
+    return out
+
+
+def vjp_coeffs(spectrum, segmentation, coef, basis, maps, beta, grad_counts, *,
+               geometry: Geometry, disc: Discretization, want_coef=True, want_maps=True,
+               ctx=None, stream=None):
+    """(grad_coef [nx*ny, K] float64, grad_maps [B, nx*ny] float64) for the upstream
+    gradient grad_counts [rows] (xrd_vjp_coeffs)."""
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, segmentation, coef, basis, maps, beta, grad_counts)
+    dev = spectrum.device
+    gc = torch.empty(coef.shape, dtype=torch.float64, device=dev) if want_coef else None
+    gm = torch.empty(maps.shape, dtype=torch.float64, device=dev) if want_maps else None
+    g, d = geometry, disc
+    check(lib().xrd_vjp_coeffs(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                               _ptr(segmentation), _ptr(coef), coef.shape[1], _ptr(basis),
+                               maps.shape[0], _ptr(maps), _ptr(beta), _ptr(grad_counts),
+                               _ptr(gc), _ptr(gm), _stream(stream)), "xrd_vjp_coeffs")
+    return gc, gm
+
+
+def poisson_nll(counts, y, background=None, want_grad=True, ctx=None, stream=None):
+    """(loss: float, grad: float32 tensor 1 - y/f or None), f = counts + background."""
+    ctx = ctx or default_context(counts.device.index or 0)
+    _require_gpu(counts, y, background)
+    grad = torch.empty_like(counts) if want_grad else None
+    v = ctypes.c_double()
+    check(lib().xrd_poisson_nll(ctx.handle, counts.numel(), _ptr(counts), _ptr(background),
+                                _ptr(y), _ptr(grad), ctypes.byref(v), _stream(stream)),
+          "xrd_poisson_nll")
+    return v.value, grad
+
+
+def coeff_problem_tensors(cp, device="cuda"):
+    """Device tensors (spectrum, segmentation, coef, basis, maps, beta) of a
+    synthetic.CoeffProblem."""
+    p = cp.base
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dt)  # noqa: E731
+    return (t(p.flat("spectrum"), torch.float32), t(p.flat("segmentation"), torch.int32),
+            t(cp.coef, torch.float32), t(cp.basis, torch.float32), t(cp.maps, torch.float32),
+            t(cp.beta, torch.float32))

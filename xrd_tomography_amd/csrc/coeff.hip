@@ -1,0 +1,623 @@
+// coeff.hip -- differentiable coefficient model: forward counts and the
+// vector-Jacobian product with respect to the per-voxel form-factor
+// coefficients and the attenuation basis maps.
+//
+// Model (the differentiable form of integrate_diffraction.cu:266-315; Compton
+// is a constant background here):
+//   yhat[v,p,ed] = sum_vox sum_{e in win(ed)} RF[e,ed] P(e) S(e)
+//   P(e) = exp(-sum_b beta[b,e] (Ls_b + Ld_b)) g R / sqrt(2 pi sigma_e^2)
+//   S(e) = sum_n exp(-(wvt_e - q_n)^2 / (2 sigma_e^2)) dq_n F_vox(q_n),
+//   F_vox(q_n) = sum_k coef[vox,k] basis[k,n].
+// The reference is B = 2 with beta = (1/e^3, 1/f2(e)) (integrate_diffraction.cu:291-296);
+// a full per-energy attenuation map is B = NRG with beta = identity.
+//
+// The VJP recomputes every line integral (no per-ray storage).  Gradients with
+// respect to the coefficients are accumulated per voxel in registers
+// (deterministic); gradients with respect to the maps are scattered along the
+// rays with FP64 atomics (order-dependent only at the 1e-16 level).
+#include "xrd_device.hpp"
+#include "xrd_internal.hpp"
+
+namespace xrd {
+
+namespace {
+
+constexpr double kLog2e = 1.4426950408889634074;
+constexpr double kRs2pi = 0.39894228040143267794;
+constexpr int TQ = 128, G = 4, NT = TQ * G, EPT = 16, KC = 16;
+constexpr int MAXB = 64, MAXK = 32;
+
+struct VoxC {
+    double s, As, Bs, gR;
+    double sx, sy;   // unused by the forward, kept for the VJP source ray
+    int ix, iy, pix, k;
+};
+
+struct PixG {
+    double sp, cp, sx, sy, dx, dy, dz, ndx, ndy, nsx, nsy, nsz;
+};
+
+__device__ __forceinline__ PixG pixel_setup(const CoeffArgs &a, int v, int p) {
+#pragma clang fp contract(off)
+    PixG pg;
+    const int col = p % a.f.n_cols, row = p / a.f.n_cols;
+    const ViewConst vc = a.f.views[v];
+    pg.sp = vc.sp;
+    pg.cp = vc.cp;
+    pg.sx = vc.src_x;
+    pg.sy = vc.src_y;
+    const double rdet = ((double)col - (double)a.f.n_cols / 2.0 + 0.5) * a.f.col_pitch;
+    pg.dx = a.f.center_x - a.f.r_det * vc.sp + rdet * vc.cp;
+    pg.dy = a.f.center_y + a.f.r_det * vc.cp + rdet * vc.sp;
+    pg.dz = a.f.det_z0 + (double)row * a.f.row_pitch;
+    pg.ndx = vc.sp;
+    pg.ndy = -vc.cp;
+    pg.nsx = -vc.sp * a.f.sta;
+    pg.nsy = vc.cp * a.f.sta;
+    pg.nsz = -a.f.cta;
+    return pg;
+}
+
+// integrate_diffraction.cu:192-255 in FP64 (same order as the FP64 oracle)
+__device__ __forceinline__ void geom_c(const ForwardArgs &a, const PixG &pg, int ix, int iy,
+                                       VoxC &o) {
+#pragma clang fp contract(off)
+    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
+    const double ax = vx - pg.sx, ay = vy - pg.sy;
+    const double proj = (ax * pg.nsx + ay * pg.nsy) / sqrt(pg.nsx * pg.nsx + pg.nsy * pg.nsy);
+    const double vz = -proj * a.tan_w2;
+    const double hvox = proj * a.tan_w;
+    const double az = vz - 0.0;
+    const double bx = pg.dx - vx, by = pg.dy - vy, bz = pg.dz - vz;
+    const double ab = ax * bx + ay * by + az * bz;
+    const double an2 = ax * ax + ay * ay + az * az;
+    const double bn2 = bx * bx + by * by + bz * bz;
+    const double cth = ab * (1.0 / sqrt(an2)) * (1.0 / sqrt(bn2));
+    const double sth = sqrt(fabs(1.0 - cth) / 2.0);
+    const double Sx = bx - ab * ax / an2, Sy = by - ab * ay / an2, Sz = bz - ab * az / an2;
+    const double Dx = -ax + ab * bx / bn2, Dy = -ay + ab * by / bn2, Dz = -az + ab * bz / bn2;
+    const double Vx = -(Sx + Dx), Vy = -(Sy + Dy), Vz = -(Sz + Dz);
+    const double Dn = Dx * pg.ndx + Dy * pg.ndy + Dz * 0.0;
+    const double Sn = Sx * pg.nsx + Sy * pg.nsy + Sz * pg.nsz;
+    const double dD2 = (Dx * Dx + Dy * Dy + Dz * Dz - Dn * Dn) * a.adet / 12.0;
+    const double vV2 =
+        (a.voxel_size * a.voxel_size * (Vx * Vx + Vy * Vy) + hvox * hvox * Vz * Vz) / 12.0;
+    const double sS2 = (Sx * Sx + Sy * Sy + Sz * Sz - Sn * Sn) * a.focal_spot * a.focal_spot / 12.0;
+    const double rdotn = -pg.sp * ax + pg.cp * ay;
+    const double cospsi = rdotn * (1.0 / sqrt(ax * ax + ay * ay));
+    const double g_sv = cospsi * (1.0 / sqrt(an2)) * a.tan_w;
+    const double g_vd = -(pg.ndx * bx + pg.ndy * by) * (1.0 / sqrt(bn2 * bn2 * bn2));
+    const double k1 = 2.0 * sth / a.hbarc;
+    o.s = sth;
+    o.As = k1 * k1 * a.e_width * a.e_width / 12.0;
+    o.Bs = (sS2 + vV2 + dD2) / (2.0 * sth) / (2.0 * sth) / an2 / bn2;
+    o.gR = g_sv * g_vd * ((1.0 + cth * cth) / 2.0);
+}
+
+// Bilinear sample of padded planar map b at (px,py): zero border by index clamping.
+__device__ __forceinline__ float bil_planar(const float *__restrict__ pm, int nx, int ny,
+                                            double px, double py) {
+    const double xt = px - 0.5, yt = py - 0.5;
+    const double fx = floor(xt), fy = floor(yt);
+    const float ax = (float)(xt - fx), ay = (float)(yt - fy);
+    const int i0 = (int)fx, j0 = (int)fy;
+    const int pw = nx + 2;
+    const int i0c = min(max(i0, -1), nx) + 1, i1c = min(max(i0 + 1, -1), nx) + 1;
+    const int j0c = min(max(j0, -1), ny) + 1, j1c = min(max(j0 + 1, -1), ny) + 1;
+    const float bx = 1.f - ax, by = 1.f - ay;
+    return by * (bx * pm[i0c + pw * j0c] + ax * pm[i1c + pw * j0c]) +
+           ay * (bx * pm[i0c + pw * j1c] + ax * pm[i1c + pw * j1c]);
+}
+
+// Phase 1 for a chunk: lane t = j + KC*(bc + nbc*r) marches samples q = r (mod sp) of
+// the detector ray of voxel j for maps 2bc, 2bc+1; partial sums combined in LDS.
+__device__ __forceinline__ void trace_chunk(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
+                                            int nk, float2 *part, float *Ld) {
+    const int B = a.n_basis, nbc = (B + 1) / 2, sp = NT / (KC * nbc);
+    const int t = threadIdx.x;
+    const int j = t % KC, bc = (t / KC) % nbc, r = t / (KC * nbc);
+    const size_t plane = (size_t)(a.f.nx + 2) * (a.f.ny + 2);
+    if (j < nk && r < sp) {
+        const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
+                                 a.f.nx, a.f.ny, a.f.ray_step);
+        const int b0 = 2 * bc, b1 = 2 * bc + 1;
+        const float *m0 = a.maps_padded + (size_t)b0 * plane;
+        const float *m1 = a.maps_padded + (size_t)(b1 < B ? b1 : b0) * plane;
+        Kahan k0, k1;
+        for (int q = r; q < ray.n; q += sp) {
+            double px, py;
+            ray_sample(ray, a.f.ray_step, q, px, py);
+            k0.add(bil_planar(m0, a.f.nx, a.f.ny, px, py));
+            if (b1 < B) k1.add(bil_planar(m1, a.f.nx, a.f.ny, px, py));
+        }
+        part[t] = make_float2(k0.value(), k1.value());
+    }
+    __syncthreads();
+    for (int idx = t; idx < nk * B; idx += NT) {
+        const int jj = idx % nk, b = idx / nk;
+        const int bcc = b >> 1, comp = b & 1;
+        double s = 0.0;
+        for (int rr = 0; rr < sp; rr++) {
+            const float2 pv = part[jj + KC * (bcc + nbc * rr)];
+            s += (double)(comp ? pv.y : pv.x);
+        }
+        Ld[jj * MAXB + b] = (float)(s * a.f.ray_step);
+    }
+}
+
+// Per-(voxel, energy) table: hi/lo wvt, c2 = -log2e/(2 sigma^2), prefactor P, band.
+__device__ __forceinline__ void table_entry(const CoeffArgs &a, const VoxC &vs, const float *Ls,
+                                            const float *Ld, int e, float4 &tw, float &band,
+                                            double &P) {
+    const EnergyConst ec = a.f.energies[e];
+    double mu = 0.0;
+    for (int b = 0; b < a.n_basis; b++)
+        mu += (double)a.beta[b * a.f.n_energy + e] * ((double)Ls[b] + (double)Ld[b]);
+    const double sig2 = vs.As + vs.Bs * ec.k2sq;
+    const double w = ec.wcoef * vs.s;
+    const float wh = (float)w;
+    const float wl = (float)(w - (double)wh);
+    const double c2 = -kLog2e / (2.0 * sig2);
+    P = exp(-mu) * vs.gR * kRs2pi / sqrt(sig2);
+    tw = make_float4(wh, wl, (float)c2, (float)P);
+    band = (float)sqrt(64.0 / -c2);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(NT) void coeff_list_kernel(CoeffArgs a) {
+    __shared__ int wsum[16];
+    const int npix = a.f.nx * a.f.ny;
+    int base = 0;
+    for (int c = 0; c < npix; c += blockDim.x) {
+        const int ro = c + threadIdx.x;
+        bool f = false;
+        int pix = 0;
+        if (ro < npix) {
+            const int ix = ro / a.f.ny, iy = ro % a.f.ny;
+            pix = ix + a.f.nx * iy;
+            f = a.seg[pix] > 0;
+        }
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const unsigned long long m = __ballot(f);
+        const int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+            if (i < w) pre += wsum[i];
+            tot += wsum[i];
+        }
+        __syncthreads();
+        if (f) a.vlist[base + pre + below] = pix;
+        base += tot;
+    }
+    if (threadIdx.x == 0) a.nvox[0] = base;
+}
+
+__global__ void coeff_pad_kernel(CoeffArgs a) {
+    const int pw = a.f.nx + 2, ph = a.f.ny + 2;
+    const size_t plane = (size_t)pw * ph;
+    const size_t n = plane * a.n_basis;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i / plane);
+        const int q = (int)(i % plane);
+        const int px = q % pw, py = q / pw;
+        float v = 0.f;
+        if (px >= 1 && px <= a.f.nx && py >= 1 && py <= a.f.ny)
+            v = a.maps[(size_t)b * a.f.nx * a.f.ny + (px - 1) + (size_t)a.f.nx * (py - 1)];
+        a.maps_padded[i] = v;
+    }
+}
+
+// Source legs: one thread per (view, voxel), all B maps.
+__global__ __launch_bounds__(256) void coeff_src_kernel(CoeffArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (k >= a.nvox[0]) return;
+    const int pix = a.vlist[k];
+    const int ix = pix % a.f.nx, iy = pix / a.f.nx;
+    const ViewConst vc = a.f.views[v];
+    const Ray r = make_ray(vc.src_x, vc.src_y, (double)ix + 0.5, (double)iy + 0.5, a.f.nx, a.f.ny,
+                           a.f.ray_step);
+    const size_t plane = (size_t)(a.f.nx + 2) * (a.f.ny + 2);
+    const size_t npix = (size_t)a.f.nx * a.f.ny;
+    for (int b0 = 0; b0 < a.n_basis; b0 += 4) {
+        Kahan ks[4];
+        for (int q = 0; q < r.n; q++) {
+            double px, py;
+            ray_sample(r, a.f.ray_step, q, px, py);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (b0 + u < a.n_basis)
+                    ks[u].add(bil_planar(a.maps_padded + (size_t)(b0 + u) * plane, a.f.nx, a.f.ny,
+                                         px, py));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (b0 + u < a.n_basis)
+                a.src_ints[((size_t)v * npix + k) * a.n_basis + b0 + u] =
+                    (float)((double)ks[u].value() * a.f.ray_step);
+    }
+}
+
+// Forward counts: one workgroup per (view, pixel).
+__global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
+    __shared__ VoxC vox[KC];
+    __shared__ float2 part[NT];
+    __shared__ float Ld[KC * MAXB];
+    __shared__ float Fq[KC * TQ];
+    __shared__ float4 tw[KC * 64];
+    __shared__ float tb[KC * 64];
+    __shared__ double red[(NT / kWave) * EPT + 64];
+    const ForwardArgs &f = a.f;
+    const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
+    const int V = f.n_views, E = f.n_energy, Q = f.n_q;
+    const int v = blockIdx.x % V, p = blockIdx.x / V;
+    const size_t npix = (size_t)f.nx * f.ny;
+    const PixG pg = pixel_setup(a, v, p);
+    float qch = 1e30f, qcl = 0.f, dqc = 0.f;
+    if (tq < Q) {
+        const double q = f.q_coarse[tq];
+        qch = (float)q;
+        qcl = (float)(q - (double)qch);
+        dqc = (float)f.q_coarse[Q + tq];
+    }
+    float lo = tq < Q ? qch : 1e30f, hi = tq < Q ? qch : -1e30f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, kWave));
+        hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
+    }
+    float pf[EPT];
+    double ft[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        pf[i] = 0.f;
+        ft[i] = 0.0;
+    }
+    const int nv = a.nvox[0];
+    for (int k0 = 0; k0 < nv; k0 += KC) {
+        const int nk = min(KC, nv - k0);
+        if (t < nk) {
+            VoxC vs;
+            vs.k = k0 + t;
+            vs.pix = a.vlist[k0 + t];
+            vs.ix = vs.pix % f.nx;
+            vs.iy = vs.pix / f.nx;
+            geom_c(f, pg, vs.ix, vs.iy, vs);
+            vox[t] = vs;
+        }
+        __syncthreads();
+        trace_chunk(a, pg, vox, nk, part, Ld);
+        // F_vox(q_n) for the chunk
+        for (int idx = t; idx < nk * TQ; idx += NT) {
+            const int j = idx / TQ, n = idx % TQ;
+            float s = 0.f;
+            if (n < Q) {
+                const float *cf = a.coef + (size_t)vox[j].pix * a.n_coef;
+                for (int kk = 0; kk < a.n_coef; kk++) s = fmaf(cf[kk], a.basis[kk * Q + n], s);
+            }
+            Fq[j * TQ + n] = s;
+        }
+        __syncthreads();
+        for (int idx = t; idx < nk * E; idx += NT) {
+            const int j = idx / E, e = idx % E;
+            float4 tv;
+            float band;
+            double P;
+            table_entry(a, vox[j], a.src_ints + ((size_t)v * npix + vox[j].k) * a.n_basis,
+                        Ld + j * MAXB, e, tv, band, P);
+            tw[j * 64 + e] = tv;
+            tb[j * 64 + e] = band;
+        }
+        __syncthreads();
+        for (int j = 0; j < nk; j++) {
+            const float wq = dqc * Fq[j * TQ + tq];
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = h + G * i;
+                if (e >= E) continue;
+                const float4 tv = tw[j * 64 + e];
+                const float band = tb[j * 64 + e];
+                if (tv.x + band >= lo && tv.x - band <= hi) {
+                    const float d = (tv.x - qch) + (tv.y - qcl);
+                    pf[i] = fmaf(tv.w * wq, __builtin_amdgcn_exp2f(d * d * tv.z), pf[i]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            ft[i] += (double)pf[i];
+            pf[i] = 0.f;
+        }
+        __syncthreads();
+    }
+    const int lane = t & 63, wid = t >> 6;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const double sv = wave_sum(ft[i]);
+        if (lane == 0) red[wid * EPT + i] = sv;
+    }
+    __syncthreads();
+    double *hf = red + (NT / kWave) * EPT;
+    if (t < E) {
+        const int hh = t % G, i = t / G;
+        double s = 0.0;
+        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++) s += red[w * EPT + i];
+        hf[t] = s;
+    }
+    __syncthreads();
+    if (t < E) {
+        int wlo, whi;
+        src_window(t, E, f.dn, wlo, whi);
+        double s = 0.0;
+        for (int e = wlo; e <= whi; e++) s += (double)f.spectrum[e + (size_t)E * t] * hf[e];
+        a.out_counts[(size_t)v + (size_t)V * ((size_t)p + (size_t)f.n_det * t)] = (float)s;
+    }
+}
+
+// Scatter gamma[b] * step * (bilinear weights) of every sample of a ray into grad[b][pix].
+__device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, const double *gam,
+                                            int r, int sp) {
+    const int nx = a.f.nx, ny = a.f.ny, B = a.n_basis;
+    const size_t npix = (size_t)nx * ny;
+    for (int q = r; q < ray.n; q += sp) {
+        double px, py;
+        ray_sample(ray, a.f.ray_step, q, px, py);
+        const double xt = px - 0.5, yt = py - 0.5;
+        const double fx = floor(xt), fy = floor(yt);
+        const double ax = xt - fx, ay = yt - fy;
+        const int i0 = (int)fx, j0 = (int)fy;
+        const int ci[4] = {i0, i0 + 1, i0, i0 + 1}, cj[4] = {j0, j0, j0 + 1, j0 + 1};
+        const double wgt[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (ci[c] < 0 || cj[c] < 0 || ci[c] >= nx || cj[c] >= ny) continue;
+            const size_t pix = (size_t)ci[c] + (size_t)nx * cj[c];
+            const double ws = wgt[c] * a.f.ray_step;
+            for (int b = 0; b < B; b++)
+                if (gam[b] != 0.0) unsafeAtomicAdd(a.grad_maps + (size_t)b * npix + pix, gam[b] * ws);
+        }
+    }
+}
+
+// VJP: one workgroup per chunk of KC voxels, looping over every (view, pixel).
+__global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a) {
+    __shared__ VoxC vox[KC];
+    __shared__ float2 part[NT];
+    __shared__ float Ld[KC * MAXB];
+    __shared__ float Fq[KC * TQ];
+    __shared__ float4 tw[KC * 64];
+    __shared__ float tb[KC * 64];
+    __shared__ double wps[KC * 64];
+    __shared__ double gam[KC * MAXB];
+    __shared__ double gsrc[KC * MAXB];
+    __shared__ double wt[64];
+    const ForwardArgs &f = a.f;
+    const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
+    const int V = f.n_views, E = f.n_energy, Q = f.n_q, B = a.n_basis, P = f.n_det;
+    const size_t npix = (size_t)f.nx * f.ny;
+    const int nv = a.nvox[0];
+    const int k0 = blockIdx.x * KC;
+    if (k0 >= nv) return;  // uniform per block
+    const int nk = min(KC, nv - k0);
+    float qch = 1e30f, qcl = 0.f, dqc = 0.f;
+    if (tq < Q) {
+        const double q = f.q_coarse[tq];
+        qch = (float)q;
+        qcl = (float)(q - (double)qch);
+        dqc = (float)f.q_coarse[Q + tq];
+    }
+    float lo = tq < Q ? qch : 1e30f, hi = tq < Q ? qch : -1e30f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, kWave));
+        hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
+    }
+    // coefficients of the chunk are fixed: F_vox(q_n) once
+    for (int idx = t; idx < nk * TQ; idx += NT) {
+        const int j = idx / TQ, n = idx % TQ;
+        float s = 0.f;
+        if (n < Q) {
+            const int pix = a.vlist[k0 + j];
+            const float *cf = a.coef + (size_t)pix * a.n_coef;
+            for (int kk = 0; kk < a.n_coef; kk++) s = fmaf(cf[kk], a.basis[kk * Q + n], s);
+        }
+        Fq[j * TQ + n] = s;
+    }
+    double ht[KC];
+#pragma unroll
+    for (int j = 0; j < KC; j++) ht[j] = 0.0;
+    const int nbc = (B + 1) / 2, sp = NT / (KC * nbc);
+    (void)nbc;
+    for (int v = 0; v < V; v++) {
+        for (int idx = t; idx < KC * MAXB; idx += NT) gsrc[idx] = 0.0;
+        for (int p = 0; p < P; p++) {
+            const PixG pg = pixel_setup(a, v, p);
+            if (t < nk) {
+                VoxC vs;
+                vs.k = k0 + t;
+                vs.pix = a.vlist[k0 + t];
+                vs.ix = vs.pix % f.nx;
+                vs.iy = vs.pix / f.nx;
+                geom_c(f, pg, vs.ix, vs.iy, vs);
+                vox[t] = vs;
+            }
+            if (t < E) {
+                // transpose of the detector-response window
+                double s = 0.0;
+                for (int ed = 0; ed < E; ed++) {
+                    int wlo, whi;
+                    src_window(ed, E, f.dn, wlo, whi);
+                    if (t >= wlo && t <= whi)
+                        s += (double)f.spectrum[t + (size_t)E * ed] *
+                             (double)a.w[(size_t)v + (size_t)V * ((size_t)p + (size_t)P * ed)];
+                }
+                wt[t] = s;
+            }
+            __syncthreads();
+            trace_chunk(a, pg, vox, nk, part, Ld);
+            __syncthreads();
+            // table + S_e per (voxel, energy)
+            for (int idx = t; idx < nk * E; idx += NT) {
+                const int j = idx / E, e = idx % E;
+                float4 tv;
+                float band;
+                double Pe;
+                table_entry(a, vox[j], a.src_ints + ((size_t)v * npix + vox[j].k) * a.n_basis,
+                            Ld + j * MAXB, e, tv, band, Pe);
+                float s = 0.f;
+                for (int n = 0; n < Q; n++) {
+                    const double q = f.q_coarse[n];
+                    const float qh = (float)q, ql = (float)(q - (double)qh);
+                    const float d = (tv.x - qh) + (tv.y - ql);
+                    s = fmaf((float)f.q_coarse[Q + n] * Fq[j * TQ + n],
+                             __builtin_amdgcn_exp2f(d * d * tv.z), s);
+                }
+                wps[j * 64 + e] = wt[e] * Pe * (double)s;
+                tv.w = (float)(wt[e] * Pe);  // phase 3 weight
+                tw[j * 64 + e] = tv;
+                tb[j * 64 + e] = band;
+            }
+            __syncthreads();
+            // gamma_b = -sum_e beta[b,e] W(e) P(e) S(e)
+            for (int idx = t; idx < nk * B; idx += NT) {
+                const int j = idx / B, b = idx % B;
+                double g = 0.0;
+                for (int e = 0; e < E; e++) g -= (double)a.beta[b * E + e] * wps[j * 64 + e];
+                gam[j * MAXB + b] = g;
+                gsrc[j * MAXB + b] += g;
+            }
+            // phase 3: h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n
+#pragma unroll
+            for (int j = 0; j < KC; j++) {
+                if (j >= nk) break;
+                float hp = 0.f;
+                for (int i = 0; i < EPT; i++) {
+                    const int e = h + G * i;
+                    if (e >= E) break;
+                    const float4 tv = tw[j * 64 + e];
+                    const float band = tb[j * 64 + e];
+                    if (tv.x + band >= lo && tv.x - band <= hi) {
+                        const float d = (tv.x - qch) + (tv.y - qcl);
+                        hp = fmaf(tv.w, __builtin_amdgcn_exp2f(d * d * tv.z), hp);
+                    }
+                }
+                ht[j] += (double)(hp * dqc);
+            }
+            __syncthreads();
+            // scatter gamma along the detector legs
+            if (a.grad_maps) {
+                const int j = t % KC, r = t / KC;
+                if (j < nk) {
+                    const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
+                                             (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
+                    scatter_ray(a, ray, gam + j * MAXB, r, NT / KC);
+                }
+            }
+            __syncthreads();
+        }
+        // source legs of view v carry the sum of gamma over the pixels
+        if (a.grad_maps) {
+            const int j = t % KC, r = t / KC;
+            if (j < nk) {
+                const ViewConst vc = f.views[v];
+                const int pix = a.vlist[k0 + j];
+                const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
+                                         (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
+                scatter_ray(a, ray, gsrc + j * MAXB, r, NT / KC);
+            }
+        }
+        __syncthreads();
+    }
+    (void)sp;
+    // reduce h over the G energy groups (through a per-block global scratch), then
+    // contract with the basis: grad_coef[vox,k] = sum_n basis[k,n] h_vox(q_n)
+    for (int j = 0; j < nk; j++)
+        a.hscratch[((size_t)blockIdx.x * G + h) * KC * TQ + (size_t)j * TQ + tq] = ht[j];
+    __syncthreads();
+    if (a.grad_coef) {
+        for (int idx = t; idx < nk * a.n_coef; idx += NT) {
+            const int j = idx / a.n_coef, kk = idx % a.n_coef;
+            double s = 0.0;
+            for (int n = 0; n < Q; n++) {
+                double hv = 0.0;
+                for (int hh = 0; hh < G; hh++)
+                    hv += a.hscratch[((size_t)blockIdx.x * G + hh) * KC * TQ + (size_t)j * TQ + n];
+                s += (double)a.basis[kk * Q + n] * hv;
+            }
+            a.grad_coef[(size_t)a.vlist[k0 + j] * a.n_coef + kk] = s;
+        }
+    }
+}
+
+__global__ void zero_f64_kernel(double *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+hipError_t launch_zero_f64(double *p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// Poisson NLL sum(f - y log f), f = counts + background, and its gradient 1 - y/f.
+__global__ void poisson_nll_kernel(int64_t n, const float *cnt, const float *bg, const float *y,
+                                   float *grad, double *partial) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double f = (double)cnt[i] + (bg ? (double)bg[i] : 0.0);
+        const double yi = (double)y[i];
+        acc += f - (yi > 0.0 ? yi * log(f) : 0.0);
+        if (grad) grad[i] = (float)(1.0 - yi / f);
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, const float *y,
+                              float *grad, double *partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(poisson_nll_kernel, dim3(nblocks), dim3(256), 0, s, n, cnt, bg, y, grad,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(coeff_list_kernel, dim3(1), dim3(NT), 0, s, a);
+    const size_t n = (size_t)(a.f.nx + 2) * (a.f.ny + 2) * a.n_basis;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(coeff_pad_kernel, dim3(blocks), dim3(256), 0, s, a);
+    dim3 grid((a.f.nx * a.f.ny + 255) / 256, a.f.n_views);
+    hipLaunchKernelGGL(coeff_src_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(coeff_forward_kernel, dim3(a.f.n_views * a.f.n_det), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+int coeff_vjp_blocks(int npix) { return (npix + KC - 1) / KC; }
+size_t coeff_hscratch_doubles(int npix) {
+    return (size_t)coeff_vjp_blocks(npix) * G * KC * TQ;
+}
+
+hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
+    const int nb = coeff_vjp_blocks(a.f.nx * a.f.ny);
+    hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace xrd

@@ -68,6 +68,8 @@ struct xrd_context {
     DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables;
     // explicit-matrix partials
     DevBuf partial, small;
+    // coefficient-model workspace
+    DevBuf cm_maps, cm_src, cm_vlist, cm_h;
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -186,7 +188,8 @@ int xrd_context_destroy(xrd_context *c) {
     hipSetDevice(c->device);
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
-                      &c->src_int, &c->tables, &c->partial, &c->small})
+                      &c->src_int, &c->tables, &c->partial, &c->small, &c->cm_maps, &c->cm_src,
+                      &c->cm_vlist, &c->cm_h})
         b->release();
     if (c->pinned) hipHostFree(c->pinned);
     if (c->staged) hipEventDestroy(c->staged);
@@ -198,7 +201,8 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     if (!c) return 0;
     int64_t s = 0;
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
-                            &c->offsets, &c->src_int, &c->tables, &c->partial, &c->small})
+                            &c->offsets, &c->src_int, &c->tables, &c->partial, &c->small,
+                            &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -259,49 +263,13 @@ hipError_t stage_to_device(xrd_context *c, const void *host, size_t bytes, void 
 
 }  // namespace
 
-extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
-                           const float *spectrum, const float *ifine,
-                           const int32_t *segmentation, const float *a1, const float *a2,
-                           const float *s2_table, double lut_q_min, double lut_q_max,
-                           double lut_a_min, double lut_a_max, float *out_counts,
-                           float *out_compton, float *out_A, void *stream) {
-    const char *fn = "xrd_forward";
-    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
-    int rc = check_geometry(g, d, fn);
-    if (rc) return rc;
-    if (!spectrum || !ifine || !segmentation || !a1 || !a2 || !s2_table || !out_counts ||
-        !out_compton)
-        return fail(XRD_ERR_NULL_POINTER, "%s: a required input/output pointer is NULL", fn);
-    const void *ptrs[] = {spectrum, ifine, segmentation, a1, a2, s2_table, out_counts,
-                          out_compton, out_A};
-    for (const void *p : ptrs)
-        if (p && !aligned(p, 16)) return fail(XRD_ERR_ALIGN, "%s: pointers must be 16-byte aligned", fn);
-    const double lims[] = {lut_q_min, lut_q_max, lut_a_min, lut_a_max};
-    for (double v : lims)
-        if (!std::isfinite(v)) return fail(XRD_ERR_NONFINITE, "%s: non-finite LUT limit", fn);
-    if (!(lut_q_max > lut_q_min) || !(lut_a_max > lut_a_min))
-        return fail(XRD_ERR_INVALID_ARG, "%s: LUT limits must satisfy max > min", fn);
-    if ((int64_t)d->n_q * d->n_mat * (int64_t)g->n_views * g->n_cols * g->n_rows * d->n_energy >
-        (int64_t)1 << 40)
-        return fail(XRD_ERR_DIMS, "%s: model matrix too large", fn);
-    if (d->n_energy > 64 || d->n_q > 256 || d->n_qf > 768)
-        return fail(XRD_ERR_UNSUPPORTED,
-                    "%s: this build supports n_energy <= 64, n_q <= 256, n_qf <= 768", fn);
-    hipStream_t s = (hipStream_t)stream;
-    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+namespace {
 
-    const int nx = g->nx, ny = g->ny, V = g->n_views, E = d->n_energy, Q = d->n_q;
-    const size_t npix = (size_t)nx * ny;
-    const size_t npad = (size_t)(nx + 2) * (ny + 2);
-    HIP_TRY(ctx->mu_padded.ensure(npad * sizeof(float2)), "workspace alloc");
-    HIP_TRY(ctx->s2u8.ensure((size_t)d->lut_nq * d->lut_na), "workspace alloc");
-    HIP_TRY(ctx->cpx.ensure(2 * npix * sizeof(float)), "workspace alloc");
-    HIP_TRY(ctx->ref_list.ensure(npix * sizeof(int)), "workspace alloc");
-    HIP_TRY(ctx->voxels.ensure(npix * sizeof(VoxelEntry)), "workspace alloc");
-    HIP_TRY(ctx->offsets.ensure((size_t)(d->n_mat + 1) * sizeof(int)), "workspace alloc");
-    HIP_TRY(ctx->src_int.ensure((size_t)V * npix * sizeof(float2)), "workspace alloc");
-
-    // host-computed tables: views, energies, coarse q axis (FP64, host libm)
+// Host FP64 tables (views, energies, coarse q axis) staged to the context and the
+// geometry/discretisation fields of ForwardArgs shared by all forward-type calls.
+int setup_forward_common(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                         const float *spectrum, ForwardArgs &fa, hipStream_t s) {
+    const int V = g->n_views, E = d->n_energy, Q = d->n_q;
     const size_t vbytes = (size_t)V * sizeof(ViewConst);
     const size_t ebytes = (size_t)E * sizeof(EnergyConst);
     const size_t qbytes = (size_t)(2 * Q + 1) * sizeof(double);
@@ -344,28 +312,9 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
     HIP_TRY(ctx->tables.ensure(host.size()), "workspace alloc");
     HIP_TRY(stage_to_device(ctx, host.data(), host.size(), ctx->tables.ptr, s), "stage tables");
 
-    PrepArgs pa{};
-    pa.nx = nx;
-    pa.ny = ny;
-    pa.n_mat = d->n_mat;
-    pa.flags = d->flags;
-    pa.seg = segmentation;
-    pa.a1 = a1;
-    pa.a2 = a2;
-    pa.s2_table = s2_table;
-    pa.s2_count = d->lut_nq * d->lut_na;
-    pa.s2u8 = (uint8_t *)ctx->s2u8.ptr;
-    pa.mu_padded = (float2 *)ctx->mu_padded.ptr;
-    pa.cpx1 = (float *)ctx->cpx.ptr;
-    pa.cpx2 = (float *)ctx->cpx.ptr + npix;
-    pa.ref_list = (int *)ctx->ref_list.ptr;
-    pa.voxels = (VoxelEntry *)ctx->voxels.ptr;
-    pa.mat_offsets = (int *)ctx->offsets.ptr;
-    HIP_TRY(launch_prepare(pa, s), "prepare kernels");
-
-    ForwardArgs fa{};
-    fa.nx = nx;
-    fa.ny = ny;
+    fa = ForwardArgs{};
+    fa.nx = g->nx;
+    fa.ny = g->ny;
     fa.n_views = V;
     fa.n_cols = g->n_cols;
     fa.n_rows = g->n_rows;
@@ -401,20 +350,89 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
     fa.kn2 = d->kn2;
     fa.ray_step = d->ray_step;
     fa.dq_fine = (d->q_max - d->q_min) / (double)d->n_qf;
+    fa.spectrum = spectrum;
+    fa.views = (const ViewConst *)ctx->tables.ptr;
+    fa.energies = (const EnergyConst *)((unsigned char *)ctx->tables.ptr + vbytes);
+    fa.q_coarse = (const double *)((unsigned char *)ctx->tables.ptr + vbytes + ebytes);
+    return XRD_OK;
+}
+
+}  // namespace
+
+extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                           const float *spectrum, const float *ifine,
+                           const int32_t *segmentation, const float *a1, const float *a2,
+                           const float *s2_table, double lut_q_min, double lut_q_max,
+                           double lut_a_min, double lut_a_max, float *out_counts,
+                           float *out_compton, float *out_A, void *stream) {
+    const char *fn = "xrd_forward";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_geometry(g, d, fn);
+    if (rc) return rc;
+    if (!spectrum || !ifine || !segmentation || !a1 || !a2 || !s2_table || !out_counts ||
+        !out_compton)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required input/output pointer is NULL", fn);
+    const void *ptrs[] = {spectrum, ifine, segmentation, a1, a2, s2_table, out_counts,
+                          out_compton, out_A};
+    for (const void *p : ptrs)
+        if (p && !aligned(p, 16)) return fail(XRD_ERR_ALIGN, "%s: pointers must be 16-byte aligned", fn);
+    const double lims[] = {lut_q_min, lut_q_max, lut_a_min, lut_a_max};
+    for (double v : lims)
+        if (!std::isfinite(v)) return fail(XRD_ERR_NONFINITE, "%s: non-finite LUT limit", fn);
+    if (!(lut_q_max > lut_q_min) || !(lut_a_max > lut_a_min))
+        return fail(XRD_ERR_INVALID_ARG, "%s: LUT limits must satisfy max > min", fn);
+    if ((int64_t)d->n_q * d->n_mat * (int64_t)g->n_views * g->n_cols * g->n_rows * d->n_energy >
+        (int64_t)1 << 40)
+        return fail(XRD_ERR_DIMS, "%s: model matrix too large", fn);
+    if (d->n_energy > 64 || d->n_q > 256 || d->n_qf > 768)
+        return fail(XRD_ERR_UNSUPPORTED,
+                    "%s: this build supports n_energy <= 64, n_q <= 256, n_qf <= 768", fn);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+
+    const int nx = g->nx, ny = g->ny, V = g->n_views;
+    const size_t npix = (size_t)nx * ny;
+    const size_t npad = (size_t)(nx + 2) * (ny + 2);
+    HIP_TRY(ctx->mu_padded.ensure(npad * sizeof(float2)), "workspace alloc");
+    HIP_TRY(ctx->s2u8.ensure((size_t)d->lut_nq * d->lut_na), "workspace alloc");
+    HIP_TRY(ctx->cpx.ensure(2 * npix * sizeof(float)), "workspace alloc");
+    HIP_TRY(ctx->ref_list.ensure(npix * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->voxels.ensure(npix * sizeof(VoxelEntry)), "workspace alloc");
+    HIP_TRY(ctx->offsets.ensure((size_t)(d->n_mat + 1) * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->src_int.ensure((size_t)V * npix * sizeof(float2)), "workspace alloc");
+    ForwardArgs fa;
+    rc = setup_forward_common(ctx, g, d, spectrum, fa, s);
+    if (rc) return rc;
+
+    PrepArgs pa{};
+    pa.nx = nx;
+    pa.ny = ny;
+    pa.n_mat = d->n_mat;
+    pa.flags = d->flags;
+    pa.seg = segmentation;
+    pa.a1 = a1;
+    pa.a2 = a2;
+    pa.s2_table = s2_table;
+    pa.s2_count = d->lut_nq * d->lut_na;
+    pa.s2u8 = (uint8_t *)ctx->s2u8.ptr;
+    pa.mu_padded = (float2 *)ctx->mu_padded.ptr;
+    pa.cpx1 = (float *)ctx->cpx.ptr;
+    pa.cpx2 = (float *)ctx->cpx.ptr + npix;
+    pa.ref_list = (int *)ctx->ref_list.ptr;
+    pa.voxels = (VoxelEntry *)ctx->voxels.ptr;
+    pa.mat_offsets = (int *)ctx->offsets.ptr;
+    HIP_TRY(launch_prepare(pa, s), "prepare kernels");
+
+    fa.ifine = ifine;
     fa.lut_q_min = lut_q_min;
     fa.lut_a_min = lut_a_min;
     fa.dq_lut = (lut_q_max - lut_q_min) / (double)d->lut_nq;
     fa.da_lut = (lut_a_max - lut_a_min) / (double)d->lut_na;
-    fa.spectrum = spectrum;
-    fa.ifine = ifine;
     fa.s2u8 = (const uint8_t *)ctx->s2u8.ptr;
     fa.mu_padded = (const float2 *)ctx->mu_padded.ptr;
     fa.voxels = (const VoxelEntry *)ctx->voxels.ptr;
     fa.mat_offsets = (const int *)ctx->offsets.ptr;
     fa.src_int = (const float2 *)ctx->src_int.ptr;
-    fa.views = (const ViewConst *)ctx->tables.ptr;
-    fa.energies = (const EnergyConst *)((unsigned char *)ctx->tables.ptr + vbytes);
-    fa.q_coarse = (const double *)((unsigned char *)ctx->tables.ptr + vbytes + ebytes);
     fa.out_n = out_counts;
     fa.out_c = out_compton;
     fa.out_a = out_A;
@@ -651,5 +669,120 @@ extern "C" int xrd_mlem_plan_destroy(xrd_mlem_plan *p) {
     p->r.release();
     p->llp.release();
     delete p;
+    return XRD_OK;
+}
+
+// ---- differentiable coefficient model ---------------------------------------------
+
+namespace {
+
+int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                const float *spectrum, const int32_t *segmentation, const float *coef,
+                int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,
+                const float *beta, CoeffArgs &ca, hipStream_t s, const char *fn) {
+    int rc = check_geometry(g, d, fn);
+    if (rc) return rc;
+    if (!spectrum || !segmentation || !coef || !basis || !maps || !beta)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required input pointer is NULL", fn);
+    if (n_coef <= 0 || n_coef > 32 || n_basis <= 0 || n_basis > 64)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: need 1 <= n_coef <= 32 and 1 <= n_basis <= 64", fn);
+    if (d->n_energy > 64 || d->n_q > 128)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: need n_energy <= 64 and n_q <= 128", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t npix = (size_t)g->nx * g->ny;
+    const size_t npad = (size_t)(g->nx + 2) * (g->ny + 2);
+    HIP_TRY(ctx->cm_maps.ensure(npad * n_basis * sizeof(float)), "workspace alloc");
+    HIP_TRY(ctx->cm_src.ensure((size_t)g->n_views * npix * n_basis * sizeof(float)), "workspace alloc");
+    HIP_TRY(ctx->cm_vlist.ensure((npix + 4) * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->cm_h.ensure(coeff_hscratch_doubles((int)npix) * sizeof(double)), "workspace alloc");
+    ForwardArgs fa;
+    rc = setup_forward_common(ctx, g, d, spectrum, fa, s);
+    if (rc) return rc;
+    ca = CoeffArgs{};
+    ca.f = fa;
+    ca.seg = segmentation;
+    ca.coef = coef;
+    ca.n_coef = n_coef;
+    ca.basis = basis;
+    ca.n_basis = n_basis;
+    ca.maps = maps;
+    ca.beta = beta;
+    ca.maps_padded = (float *)ctx->cm_maps.ptr;
+    ca.src_ints = (float *)ctx->cm_src.ptr;
+    ca.vlist = (int *)ctx->cm_vlist.ptr;
+    ca.nvox = (int *)ctx->cm_vlist.ptr + npix;
+    ca.hscratch = (double *)ctx->cm_h.ptr;
+    HIP_TRY(launch_coeff_prepare(ca, s), "coefficient-model preparation");
+    return XRD_OK;
+}
+
+}  // namespace
+
+extern "C" int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g,
+                                  const xrd_discretization *d, const float *spectrum,
+                                  const int32_t *segmentation, const float *coef,
+                                  int32_t n_coef, const float *basis, int32_t n_basis,
+                                  const float *maps, const float *beta, float *out_counts,
+                                  void *stream) {
+    const char *fn = "xrd_forward_coeffs";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!out_counts) return fail(XRD_ERR_NULL_POINTER, "%s: out_counts is NULL", fn);
+    hipStream_t s = (hipStream_t)stream;
+    CoeffArgs ca;
+    int rc = coeff_setup(ctx, g, d, spectrum, segmentation, coef, n_coef, basis, n_basis, maps,
+                         beta, ca, s, fn);
+    if (rc) return rc;
+    ca.out_counts = out_counts;
+    HIP_TRY(launch_coeff_forward(ca, s), "coefficient forward kernel");
+    return XRD_OK;
+}
+
+extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
+                              const xrd_discretization *d, const float *spectrum,
+                              const int32_t *segmentation, const float *coef, int32_t n_coef,
+                              const float *basis, int32_t n_basis, const float *maps,
+                              const float *beta, const float *grad_counts, double *grad_coef,
+                              double *grad_maps, void *stream) {
+    const char *fn = "xrd_vjp_coeffs";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!grad_counts) return fail(XRD_ERR_NULL_POINTER, "%s: grad_counts is NULL", fn);
+    if (!grad_coef && !grad_maps)
+        return fail(XRD_ERR_NULL_POINTER, "%s: grad_coef and grad_maps are both NULL", fn);
+    hipStream_t s = (hipStream_t)stream;
+    CoeffArgs ca;
+    int rc = coeff_setup(ctx, g, d, spectrum, segmentation, coef, n_coef, basis, n_basis, maps,
+                         beta, ca, s, fn);
+    if (rc) return rc;
+    const size_t npix = (size_t)g->nx * g->ny;
+    ca.w = grad_counts;
+    ca.grad_coef = grad_coef;
+    ca.grad_maps = grad_maps;
+    if (grad_coef) HIP_TRY(launch_zero_f64(grad_coef, npix * n_coef, s), "zero kernel");
+    if (grad_maps) HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
+    HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
+    return XRD_OK;
+}
+
+extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
+                               const float *background, const float *y, float *grad_counts,
+                               double *out_loss_host, void *stream) {
+    const char *fn = "xrd_poisson_nll";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (n <= 0) return fail(XRD_ERR_DIMS, "%s: n must be > 0", fn);
+    if (!counts || !y || !out_loss_host)
+        return fail(XRD_ERR_NULL_POINTER, "%s: counts/y/out_loss_host is NULL", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = 256;
+    HIP_TRY(ctx->small.ensure(nb * sizeof(double)), "workspace alloc");
+    HIP_TRY(launch_poisson_nll(n, counts, background, y, grad_counts, (double *)ctx->small.ptr,
+                               nb, s), "nll kernel");
+    std::vector<double> h(nb);
+    HIP_TRY(hipMemcpyAsync(h.data(), ctx->small.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double t = 0.0;
+    for (double v : h) t += v;
+    *out_loss_host = t;
     return XRD_OK;
 }

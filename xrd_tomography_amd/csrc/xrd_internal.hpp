@@ -83,6 +83,35 @@ hipError_t launch_trace_rays(int nx, int ny, double step, int64_t n, const doubl
                              int32_t *texels, int32_t n_basis, const float *maps, float *ints,
                              hipStream_t s);
 
+// ---- differentiable coefficient model ----------------------------------------------
+struct CoeffArgs {
+    ForwardArgs f;           // geometry, discretisation, host tables, spectrum
+    const int32_t *seg;      // [nx*ny]; voxels with seg > 0 scatter
+    const float *coef;       // [nx*ny x K] (k fastest)
+    int n_coef;              // K <= 32
+    const float *basis;      // [K x NWT]
+    int n_basis;             // B <= 64
+    const float *maps;       // [B x nx*ny]
+    const float *beta;       // [B x NRG]
+    float *maps_padded;      // [B x (nx+2)(ny+2)] workspace
+    float *src_ints;         // [V x nx*ny x B] workspace
+    int *vlist;              // [nx*ny] workspace
+    int *nvox;               // [1] workspace
+    const float *w;          // upstream gradient [rows]
+    float *out_counts;       // [rows]
+    double *grad_coef;       // [nx*ny x K] or NULL
+    double *grad_maps;       // [B x nx*ny] or NULL (zeroed by the caller)
+    double *hscratch;        // coeff_hscratch_doubles(nx*ny)
+};
+
+hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
+hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
+hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
+size_t coeff_hscratch_doubles(int npix);
+hipError_t launch_zero_f64(double *p, size_t n, hipStream_t s);
+hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, const float *y,
+                              float *grad, double *partial, int nblocks, hipStream_t s);
+
 // ---- explicit-matrix kernels -------------------------------------------------------
 struct MatArgs {
     const void *A;

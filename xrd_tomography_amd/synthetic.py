@@ -303,3 +303,60 @@ def poisson_counts(expected: np.ndarray, seed: int) -> np.ndarray:
     """Poisson noise (the poissrnd of main.m:141), seeded."""
     rng = np.random.Generator(np.random.PCG64(seed))
     return rng.poisson(np.clip(expected, 0.0, None)).astype(np.float32)
+
+
+@dataclasses.dataclass
+class CoeffProblem:
+    """Inputs of the differentiable coefficient model (per-voxel form factors on a
+    Gaussian-bump basis of the coarse q axis, attenuation from B basis maps)."""
+    base: Problem
+    coef: np.ndarray   # [NX*NY, K] float32 (k fastest), zero on air voxels
+    basis: np.ndarray  # [K, NWT] float32
+    maps: np.ndarray   # [B, NX*NY] float32, 1/voxel
+    beta: np.ndarray   # [B, NRG] float32
+
+    @property
+    def K(self) -> int:
+        return self.coef.shape[1]
+
+    @property
+    def B(self) -> int:
+        return self.maps.shape[0]
+
+
+def gaussian_bump_basis(d: Discretization, K: int, rng) -> np.ndarray:
+    """K seeded Gaussian bumps over the coarse q axis [K, NWT]."""
+    q, dq, _, _ = q_axis(d)
+    q = np.asarray(q)
+    centres = np.linspace(d.q_min, d.q_max, K) + rng.uniform(-0.2, 0.2, K) * (d.q_max - d.q_min) / K
+    width = (d.q_max - d.q_min) / K * rng.uniform(0.6, 1.2, K)
+    return np.exp(-0.5 * ((q[None, :] - centres[:, None]) / width[:, None]) ** 2)
+
+
+def make_coeff_problem(prob: Problem, K: int = 16, full_mu: bool = True,
+                       seed: int = 21) -> CoeffProblem:
+    """Coefficient model on top of `prob`: per-voxel coefficients = material profile
+    + 20% seeded noise; attenuation either as the full per-energy map (B = NRG,
+    beta = identity) or the reference's two-basis form (B = 2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = prob.disc
+    basis = gaussian_bump_basis(d, K, rng)
+    npix = prob.geometry.nx * prob.geometry.ny
+    seg = prob.flat("segmentation")
+    mat_coef = rng.lognormal(0.0, 1.0, (d.n_mat, K)) * 0.05
+    coef = np.zeros((npix, K))
+    live = seg > 0
+    coef[live] = mat_coef[seg[live] - 1] * rng.uniform(0.8, 1.2, (int(live.sum()), K))
+    E = d.n_energy
+    centres = d.e_min + d.e_width * (np.arange(E) + 0.5)
+    e = centres / d.mc2
+    b1, b2 = 1.0 / e ** 3, 1.0 / f2_kn(e, d)
+    a1, a2 = prob.flat("a1").astype(np.float64), prob.flat("a2").astype(np.float64)
+    if full_mu:
+        maps = b1[:, None] * a1[None, :] + b2[:, None] * a2[None, :]
+        beta = np.eye(E)
+    else:
+        maps = np.stack([a1, a2])
+        beta = np.stack([b1, b2])
+    return CoeffProblem(base=prob, coef=coef.astype(np.float32), basis=basis.astype(np.float32),
+                        maps=maps.astype(np.float32), beta=beta.astype(np.float32))
