@@ -231,6 +231,14 @@ int xrd_sensitivity(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows
 int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, int out_dtype, void *out,
                        void *stream);
 
+/* Convert with a per-column power-of-two storage scale (the per-material scale of
+ * the reduced-precision path): out[r,j] = round(A[r,j] * col_scale[j]), where
+ * col_scale (device, n_cols floats, written here) maps each column's maximum to
+ * about half the dtype's largest finite value.  Pass col_scale to
+ * xrd_mlem_plan_create to iterate on the scaled matrix. */
+int xrd_convert_matrix_scaled(xrd_context *ctx, const float *A, int64_t n_rows, int32_t n_cols,
+                              int out_dtype, void *out, float *col_scale, void *stream);
+
 /* ---- MLEM (main.m:141-176) ----------------------------------------------------
  * x <- x .* A'( y ./ (A x + c + 1e-20) ) ./ (sum(A,1)' + 1e-7); non-finite -> 0; x<0 -> 0.
  * The plan keeps pointers to A, y, c and x (caller-owned, must outlive the plan),
@@ -241,9 +249,11 @@ typedef struct xrd_mlem_plan xrd_mlem_plan;
 #define XRD_MLEM_FUSED    0   /* one pass over A per iteration (row dot, ratio, row axpy) */
 #define XRD_MLEM_UNFUSED  1   /* plain: A x, then A' r as two passes over A               */
 
+/* col_scale: NULL, or the per-column storage scale of A (xrd_convert_matrix_scaled). */
 int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
-                         int32_t n_cols, const float *y, const float *c, float *x,
-                         int mode, void *stream, xrd_mlem_plan **out);
+                         int32_t n_cols, const float *col_scale, const float *y,
+                         const float *c, float *x, int mode, void *stream,
+                         xrd_mlem_plan **out);
 /* Run n_iter updates of x in place.  use_graph != 0 replays a captured hipGraph of
  * one iteration (captured on first use). */
 int xrd_mlem_plan_iterate(xrd_mlem_plan *plan, int32_t n_iter, int use_graph, void *stream);

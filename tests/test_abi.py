@@ -82,7 +82,7 @@ def test_error_paths_without_device(L):
     assert L.xrd_geometry_reference(0, 16, 4, 8, 1, ctypes.byref(g)) == -3
     assert L.xrd_discretization_reference(16, 8, 24, 2, None) == -2
     assert L.xrd_project(None, None, 0, 10, 8, None, None, None) == -2
-    assert L.xrd_mlem_plan_create(None, None, 0, 10, 8, None, None, None, 0, None,
+    assert L.xrd_mlem_plan_create(None, None, 0, 10, 8, None, None, None, None, 0, None,
                                   ctypes.byref(ctypes.c_void_p())) == -2
     assert L.xrd_mlem_plan_iterate(None, 1, 0, None) == -2
     assert L.xrd_trace_rays(None, 4, 4, 0.25, 1, None, None, None, None, 0, None, 0, None,

@@ -452,3 +452,21 @@ def test_vjp_central_differences(oracle):
                            oracle.forward_coeffs(cp, maps=mm), y, c) / (2 * h)
             assert abs(fd - gm[b, pix]) <= 1e-6 * abs(gm[b, pix]), (b, pix, fd, gm[b, pix])
     assert npix == cp.base.geometry.nx * cp.base.geometry.ny
+
+
+# ---------------------------------------------------------------------------
+# committed FP64 fixtures (tests/golden/make_golden.py)
+
+@pytest.mark.parametrize("name,flags", [("reference", 3), ("small", 0)])
+def test_oracle_reproduces_golden(oracle, name, flags):
+    import os
+    from xrd_tomography_amd.synthetic import make_problem
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}_flags{flags}.npz"))
+    p = make_problem(name, flags=flags)
+    assert np.array_equal(p.flat("spectrum"), gold["spectrum"])      # generator unchanged
+    assert np.array_equal(p.flat("segmentation"), gold["segmentation"])
+    N, C, A = oracle.forward(p, nthreads=3)
+    assert np.allclose(N, gold["N"], rtol=1e-12, atol=0)
+    assert np.allclose(C, gold["C"], rtol=1e-12, atol=0)
+    assert np.allclose(A[::p.geometry.n_views], gold["A_view0"], rtol=1e-12, atol=0)
+    assert np.allclose(A.sum(0), gold["A_colsum"], rtol=1e-12, atol=0)

@@ -30,6 +30,7 @@ EXPORTS = (
     "xrd_backproject", "xrd_sensitivity", "xrd_convert_matrix", "xrd_mlem_plan_create",
     "xrd_mlem_plan_iterate", "xrd_mlem_plan_loglik", "xrd_mlem_plan_sensitivity",
     "xrd_mlem_plan_destroy", "xrd_forward_coeffs", "xrd_vjp_coeffs", "xrd_poisson_nll",
+    "xrd_convert_matrix_scaled",
 )
 
 
@@ -65,7 +66,10 @@ def _declare(L):
         "xrd_sensitivity": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp, _vp]),
         "xrd_convert_matrix": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _vp, _vp]),
         "xrd_mlem_plan_create": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp, _vp,
-                                                _vp, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+                                                _vp, _vp, ctypes.c_int, _vp,
+                                                ctypes.POINTER(_vp)]),
+        "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
+                                                     _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),
         "xrd_mlem_plan_loglik": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl), _vp]),
         "xrd_mlem_plan_sensitivity": (_vp, [_vp]),

@@ -210,7 +210,8 @@ class MlemPlan:
     x is updated in place; y, c and A must outlive the plan."""
 
     def __init__(self, A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, x: torch.Tensor,
-                 fused: bool = True, ctx=None, stream=None):
+                 fused: bool = True, col_scale: Optional[torch.Tensor] = None, ctx=None,
+                 stream=None):
         self.ctx = ctx or default_context(A.device.index or 0)
         dt, nr, nc = _mat(A)
         _require_gpu(y, c, x)
@@ -219,11 +220,12 @@ class MlemPlan:
         for t in (y, c, x):
             if t.dtype != torch.float32:
                 raise TypeError("y, c and x must be float32")
-        self._keep = (A, y, c, x)
+        self._keep = (A, y, c, x, col_scale)
         self.x = x
         h = ctypes.c_void_p()
-        check(lib().xrd_mlem_plan_create(self.ctx.handle, _ptr(A), dt, nr, nc, _ptr(y), _ptr(c),
-                                         _ptr(x), MLEM_FUSED if fused else MLEM_UNFUSED,
+        check(lib().xrd_mlem_plan_create(self.ctx.handle, _ptr(A), dt, nr, nc, _ptr(col_scale),
+                                         _ptr(y), _ptr(c), _ptr(x),
+                                         MLEM_FUSED if fused else MLEM_UNFUSED,
                                          _stream(stream), ctypes.byref(h)),
               "xrd_mlem_plan_create")
         self.handle = h
@@ -269,11 +271,24 @@ def _wrap_device_ptr(ptr: int, n: int, dtype, device):
     return torch.as_tensor(h, device=device)
 
 
+def convert_matrix_scaled(A: torch.Tensor, dtype: torch.dtype, ctx=None, stream=None):
+    """(A16, col_scale): BF16/F16 storage of A*col_scale with power-of-two per-column
+    scales chosen by the library (xrd_convert_matrix_scaled)."""
+    ctx = ctx or default_context(A.device.index or 0)
+    _require_gpu(A)
+    out = torch.empty(A.shape, dtype=dtype, device=A.device)
+    sc = torch.empty(A.shape[1], dtype=torch.float32, device=A.device)
+    check(lib().xrd_convert_matrix_scaled(ctx.handle, _ptr(A), A.shape[0], A.shape[1],
+                                          _DTYPES[dtype], _ptr(out), _ptr(sc), _stream(stream)),
+          "xrd_convert_matrix_scaled")
+    return out, sc
+
+
 def mlem(A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, n_iter: int, x0: float = 10.0,
-         fused: bool = True, use_graph: bool = False, ctx=None) -> torch.Tensor:
+         fused: bool = True, use_graph: bool = False, col_scale=None, ctx=None) -> torch.Tensor:
     """n_iter MLEM updates from x = x0 (main.m:154) on device; returns x (float32)."""
     x = torch.full((A.shape[1],), float(x0), dtype=torch.float32, device=A.device)
-    plan = MlemPlan(A, y, c, x, fused=fused, ctx=ctx)
+    plan = MlemPlan(A, y, c, x, fused=fused, col_scale=col_scale, ctx=ctx)
     plan.iterate(n_iter, use_graph=use_graph)
     torch.cuda.current_stream().synchronize()
     plan.close()

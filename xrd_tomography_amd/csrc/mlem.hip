@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int6
                                                              int n_cols, int64_t rpb,
                                                              const float *x, const float *y,
                                                              const float *c, const float *w,
-                                                             double *partial) {
+                                                             const float *scale, double *partial) {
     __shared__ double red[kMatWaves][NG * 256];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
@@ -89,7 +89,13 @@ __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int6
         const int qi = lane + 64 * g;
         gv[g] = qi < nq;
         xv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (MODE == kColsMlem && gv[g]) xv[g] = *reinterpret_cast<const float4 *>(x + 4 * qi);
+        if (MODE == kColsMlem && gv[g]) {
+            xv[g] = *reinterpret_cast<const float4 *>(x + 4 * qi);
+            if (scale) {
+                const float4 sc = *reinterpret_cast<const float4 *>(scale + 4 * qi);
+                xv[g] = make_float4(xv[g].x / sc.x, xv[g].y / sc.y, xv[g].z / sc.z, xv[g].w / sc.w);
+            }
+        }
     }
     double acc[NG][4];
 #pragma unroll
@@ -160,7 +166,8 @@ __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int6
 template <typename T, int NG, int R>
 __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int64_t n_rows,
                                                               int n_cols, int64_t rpb,
-                                                              const float *x, float *out) {
+                                                              const float *x, const float *scale,
+                                                              float *out) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
@@ -173,6 +180,10 @@ __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int
         const int qi = lane + 64 * g;
         gv[g] = qi < nq;
         xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (scale && gv[g]) {
+            const float4 sc = *reinterpret_cast<const float4 *>(scale + 4 * qi);
+            xv[g] = make_float4(xv[g].x / sc.x, xv[g].y / sc.y, xv[g].z / sc.z, xv[g].w / sc.w);
+        }
     }
     for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
         float4 av[R][NG];
@@ -206,7 +217,7 @@ __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int
 __global__ __launch_bounds__(1024) void reduce_partials_kernel(const double *partial,
                                                                int nblocks, int n_cols,
                                                                double *out, const double *sens,
-                                                               float *x) {
+                                                               float *x, const float *scale) {
     __shared__ double red[16][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int j = blockIdx.x * 64 + lane;
@@ -218,6 +229,7 @@ __global__ __launch_bounds__(1024) void reduce_partials_kernel(const double *par
     if (wid == 0 && j < n_cols) {
         double t = red[0][lane];
         for (int q = 1; q < 16; q++) t += red[q][lane];
+        if (scale) t /= (double)scale[j];
         if (x) {
             // main.m:162-163,176
             double v = (double)x[j] * t / (sens[j] + 1e-7);
@@ -244,7 +256,8 @@ template <typename T, int NG, int R>
 __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int64_t n_rows,
                                                              int n_cols, int64_t rpb,
                                                              const float *x, const float *y,
-                                                             const float *c, double *partial) {
+                                                             const float *c, const float *scale,
+                                                             double *partial) {
     __shared__ double red[kMatWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
@@ -258,6 +271,10 @@ __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int6
         const int qi = lane + 64 * g;
         gv[g] = qi < nq;
         xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (scale && gv[g]) {
+            const float4 sc = *reinterpret_cast<const float4 *>(scale + 4 * qi);
+            xv[g] = make_float4(xv[g].x / sc.x, xv[g].y / sc.y, xv[g].z / sc.z, xv[g].w / sc.w);
+        }
     }
     double ll = 0.0;
     for (int64_t r = r0 + wid; r < r1; r += kMatWaves) {
@@ -303,6 +320,56 @@ __global__ void convert_kernel(const float *A, int64_t n, int dtype, void *out) 
             reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
         }
     }
+}
+
+__global__ void convert_scaled_kernel(const float *A, int64_t n_rows, int n_cols, int dtype,
+                                      const float *scale, void *out) {
+    const int64_t n = n_rows * n_cols;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float v = A[i];
+        if (scale) v *= scale[i % n_cols];  // power-of-two scale: exact
+        if (dtype == XRD_DTYPE_BF16) {
+            uint32_t u = __float_as_uint(v);
+            uint16_t b;
+            if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) {
+                b = (uint16_t)((u >> 16) | 0x40u);
+            } else {
+                u += 0x7fffu + ((u >> 16) & 1u);
+                b = (uint16_t)(u >> 16);
+            }
+            reinterpret_cast<uint16_t *>(out)[i] = b;
+        } else {
+            reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
+        }
+    }
+}
+
+// partial_max[b, j] = max over the rows of block b of |A[r, j]|
+__global__ __launch_bounds__(256) void column_max_kernel(const float *A, int64_t n_rows,
+                                                         int n_cols, int64_t rpb,
+                                                         float *partial_max) {
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    int64_t r1 = r0 + rpb;
+    if (r1 > n_rows) r1 = n_rows;
+    for (int j = threadIdx.x; j < n_cols; j += blockDim.x) {
+        float m = 0.f;
+        for (int64_t r = r0; r < r1; r++) m = fmaxf(m, fabsf(A[r * n_cols + j]));
+        partial_max[(size_t)blockIdx.x * n_cols + j] = m;
+    }
+}
+
+__global__ void column_scale_kernel(const float *partial_max, int nblocks, int n_cols, int dtype,
+                                    float *scale) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_cols) return;
+    float m = 0.f;
+    for (int b = 0; b < nblocks; b++) m = fmaxf(m, partial_max[(size_t)b * n_cols + j]);
+    // map the column maximum to about half the largest finite value
+    const float target = dtype == XRD_DTYPE_F16 ? 32768.f : 1.0e38f;
+    float sc = 1.f;
+    if (m > 0.f) sc = exp2f(floorf(log2f(target / m)));
+    scale[j] = sc;
 }
 
 // ---------------------------------------------------------------------------
@@ -351,7 +418,7 @@ static hipError_t colsum_t(const MatArgs &m, const float *x, const float *y, con
     XRD_DISPATCH_NG(NG, {
         constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
         hipLaunchKernelGGL((colsum_kernel<T, NG, R, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
-                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, partial);
+                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
     });
     return hipGetLastError();
 }
@@ -385,7 +452,7 @@ static hipError_t project_t(const MatArgs &m, const float *x, float *out, hipStr
     XRD_DISPATCH_NG(NG, {
         constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
         hipLaunchKernelGGL((project_kernel<T, NG, R>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
-                           m.n_rows, m.n_cols, rpb, x, out);
+                           m.n_rows, m.n_cols, rpb, x, m.scale, out);
     });
     return hipGetLastError();
 }
@@ -406,7 +473,7 @@ static hipError_t loglik_t(const MatArgs &m, const float *x, const float *y, con
     const int64_t rpb = rows_per_block(m, nb);
     XRD_DISPATCH_NG(NG, {
         hipLaunchKernelGGL((loglik_kernel<T, NG, 1>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
-                           m.n_rows, m.n_cols, rpb, x, y, c, partial);
+                           m.n_rows, m.n_cols, rpb, x, y, c, m.scale, partial);
     });
     return hipGetLastError();
 }
@@ -422,16 +489,16 @@ hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float 
 }
 
 hipError_t launch_reduce_partials(const double *partial, int nblocks, int n_cols, double *out,
-                                  hipStream_t s) {
+                                  hipStream_t s, const float *scale) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((n_cols + 63) / 64), dim3(1024), 0, s,
-                       partial, nblocks, n_cols, out, nullptr, nullptr);
+                       partial, nblocks, n_cols, out, nullptr, nullptr, scale);
     return hipGetLastError();
 }
 
 hipError_t launch_mlem_update(const double *partial, int nblocks, int n_cols,
-                              const double *sens, float *x, hipStream_t s) {
+                              const double *sens, float *x, hipStream_t s, const float *scale) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((n_cols + 63) / 64), dim3(1024), 0, s,
-                       partial, nblocks, n_cols, nullptr, sens, x);
+                       partial, nblocks, n_cols, nullptr, sens, x, scale);
     return hipGetLastError();
 }
 
@@ -440,6 +507,27 @@ hipError_t launch_ratio(int64_t n, const float *f, const float *y, const float *
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ratio_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, f,
                        y, c, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_column_scale(const float *A, int64_t n_rows, int n_cols, int dtype,
+                               float *partial_max, int nblocks, float *scale, hipStream_t s) {
+    const int64_t rpb = (n_rows + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(column_max_kernel, dim3(nblocks), dim3(256), 0, s, A, n_rows, n_cols, rpb,
+                       partial_max);
+    hipLaunchKernelGGL(column_scale_kernel, dim3((n_cols + 255) / 256), dim3(256), 0, s,
+                       partial_max, nblocks, n_cols, dtype, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_scaled(const float *A, int64_t n_rows, int n_cols, int dtype,
+                                 const float *scale, void *out, hipStream_t s) {
+    const int64_t n = n_rows * n_cols;
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(convert_scaled_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, n_rows,
+                       n_cols, dtype, scale, out);
     return hipGetLastError();
 }
 

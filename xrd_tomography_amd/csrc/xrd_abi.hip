@@ -543,6 +543,28 @@ extern "C" int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, i
     return XRD_OK;
 }
 
+extern "C" int xrd_convert_matrix_scaled(xrd_context *ctx, const float *A, int64_t n_rows,
+                                         int32_t n_cols, int out_dtype, void *out,
+                                         float *col_scale, void *stream) {
+    const char *fn = "xrd_convert_matrix_scaled";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!A || !out || !col_scale) return fail(XRD_ERR_NULL_POINTER, "%s: NULL argument", fn);
+    if (n_rows <= 0 || n_cols <= 0) return fail(XRD_ERR_DIMS, "%s: non-positive size", fn);
+    if (out_dtype != XRD_DTYPE_BF16 && out_dtype != XRD_DTYPE_F16)
+        return fail(XRD_ERR_INVALID_ARG, "%s: out_dtype must be BF16 or F16", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    int nb = 1024;
+    if (nb > n_rows) nb = (int)n_rows;
+    HIP_TRY(ctx->small.ensure((size_t)nb * n_cols * sizeof(float)), "workspace alloc");
+    HIP_TRY(launch_column_scale(A, n_rows, n_cols, out_dtype, (float *)ctx->small.ptr, nb,
+                                col_scale, s),
+            "column-scale kernels");
+    HIP_TRY(launch_convert_scaled(A, n_rows, n_cols, out_dtype, col_scale, out, s),
+            "convert kernel");
+    return XRD_OK;
+}
+
 // ---- MLEM -------------------------------------------------------------------------
 
 namespace {
@@ -561,14 +583,15 @@ hipError_t mlem_one_iteration(xrd_mlem_plan *p, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     return launch_mlem_update((const double *)p->partial.ptr, p->nblocks, p->m.n_cols,
-                              (const double *)p->sens.ptr, p->x, s);
+                              (const double *)p->sens.ptr, p->x, s, p->m.scale);
 }
 
 }  // namespace
 
 extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
-                                    int32_t n_cols, const float *y, const float *c, float *x,
-                                    int mode, void *stream, xrd_mlem_plan **out) {
+                                    int32_t n_cols, const float *col_scale, const float *y,
+                                    const float *c, float *x, int mode, void *stream,
+                                    xrd_mlem_plan **out) {
     const char *fn = "xrd_mlem_plan_create";
     if (!out) return fail(XRD_ERR_NULL_POINTER, "%s: out is NULL", fn);
     *out = nullptr;
@@ -582,7 +605,7 @@ extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     xrd_mlem_plan *p = new xrd_mlem_plan();
     p->ctx = ctx;
-    p->m = MatArgs{A, a_dtype, n_rows, n_cols};
+    p->m = MatArgs{A, a_dtype, n_rows, n_cols, col_scale};
     p->y = y;
     p->c = c;
     p->x = x;
@@ -600,7 +623,7 @@ extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype
     if (e == hipSuccess) e = launch_colsum_partials(p->m, nullptr, (double *)p->partial.ptr, s);
     if (e == hipSuccess)
         e = launch_reduce_partials((const double *)p->partial.ptr, p->nblocks, n_cols,
-                                   (double *)p->sens.ptr, s);
+                                   (double *)p->sens.ptr, s, col_scale);
     if (e != hipSuccess) {
         xrd_mlem_plan_destroy(p);
         return hip_fail(e, "xrd_mlem_plan_create");

@@ -118,7 +118,18 @@ struct MatArgs {
     int dtype;
     int64_t n_rows;
     int n_cols;
+    // Per-column storage scale: the stored matrix is A[r,j]*scale[j] (NULL = 1).
+    // Kernels divide x by it on load and divide column sums by it on reduction.
+    const float *scale = nullptr;
 };
+
+// Per-column maxima of |A| (FP32 matrix) and power-of-two scales that map each
+// column's maximum to about half the largest finite value of dtype (BF16/F16).
+hipError_t launch_column_scale(const float *A, int64_t n_rows, int n_cols, int dtype,
+                               float *partial_max, int nblocks, float *scale, hipStream_t s);
+// Convert with per-column scale (scale may be NULL).
+hipError_t launch_convert_scaled(const float *A, int64_t n_rows, int n_cols, int dtype,
+                                 const float *scale, void *out, hipStream_t s);
 
 int mat_num_blocks(const MatArgs &m);
 hipError_t launch_project(const MatArgs &m, const float *x, float *out, hipStream_t s);
@@ -128,7 +139,7 @@ hipError_t launch_colsum_partials(const MatArgs &m, const float *w, double *part
                                   hipStream_t s);
 // out[j] = sum_b partial[b, j]  (fixed order)
 hipError_t launch_reduce_partials(const double *partial, int nblocks, int n_cols, double *out,
-                                  hipStream_t s);
+                                  hipStream_t s, const float *scale = nullptr);
 // fused MLEM forward+adjoint pass: partial[b, j] = sum_{r in block b} A[r,j] y_r/(A_r x + c_r + 1e-20)
 hipError_t launch_mlem_fused(const MatArgs &m, const float *x, const float *y, const float *c,
                              double *partial, hipStream_t s);
@@ -137,7 +148,8 @@ hipError_t launch_ratio(int64_t n, const float *f, const float *y, const float *
                         hipStream_t s);
 // x <- x * (sum_b partial[b,j]) / (sens[j] + 1e-7), non-finite -> 0, < 0 -> 0
 hipError_t launch_mlem_update(const double *partial, int nblocks, int n_cols,
-                              const double *sens, float *x, hipStream_t s);
+                              const double *sens, float *x, hipStream_t s,
+                              const float *scale = nullptr);
 // per-block FP64 partial Poisson log-likelihood
 hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float *y,
                                   const float *c, double *partial, hipStream_t s);
