@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=12, help="views in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 2 GiB device-to-device copy 10 times (calibrates HBM "
+                         "byte counters: 2 GiB read + 2 GiB written per launch)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the MLEM path would "
@@ -53,6 +56,22 @@ def main():
     from xrd_tomography_amd.synthetic import make_problem
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        n = (2 << 30) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                          "ms_per_copy": round(ms, 4),
+                          "copy_GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}))
+        return
     prob = make_problem(args.workload)
     g, d = prob.geometry, prob.disc
     nrow, ncol = prob.n_rows, prob.n_cols
@@ -80,15 +99,16 @@ def main():
     expected = (N * scale + c).double().cpu().numpy()
     y = torch.from_numpy(np.random.default_rng(11).poisson(expected).astype(np.float32)).to(dev)
     del N
+    col_scale = None
     if args.dtype != "f32":
         dt = torch.bfloat16 if args.dtype == "bf16" else torch.float16
-        Am = api.convert_matrix(A, dt)
+        Am, col_scale = api.convert_matrix_scaled(A, dt)
     else:
         Am = A
 
     # ---- MLEM timing
     x = torch.full((ncol,), 10.0, dtype=torch.float32, device=dev)
-    plan = api.MlemPlan(Am, y, c, x, fused=not args.unfused)
+    plan = api.MlemPlan(Am, y, c, x, fused=not args.unfused, col_scale=col_scale)
     use_graph = not args.no_graph
     plan.iterate(args.warmup, use_graph=use_graph)
     torch.cuda.synchronize()
@@ -122,7 +142,11 @@ def main():
         ys = y.index_select(0, rows_t).contiguous()
         cs = c.index_select(0, rows_t).contiguous()
         k_err = 5
-        xg = api.mlem(As, ys, cs, k_err)
+        # GPU MLEM on the stored matrix (FP32, or BF16/F16 with its column scale) vs
+        # the FP64 oracle on the FP32 matrix
+        As_stored = Am.index_select(0, rows_t).contiguous() if col_scale is not None else As
+        xg = api.mlem(As_stored, ys, cs, k_err, col_scale=col_scale)
+        del As_stored
         A64 = As.double().cpu().numpy()
         y64 = ys.double().cpu().numpy()
         c64 = cs.double().cpu().numpy()

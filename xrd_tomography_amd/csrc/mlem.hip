@@ -365,10 +365,11 @@ __global__ void column_scale_kernel(const float *partial_max, int nblocks, int n
     if (j >= n_cols) return;
     float m = 0.f;
     for (int b = 0; b < nblocks; b++) m = fmaxf(m, partial_max[(size_t)b * n_cols + j]);
-    // map the column maximum to about half the largest finite value
-    const float target = dtype == XRD_DTYPE_F16 ? 32768.f : 1.0e38f;
+    // F16: map the column maximum to about half the largest finite value (2^15),
+    // clamped to 2^-100..2^100.  BF16 has the FP32 exponent range: no scaling.
     float sc = 1.f;
-    if (m > 0.f) sc = exp2f(floorf(log2f(target / m)));
+    if (dtype == XRD_DTYPE_F16 && m > 0.f)
+        sc = exp2f(fminf(fmaxf(floorf(log2f(32768.f / m)), -100.f), 100.f));
     scale[j] = sc;
 }
 
