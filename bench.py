@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "bf16", "f16"])
     ap.add_argument("--unfused", action="store_true", help="plain two-pass A x then A' r")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape of the fused pass")
     ap.add_argument("--cpu-views", type=int, default=12, help="views in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -109,6 +111,10 @@ def main():
     # ---- MLEM timing
     x = torch.full((ncol,), 10.0, dtype=torch.float32, device=dev)
     plan = api.MlemPlan(Am, y, c, x, fused=not args.unfused, col_scale=col_scale)
+    tuned = None
+    if not args.unfused and not args.no_autotune:
+        tuned = plan.autotune(reps=2)
+        log(f"[bench] autotuned launch: {tuned}")
     use_graph = not args.no_graph
     plan.iterate(args.warmup, use_graph=use_graph)
     torch.cuda.synchronize()
@@ -191,6 +197,7 @@ def main():
                         f"{'unfused two-pass' if args.unfused else 'fused single-pass'} "
                         f"MLEM{', hipGraph replay' if use_graph else ''}",
             "rows": nrow, "cols": ncol,
+            "launch": tuned,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -260,6 +260,16 @@ int xrd_mlem_plan_iterate(xrd_mlem_plan *plan, int32_t n_iter, int use_graph, vo
 /* Poisson log-likelihood sum_r (y log f - f), f = A x + c + 1e-20, FP64, written to
  * *out_host after synchronising `stream`. */
 int xrd_mlem_plan_loglik(xrd_mlem_plan *plan, double *out_host, void *stream);
+/* Autotuner: times the fused pass for each launch shape (workgroups per CU in
+ * {1,2,3,4,6,8} x rows per wave in {1,2,4}) on the plan's own A, y, c and x (x is
+ * not modified), keeps the fastest and reports it.  Synchronises `stream`. */
+int xrd_mlem_plan_autotune(xrd_mlem_plan *plan, int32_t reps, void *stream,
+                           int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
+                           float *out_ms);
+/* Sets the launch shape explicitly (0 = default for either field), e.g. from a cache
+ * of earlier autotuning results. */
+int xrd_mlem_plan_set_launch(xrd_mlem_plan *plan, int32_t blocks_per_cu,
+                             int32_t rows_per_wave);
 /* Device pointer to the FP64 sensitivity vector (n_cols) held by the plan. */
 const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *plan);
 int xrd_mlem_plan_destroy(xrd_mlem_plan *plan);

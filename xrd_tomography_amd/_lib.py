@@ -30,7 +30,7 @@ EXPORTS = (
     "xrd_backproject", "xrd_sensitivity", "xrd_convert_matrix", "xrd_mlem_plan_create",
     "xrd_mlem_plan_iterate", "xrd_mlem_plan_loglik", "xrd_mlem_plan_sensitivity",
     "xrd_mlem_plan_destroy", "xrd_forward_coeffs", "xrd_vjp_coeffs", "xrd_poisson_nll",
-    "xrd_convert_matrix_scaled",
+    "xrd_convert_matrix_scaled", "xrd_mlem_plan_autotune", "xrd_mlem_plan_set_launch",
 )
 
 
@@ -68,6 +68,10 @@ def _declare(L):
         "xrd_mlem_plan_create": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp, _vp,
                                                 _vp, _vp, ctypes.c_int, _vp,
                                                 ctypes.POINTER(_vp)]),
+        "xrd_mlem_plan_autotune": (ctypes.c_int, [_vp, _i32, _vp, ctypes.POINTER(_i32),
+                                                  ctypes.POINTER(_i32),
+                                                  ctypes.POINTER(ctypes.c_float)]),
+        "xrd_mlem_plan_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),

@@ -236,6 +236,38 @@ class MlemPlan:
                                           _stream(stream)), "xrd_mlem_plan_iterate")
         return self.x
 
+    def autotune(self, reps: int = 3, cache: Optional[str] = None, stream=None) -> dict:
+        """Pick the launch shape of the fused pass (xrd_mlem_plan_autotune).  With
+        `cache` (a JSON file path) an earlier result for the same matrix shape, dtype and
+        device is reused instead of re-timing."""
+        import json
+        import os
+        key = f"{torch.cuda.get_device_name(self.x.device)}|{self._keep[0].shape[0]}x" \
+              f"{self.n_cols}|{self._keep[0].dtype}"
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+        if key in table:
+            b, r = table[key]["blocks_per_cu"], table[key]["rows_per_wave"]
+            check(lib().xrd_mlem_plan_set_launch(self.handle, b, r), "xrd_mlem_plan_set_launch")
+            return dict(table[key], cached=True)
+        b, r, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        check(lib().xrd_mlem_plan_autotune(self.handle, int(reps), _stream(stream),
+                                           ctypes.byref(b), ctypes.byref(r), ctypes.byref(ms)),
+              "xrd_mlem_plan_autotune")
+        res = {"blocks_per_cu": b.value, "rows_per_wave": r.value, "ms_fused_pass": ms.value}
+        if cache:
+            table[key] = res
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(table, f, indent=1)
+        return dict(res, cached=False)
+
+    def set_launch(self, blocks_per_cu: int = 0, rows_per_wave: int = 0):
+        check(lib().xrd_mlem_plan_set_launch(self.handle, blocks_per_cu, rows_per_wave),
+              "xrd_mlem_plan_set_launch")
+
     def loglik(self, stream=None) -> float:
         v = ctypes.c_double()
         check(lib().xrd_mlem_plan_loglik(self.handle, ctypes.byref(v), _stream(stream)),

@@ -391,8 +391,11 @@ static int num_cus() {
 }
 
 int mat_num_blocks(const MatArgs &m) {
-    // two resident workgroups per CU, never more blocks than 16-row groups
-    int64_t nb = (int64_t)num_cus() * 2;
+    // blocks_per_cu resident workgroups per CU (default 2), never more blocks than
+    // 16-row groups
+    int bpc = m.blocks_per_cu > 0 ? m.blocks_per_cu : 2;
+    if (bpc > kMaxBlocksPerCu) bpc = kMaxBlocksPerCu;
+    int64_t nb = (int64_t)num_cus() * bpc;
     const int64_t cap = (m.n_rows + 15) / 16;
     if (nb > cap) nb = cap;
     if (nb < 1) nb = 1;
@@ -411,16 +414,32 @@ static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb 
         else return hipErrorInvalidValue;                          \
     } while (0)
 
+template <typename T, int NG, int MODE>
+static void colsum_launch_r(const MatArgs &m, int nb, int64_t rpb, const float *x,
+                            const float *y, const float *c, const float *w, double *partial,
+                            hipStream_t s) {
+    // rows per wave and step: the autotuned value for the MLEM pass, else a default
+    // that keeps 8 KB (FP32) of a row block in flight per wave
+    int r = (MODE == kColsMlem && m.rows_per_wave > 0) ? m.rows_per_wave
+                                                       : (NG >= 8 ? 1 : (NG >= 4 ? 2 : 4));
+    if (r >= 4 && NG <= 4) {
+        hipLaunchKernelGGL((colsum_kernel<T, NG, 4, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
+    } else if (r >= 2) {
+        hipLaunchKernelGGL((colsum_kernel<T, NG, 2, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
+    } else {
+        hipLaunchKernelGGL((colsum_kernel<T, NG, 1, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
+    }
+}
+
 template <typename T, int MODE>
 static hipError_t colsum_t(const MatArgs &m, const float *x, const float *y, const float *c,
                            const float *w, double *partial, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(NG, {
-        constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
-        hipLaunchKernelGGL((colsum_kernel<T, NG, R, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
-                           m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
-    });
+    XRD_DISPATCH_NG(NG, { colsum_launch_r<T, NG, MODE>(m, nb, rpb, x, y, c, w, partial, s); });
     return hipGetLastError();
 }
 

@@ -612,9 +612,13 @@ extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype
     p->mode = mode;
     p->nblocks = mat_num_blocks(p->m);
     hipStream_t s = (hipStream_t)stream;
+    // partials sized for the largest launch shape the autotuner may pick
+    MatArgs mmax = p->m;
+    mmax.blocks_per_cu = kMaxBlocksPerCu;
+    const int nb_max = mat_num_blocks(mmax);
     hipError_t e = p->sens.ensure((size_t)n_cols * sizeof(double));
-    if (e == hipSuccess) e = p->partial.ensure((size_t)p->nblocks * n_cols * sizeof(double));
-    if (e == hipSuccess) e = p->llp.ensure((size_t)p->nblocks * sizeof(double) + 8);
+    if (e == hipSuccess) e = p->partial.ensure((size_t)nb_max * n_cols * sizeof(double));
+    if (e == hipSuccess) e = p->llp.ensure((size_t)nb_max * sizeof(double) + 8);
     if (e == hipSuccess && mode == XRD_MLEM_UNFUSED) {
         e = p->f.ensure((size_t)n_rows * sizeof(float));
         if (e == hipSuccess) e = p->r.ensure((size_t)n_rows * sizeof(float));
@@ -674,6 +678,88 @@ extern "C" int xrd_mlem_plan_loglik(xrd_mlem_plan *p, double *out_host, void *st
     double t = 0.0;
     for (double v : h) t += v;
     *out_host = t;
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_plan_autotune(xrd_mlem_plan *p, int32_t reps, void *stream,
+                                      int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
+                                      float *out_ms) {
+    const char *fn = "xrd_mlem_plan_autotune";
+    if (!p) return fail(XRD_ERR_NULL_POINTER, "%s: plan is NULL", fn);
+    if (reps <= 0) return fail(XRD_ERR_INVALID_ARG, "%s: reps must be > 0", fn);
+    if (p->mode != XRD_MLEM_FUSED) return fail(XRD_ERR_UNSUPPORTED, "%s: fused plans only", fn);
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    const int bpcs[] = {1, 2, 3, 4, 6, 8};
+    const int rows[] = {1, 2, 4};
+    float best = 1e30f;
+    int best_b = 2, best_r = 0;
+    hipError_t err = hipSuccess;
+    for (int b : bpcs) {
+        for (int r : rows) {
+            MatArgs m = p->m;
+            m.blocks_per_cu = b;
+            m.rows_per_wave = r;
+            // the fused pass only reads x, y, c and writes the partials: x is untouched
+            err = launch_mlem_fused(m, p->x, p->y, p->c, (double *)p->partial.ptr, s);
+            if (err == hipSuccess) err = hipEventRecord(e0, s);
+            for (int i = 0; i < reps && err == hipSuccess; i++)
+                err = launch_mlem_fused(m, p->x, p->y, p->c, (double *)p->partial.ptr, s);
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err != hipSuccess) break;
+            ms /= (float)reps;
+            if (ms < best) {
+                best = ms;
+                best_b = b;
+                best_r = r;
+            }
+        }
+        if (err != hipSuccess) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (err != hipSuccess) return hip_fail(err, fn);
+    p->m.blocks_per_cu = best_b;
+    p->m.rows_per_wave = best_r;
+    p->nblocks = mat_num_blocks(p->m);
+    if (p->exec) {  // the captured iteration used the old launch shape
+        hipGraphExecDestroy(p->exec);
+        p->exec = nullptr;
+    }
+    if (p->graph) {
+        hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    if (out_blocks_per_cu) *out_blocks_per_cu = best_b;
+    if (out_rows_per_wave) *out_rows_per_wave = best_r;
+    if (out_ms) *out_ms = best;
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_plan_set_launch(xrd_mlem_plan *p, int32_t blocks_per_cu,
+                                        int32_t rows_per_wave) {
+    if (!p) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_plan_set_launch: plan is NULL");
+    if (blocks_per_cu < 0 || blocks_per_cu > kMaxBlocksPerCu || rows_per_wave < 0 ||
+        rows_per_wave > 4)
+        return fail(XRD_ERR_INVALID_ARG, "xrd_mlem_plan_set_launch: blocks_per_cu in [0,%d], "
+                    "rows_per_wave in [0,4] (0 = default)", kMaxBlocksPerCu);
+    p->m.blocks_per_cu = blocks_per_cu;
+    p->m.rows_per_wave = rows_per_wave;
+    p->nblocks = mat_num_blocks(p->m);
+    if (p->exec) {
+        hipGraphExecDestroy(p->exec);
+        p->exec = nullptr;
+    }
+    if (p->graph) {
+        hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
     return XRD_OK;
 }
 

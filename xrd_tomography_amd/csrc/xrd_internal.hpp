@@ -121,7 +121,14 @@ struct MatArgs {
     // Per-column storage scale: the stored matrix is A[r,j]*scale[j] (NULL = 1).
     // Kernels divide x by it on load and divide column sums by it on reduction.
     const float *scale = nullptr;
+    // Launch shape of the streaming kernels (0 = default): resident workgroups per
+    // CU and consecutive rows each wave loads per step.
+    int blocks_per_cu = 0;
+    int rows_per_wave = 0;
 };
+
+// Largest blocks_per_cu the partial buffers must be sized for.
+constexpr int kMaxBlocksPerCu = 8;
 
 // Per-column maxima of |A| (FP32 matrix) and power-of-two scales that map each
 // column's maximum to about half the largest finite value of dtype (BF16/F16).
