@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape of the fused pass")
+    ap.add_argument("--launch", default=None,
+                    help="B,R: fix the fused pass to B workgroups per CU and R rows per wave "
+                         "(no autotuning)")
     ap.add_argument("--cpu-views", type=int, default=12, help="views in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -112,7 +115,11 @@ def main():
     x = torch.full((ncol,), 10.0, dtype=torch.float32, device=dev)
     plan = api.MlemPlan(Am, y, c, x, fused=not args.unfused, col_scale=col_scale)
     tuned = None
-    if not args.unfused and not args.no_autotune:
+    if args.launch:
+        b, r = (int(v) for v in args.launch.split(","))
+        plan.set_launch(b, r)
+        tuned = {"blocks_per_cu": b, "rows_per_wave": r, "fixed": True}
+    elif not args.unfused and not args.no_autotune:
         tuned = plan.autotune(reps=2)
         log(f"[bench] autotuned launch: {tuned}")
     use_graph = not args.no_graph

@@ -261,7 +261,7 @@ int xrd_mlem_plan_iterate(xrd_mlem_plan *plan, int32_t n_iter, int use_graph, vo
  * *out_host after synchronising `stream`. */
 int xrd_mlem_plan_loglik(xrd_mlem_plan *plan, double *out_host, void *stream);
 /* Autotuner: times the fused pass for each launch shape (workgroups per CU in
- * {1,2,3,4,6,8} x rows per wave in {1,2,4}) on the plan's own A, y, c and x (x is
+ * {1,2,3,4,6} x rows per wave in {1,2,4}) on the plan's own A, y, c and x (x is
  * not modified), keeps the fastest and reports it.  Synchronises `stream`. */
 int xrd_mlem_plan_autotune(xrd_mlem_plan *plan, int32_t reps, void *stream,
                            int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,

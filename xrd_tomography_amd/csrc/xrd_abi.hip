@@ -693,7 +693,7 @@ extern "C" int xrd_mlem_plan_autotune(xrd_mlem_plan *p, int32_t reps, void *stre
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
-    const int bpcs[] = {1, 2, 3, 4, 6, 8};
+    const int bpcs[] = {1, 2, 3, 4, 6};
     const int rows[] = {1, 2, 4};
     float best = 1e30f;
     int best_b = 2, best_r = 0;
