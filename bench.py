@@ -23,6 +23,10 @@ if ROOT not in sys.path:
 
 METRIC = "MLEM fwd+adj iters/s @256²×360 views; % HBM roofline; rel err vs FP64"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the streaming pass, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (profiles/mlem_fused_fan256_hbm_traffic.md): FETCH_SIZE scaled by 2.0 and
+# WRITE_SIZE by 1.0, the factors that map each counter onto a 2 GiB copy kernel.
+MEASURED_TRAFFIC = {("fan256", "f32", "fused"): 23_076_117_299}
 
 
 def log(*a):
@@ -136,10 +140,15 @@ def main():
     iters_s = 1000.0 / ms_step
     log(f"[bench] {args.steps} MLEM steps: {ms_step:.3f} ms/step, {iters_s:.1f} iters/s")
 
+    # the dominant kernel alone: the streaming pass over A, timed live with HIP events
+    ms_pass = plan.time_pass(reps=max(5, args.steps // 5))
+    log(f"[bench] streaming pass alone: {ms_pass:.3f} ms")
     esize = {"f32": 4, "bf16": 2, "f16": 2}[args.dtype]
     bytes_step = nrow * ncol * esize + nrow * 8 + ncol * 12
     passes = 2 if args.unfused else 1
-    achieved = bytes_step * passes / (ms_step * 1e-3) / 1e9
+    achieved = bytes_step * passes / (ms_pass * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.dtype,
+                                    "unfused" if args.unfused else "fused"))
 
     # ---- error vs FP64 and CPU baseline on a view subsample (whole views, all rows of them)
     rel_err = None
@@ -208,7 +217,8 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic, "kernel_ms": round(ms_pass, 4),
+                     "algorithmic_bytes": bytes_step * passes},
         "cpu_baseline": cpu,
         "rel_err_vs_fp64": rel_err,
         "a_build_s": round(a_build_s, 3),

@@ -266,6 +266,10 @@ int xrd_mlem_plan_loglik(xrd_mlem_plan *plan, double *out_host, void *stream);
 int xrd_mlem_plan_autotune(xrd_mlem_plan *plan, int32_t reps, void *stream,
                            int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
                            float *out_ms);
+/* Average time in ms of the streaming pass over A alone (the fused pass, or for
+ * XRD_MLEM_UNFUSED the A x, ratio and A'r kernels), over reps back-to-back passes
+ * timed with HIP events on `stream`; x is not modified.  Synchronises `stream`. */
+int xrd_mlem_plan_time_pass(xrd_mlem_plan *plan, int32_t reps, void *stream, float *out_ms);
 /* Sets the launch shape explicitly (0 = default for either field), e.g. from a cache
  * of earlier autotuning results. */
 int xrd_mlem_plan_set_launch(xrd_mlem_plan *plan, int32_t blocks_per_cu,
@@ -273,6 +277,27 @@ int xrd_mlem_plan_set_launch(xrd_mlem_plan *plan, int32_t blocks_per_cu,
 /* Device pointer to the FP64 sensitivity vector (n_cols) held by the plan. */
 const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *plan);
 int xrd_mlem_plan_destroy(xrd_mlem_plan *plan);
+
+/* ---- batched MLEM ------------------------------------------------------------
+ * MLEM for n_problems independent explicit matrices (phantoms or slices of uneven
+ * size and view count) with one persistent launch per pass over all of them.
+ * Host arrays of n_problems device pointers/sizes are read at creation; col_scale
+ * may be NULL or hold NULL entries.  Every problem's rows are cut into tiles of
+ * rows_per_tile rows (<= 0: 8192); results do not depend on the tile schedule.
+ * The sensitivities are computed (FP64) at creation. */
+typedef struct xrd_mlem_batch xrd_mlem_batch;
+
+int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a_dtype,
+                          const void *const *A, const int64_t *n_rows, const int32_t *n_cols,
+                          const float *const *col_scale, const float *const *y,
+                          const float *const *c, float *const *x, int32_t rows_per_tile,
+                          void *stream, xrd_mlem_batch **out);
+/* n_iter updates of every x[k]; use_graph replays a captured iteration. */
+int xrd_mlem_batch_iterate(xrd_mlem_batch *batch, int32_t n_iter, int use_graph, void *stream);
+/* Launch shape of the persistent pass (0 = default for either field). */
+int xrd_mlem_batch_set_launch(xrd_mlem_batch *batch, int32_t blocks_per_cu,
+                              int32_t rows_per_wave);
+int xrd_mlem_batch_destroy(xrd_mlem_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -147,3 +147,37 @@ def test_scaled_reduced_precision_mlem(xrd, oracle, dtype):
     print(f"\n[scaled {dtype}] vs FP64 on stored matrix {rel:.2e}; vs FP64 on the FP32 matrix "
           f"{storage:.2e}")
     assert rel <= 2e-5
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float16"])
+def test_batched_mlem_matches_single(xrd, oracle, dtype):
+    """Problems of uneven rows and columns in one persistent launch: each x equals the
+    single-problem plan's x (1e-6) and the FP64 oracle's (2e-5 on the stored matrix)."""
+    import torch
+    shapes = [(3001, 48), (20000, 512), (777, 1024), (12000, 256), (5, 16)]
+    probs, singles, refs = [], [], []
+    for i, (r, cc) in enumerate(shapes):
+        A, y, c = _problem(20 + i, rows=r, cols=cc)
+        Ad = torch.from_numpy(A).cuda()
+        sc = None
+        if dtype == "float16":
+            Ad, sc = xrd.convert_matrix_scaled(Ad, torch.float16)
+        yd, cd = torch.from_numpy(y).cuda(), torch.from_numpy(c).cuda()
+        x = torch.full((cc,), 10.0, device="cuda")
+        probs.append((Ad, yd, cd, x, sc) if sc is not None else (Ad, yd, cd, x))
+        singles.append(xrd.mlem(Ad, yd, cd, 8, col_scale=sc))
+        Aeff = Ad.double().cpu().numpy() / (sc.double().cpu().numpy()[None, :] if sc is not None else 1.0)
+        refs.append(oracle.mlem(Aeff, y, c, np.full(cc, 10.0), 8))
+    batch = xrd.MlemBatch(probs, rows_per_tile=1000)
+    batch.iterate(4, use_graph=False)
+    batch.iterate(4, use_graph=True)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for (p, xs, xo) in zip(probs, singles, refs):
+        xb = p[3].cpu().numpy()
+        assert np.allclose(xb, xs.cpu().numpy(), rtol=1e-6, atol=1e-6 * xs.abs().max().item())
+        live = xo > 1e-6 * xo.max()
+        worst = max(worst, float(np.max(np.abs(xb[live] - xo[live]) / xo[live])))
+    print(f"\n[batched {dtype}] worst rel err vs FP64 {worst:.2e}")
+    assert worst <= 2e-5
+    batch.close()

@@ -31,6 +31,8 @@ EXPORTS = (
     "xrd_mlem_plan_iterate", "xrd_mlem_plan_loglik", "xrd_mlem_plan_sensitivity",
     "xrd_mlem_plan_destroy", "xrd_forward_coeffs", "xrd_vjp_coeffs", "xrd_poisson_nll",
     "xrd_convert_matrix_scaled", "xrd_mlem_plan_autotune", "xrd_mlem_plan_set_launch",
+    "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
+    "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass",
 )
 
 
@@ -72,6 +74,15 @@ def _declare(L):
                                                   ctypes.POINTER(_i32),
                                                   ctypes.POINTER(ctypes.c_float)]),
         "xrd_mlem_plan_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
+        "xrd_mlem_plan_time_pass": (ctypes.c_int, [_vp, _i32, _vp, ctypes.POINTER(ctypes.c_float)]),
+        "xrd_mlem_batch_create": (ctypes.c_int, [_vp, _i32, ctypes.c_int, ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(_i64), ctypes.POINTER(_i32),
+                                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i32,
+                                                 _vp, ctypes.POINTER(_vp)]),
+        "xrd_mlem_batch_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),
+        "xrd_mlem_batch_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
+        "xrd_mlem_batch_destroy": (ctypes.c_int, [_vp]),
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),

@@ -268,6 +268,13 @@ class MlemPlan:
         check(lib().xrd_mlem_plan_set_launch(self.handle, blocks_per_cu, rows_per_wave),
               "xrd_mlem_plan_set_launch")
 
+    def time_pass(self, reps: int = 10, stream=None) -> float:
+        """Average ms of the streaming pass over A alone (HIP events on the stream)."""
+        ms = ctypes.c_float()
+        check(lib().xrd_mlem_plan_time_pass(self.handle, int(reps), _stream(stream),
+                                            ctypes.byref(ms)), "xrd_mlem_plan_time_pass")
+        return ms.value
+
     def loglik(self, stream=None) -> float:
         v = ctypes.c_double()
         check(lib().xrd_mlem_plan_loglik(self.handle, ctypes.byref(v), _stream(stream)),
@@ -383,3 +390,54 @@ def coeff_problem_tensors(cp, device="cuda"):
     return (t(p.flat("spectrum"), torch.float32), t(p.flat("segmentation"), torch.int32),
             t(cp.coef, torch.float32), t(cp.basis, torch.float32), t(cp.maps, torch.float32),
             t(cp.beta, torch.float32))
+
+
+class MlemBatch:
+    """MLEM of main.m:153-176 for many independent problems with one persistent launch
+    per pass (xrd_mlem_batch_*).  problems: list of (A, y, c, x[, col_scale]); all A share
+    one dtype; every x is updated in place."""
+
+    def __init__(self, problems, rows_per_tile: int = 8192, ctx=None, stream=None):
+        if not problems:
+            raise ValueError("no problems")
+        self.ctx = ctx or default_context(problems[0][0].device.index or 0)
+        dts = {p[0].dtype for p in problems}
+        if len(dts) != 1:
+            raise TypeError("all matrices of a batch must share one dtype")
+        n = len(problems)
+        VP = ctypes.c_void_p * n
+        A, y, c, x, sc = VP(), VP(), VP(), VP(), VP()
+        rows, cols = (ctypes.c_int64 * n)(), (ctypes.c_int32 * n)()
+        for k, p in enumerate(problems):
+            Ak, yk, ck, xk = p[:4]
+            s_k = p[4] if len(p) > 4 else None
+            dt, nr, nc = _mat(Ak)
+            _require_gpu(yk, ck, xk, s_k)
+            A[k], y[k], c[k], x[k] = Ak.data_ptr(), yk.data_ptr(), ck.data_ptr(), xk.data_ptr()
+            sc[k] = s_k.data_ptr() if s_k is not None else None
+            rows[k], cols[k] = nr, nc
+        self._keep = problems
+        h = ctypes.c_void_p()
+        check(lib().xrd_mlem_batch_create(self.ctx.handle, n, _DTYPES[dts.pop()], A, rows, cols,
+                                          sc, y, c, x, int(rows_per_tile), _stream(stream),
+                                          ctypes.byref(h)), "xrd_mlem_batch_create")
+        self.handle = h
+
+    def iterate(self, n_iter: int, use_graph: bool = False, stream=None):
+        check(lib().xrd_mlem_batch_iterate(self.handle, int(n_iter), int(bool(use_graph)),
+                                           _stream(stream)), "xrd_mlem_batch_iterate")
+
+    def set_launch(self, blocks_per_cu: int = 0, rows_per_wave: int = 0):
+        check(lib().xrd_mlem_batch_set_launch(self.handle, blocks_per_cu, rows_per_wave),
+              "xrd_mlem_batch_set_launch")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().xrd_mlem_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

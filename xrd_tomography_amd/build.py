@@ -13,7 +13,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libxrd_tomo.so")
-SOURCES = ("forward.hip", "mlem.hip", "coeff.hip", "xrd_abi.hip")
+SOURCES = ("forward.hip", "mlem.hip", "batch.hip", "coeff.hip", "xrd_abi.hip")
 ARCH = os.environ.get("XRD_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-Wall",

@@ -742,6 +742,41 @@ extern "C" int xrd_mlem_plan_autotune(xrd_mlem_plan *p, int32_t reps, void *stre
     return XRD_OK;
 }
 
+extern "C" int xrd_mlem_plan_time_pass(xrd_mlem_plan *p, int32_t reps, void *stream,
+                                       float *out_ms) {
+    const char *fn = "xrd_mlem_plan_time_pass";
+    if (!p || !out_ms) return fail(XRD_ERR_NULL_POINTER, "%s: NULL argument", fn);
+    if (reps <= 0) return fail(XRD_ERR_INVALID_ARG, "%s: reps must be > 0", fn);
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    hipError_t err = hipEventRecord(e0, s);
+    for (int i = 0; i < reps && err == hipSuccess; i++) {
+        if (p->mode == XRD_MLEM_FUSED) {
+            err = launch_mlem_fused(p->m, p->x, p->y, p->c, (double *)p->partial.ptr, s);
+        } else {
+            err = launch_project(p->m, p->x, (float *)p->f.ptr, s);
+            if (err == hipSuccess)
+                err = launch_ratio(p->m.n_rows, (const float *)p->f.ptr, p->y, p->c,
+                                   (float *)p->r.ptr, s);
+            if (err == hipSuccess)
+                err = launch_colsum_partials(p->m, (const float *)p->r.ptr,
+                                             (double *)p->partial.ptr, s);
+        }
+    }
+    if (err == hipSuccess) err = hipEventRecord(e1, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return hip_fail(err, fn);
+    *out_ms = ms / (float)reps;
+    return XRD_OK;
+}
+
 extern "C" int xrd_mlem_plan_set_launch(xrd_mlem_plan *p, int32_t blocks_per_cu,
                                         int32_t rows_per_wave) {
     if (!p) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_plan_set_launch: plan is NULL");
@@ -893,5 +928,183 @@ extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
     double t = 0.0;
     for (double v : h) t += v;
     *out_loss_host = t;
+    return XRD_OK;
+}
+
+// ---- batched MLEM -----------------------------------------------------------------
+
+struct xrd_mlem_batch {
+    xrd_context *ctx = nullptr;
+    BatchArgs args{};
+    int blocks_per_cu = 2;
+    DevBuf probs, tiles, partial, sens;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_grid(int blocks_per_cu) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    return n * blocks_per_cu;
+}
+
+void batch_drop_graph(xrd_mlem_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+hipError_t batch_iteration(xrd_mlem_batch *b, hipStream_t s) {
+    hipError_t e = launch_batch_stream(b->args, true, s);
+    if (e != hipSuccess) return e;
+    return launch_batch_update(b->args, nullptr, true, s);
+}
+
+}  // namespace
+
+extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a_dtype,
+                                     const void *const *A, const int64_t *n_rows,
+                                     const int32_t *n_cols, const float *const *col_scale,
+                                     const float *const *y, const float *const *c,
+                                     float *const *x, int32_t rows_per_tile, void *stream,
+                                     xrd_mlem_batch **out) {
+    const char *fn = "xrd_mlem_batch_create";
+    if (!out) return fail(XRD_ERR_NULL_POINTER, "%s: out is NULL", fn);
+    *out = nullptr;
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (n_problems <= 0) return fail(XRD_ERR_DIMS, "%s: n_problems must be > 0", fn);
+    if (!A || !n_rows || !n_cols || !y || !c || !x)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required host array is NULL", fn);
+    if (rows_per_tile <= 0) rows_per_tile = 8192;
+    std::vector<BatchProblem> probs(n_problems);
+    std::vector<BatchTile> tiles;
+    int max_cols = 0;
+    for (int k = 0; k < n_problems; k++) {
+        int rc = check_matrix(A[k], a_dtype, n_rows[k], n_cols[k], fn);
+        if (rc) return rc;
+        if (!y[k] || !c[k] || !x[k]) return fail(XRD_ERR_NULL_POINTER, "%s: y/c/x[%d] is NULL", fn, k);
+        if (!aligned(x[k], 16)) return fail(XRD_ERR_ALIGN, "%s: x[%d] misaligned", fn, k);
+        BatchProblem &p = probs[k];
+        p.A = A[k];
+        p.n_rows = n_rows[k];
+        p.n_cols = n_cols[k];
+        p.y = y[k];
+        p.c = c[k];
+        p.x = x[k];
+        p.scale = col_scale ? col_scale[k] : nullptr;
+        p.sens_offset = 0;
+        p.tile_begin = (int)tiles.size();
+        for (int64_t r = 0; r < n_rows[k]; r += rows_per_tile) {
+            BatchTile t{};
+            t.problem = k;
+            t.r0 = r;
+            t.r1 = r + rows_per_tile < n_rows[k] ? r + rows_per_tile : n_rows[k];
+            tiles.push_back(t);
+        }
+        p.tile_end = (int)tiles.size();
+        if (n_cols[k] > max_cols) max_cols = n_cols[k];
+    }
+    if (tiles.size() > (size_t)(1 << 30)) return fail(XRD_ERR_DIMS, "%s: too many tiles", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    xrd_mlem_batch *b = new xrd_mlem_batch();
+    b->ctx = ctx;
+    size_t sens_total = 0;
+    for (auto &p : probs) {
+        p.sens_offset = (int)sens_total;
+        sens_total += (size_t)p.n_cols;
+    }
+    hipError_t e = b->sens.ensure(sens_total * sizeof(double));
+    if (e == hipSuccess) e = b->probs.ensure(probs.size() * sizeof(BatchProblem));
+    if (e == hipSuccess) e = b->tiles.ensure(tiles.size() * sizeof(BatchTile));
+    if (e == hipSuccess)
+        e = b->partial.ensure(tiles.size() * (size_t)max_cols * sizeof(double));
+    if (e == hipSuccess)
+        for (auto &p : probs) p.sens = (const double *)b->sens.ptr + p.sens_offset;
+    // descriptors are small: synchronous copies from pageable memory
+    if (e == hipSuccess)
+        e = hipMemcpy(b->probs.ptr, probs.data(), probs.size() * sizeof(BatchProblem),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->tiles.ptr, tiles.data(), tiles.size() * sizeof(BatchTile),
+                      hipMemcpyHostToDevice);
+    BatchArgs &a = b->args;
+    a.probs = (const BatchProblem *)b->probs.ptr;
+    a.tiles = (const BatchTile *)b->tiles.ptr;
+    a.n_problems = n_problems;
+    a.n_tiles = (int)tiles.size();
+    a.max_cols = max_cols;
+    a.stride = max_cols;
+    a.dtype = a_dtype;
+    a.grid = batch_grid(b->blocks_per_cu);
+    a.rows_per_wave = 0;
+    a.partial = (double *)b->partial.ptr;
+    // sensitivities sum(A_k,1)' once, FP64
+    if (e == hipSuccess) e = launch_batch_stream(a, false, s);
+    if (e == hipSuccess) e = launch_batch_update(a, (double *)b->sens.ptr, false, s);
+    if (e != hipSuccess) {
+        xrd_mlem_batch_destroy(b);
+        return hip_fail(e, fn);
+    }
+    *out = b;
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_batch_iterate(xrd_mlem_batch *b, int32_t n_iter, int use_graph,
+                                      void *stream) {
+    const char *fn = "xrd_mlem_batch_iterate";
+    if (!b) return fail(XRD_ERR_NULL_POINTER, "%s: batch is NULL", fn);
+    if (n_iter < 0) return fail(XRD_ERR_INVALID_ARG, "%s: n_iter < 0", fn);
+    HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) {
+        for (int i = 0; i < n_iter; i++) HIP_TRY(batch_iteration(b, s), "batched iteration");
+        return XRD_OK;
+    }
+    if (!b->exec) {
+        hipStream_t cs;
+        HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t e2 = batch_iteration(b, cs);
+            e = hipStreamEndCapture(cs, &b->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+    }
+    for (int i = 0; i < n_iter; i++) HIP_TRY(hipGraphLaunch(b->exec, s), "hipGraphLaunch");
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_batch_set_launch(xrd_mlem_batch *b, int32_t blocks_per_cu,
+                                         int32_t rows_per_wave) {
+    if (!b) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_batch_set_launch: batch is NULL");
+    if (blocks_per_cu < 0 || blocks_per_cu > kMaxBlocksPerCu || rows_per_wave < 0 ||
+        rows_per_wave > 4)
+        return fail(XRD_ERR_INVALID_ARG, "xrd_mlem_batch_set_launch: blocks_per_cu in [0,%d], "
+                    "rows_per_wave in [0,4]", kMaxBlocksPerCu);
+    b->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 2;
+    b->args.grid = batch_grid(b->blocks_per_cu);
+    b->args.rows_per_wave = rows_per_wave;
+    batch_drop_graph(b);
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_batch_destroy(xrd_mlem_batch *b) {
+    if (!b) return XRD_OK;
+    (void)hipSetDevice(b->ctx->device);
+    batch_drop_graph(b);
+    b->probs.release();
+    b->tiles.release();
+    b->partial.release();
+    b->sens.release();
+    delete b;
     return XRD_OK;
 }

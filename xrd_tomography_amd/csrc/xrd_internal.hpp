@@ -162,4 +162,35 @@ hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float 
                                   const float *c, double *partial, hipStream_t s);
 hipError_t launch_convert(const float *A, int64_t n, int dtype, void *out, hipStream_t s);
 
+// ---- batched MLEM ------------------------------------------------------------------
+struct BatchProblem {
+    const void *A;        // [n_rows x n_cols], dtype of the batch
+    int64_t n_rows;
+    int n_cols;           // multiple of 4
+    int tile_begin, tile_end;  // this problem's tiles in the global list
+    int sens_offset;      // offset of its sensitivity in the batch's sensitivity buffer
+    const float *y, *c;
+    float *x;
+    const float *scale;   // per-column storage scale or NULL
+    const double *sens;   // device pointer to its FP64 sensitivity
+};
+
+struct BatchTile {
+    int problem;
+    int pad;
+    int64_t r0, r1;
+};
+
+struct BatchArgs {
+    const BatchProblem *probs;
+    const BatchTile *tiles;
+    int n_problems, n_tiles, max_cols, stride, dtype;
+    int grid;             // persistent workgroups
+    int rows_per_wave;    // 0 = default
+    double *partial;      // [n_tiles x stride]
+};
+
+hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s);
+hipError_t launch_batch_update(const BatchArgs &b, double *sens_out, bool update, hipStream_t s);
+
 }  // namespace xrd
