@@ -17,32 +17,23 @@ __global__ __launch_bounds__(kMatThreads) void batch_stream_kernel(const BatchPr
                                                                    const BatchTile *tiles,
                                                                    int n_tiles, int stride,
                                                                    double *partial) {
-    __shared__ double red[kMatWaves][NG * 256];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int W = Vec<T>::W;
+    __shared__ double red[kMatWaves * NG * 64 * W];
+    const int lane = threadIdx.x & 63;
     for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const BatchTile tl = tiles[t];
         const BatchProblem pb = probs[tl.problem];
-        float4 xv[NG];
+        float xv[NG][W];
         bool gv[NG];
-        load_x<NG>(MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.n_cols >> 2, lane, xv, gv);
-        double acc[NG][4];
+        load_x<W, NG>(MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.n_cols, lane, xv, gv);
+        double acc[NG][W];
 #pragma unroll
         for (int g = 0; g < NG; g++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) acc[g][i] = 0.0;
+            for (int i = 0; i < W; i++) acc[g][i] = 0.0;
         stream_rows<T, NG, R, MODE>(pb.A, pb.n_cols, tl.r0, tl.r1, xv, gv, pb.y, pb.c, nullptr,
                                     acc);
-#pragma unroll
-        for (int g = 0; g < NG; g++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) red[wid][4 * (lane + 64 * g) + i] = acc[g][i];
-        __syncthreads();
-        for (int j = threadIdx.x; j < pb.n_cols; j += kMatThreads) {
-            double s = red[0][j];
-#pragma unroll
-            for (int q = 1; q < kMatWaves; q++) s += red[q][j];
-            partial[(size_t)t * stride + j] = s;
-        }
+        block_column_sums<NG, W>(acc, red, pb.n_cols, partial + (size_t)t * stride);
         __syncthreads();
     }
 }
@@ -82,7 +73,8 @@ __global__ __launch_bounds__(1024) void batch_update_kernel(const BatchProblem *
 
 template <typename T, int MODE>
 static hipError_t batch_stream_t(const BatchArgs &b, hipStream_t s) {
-    const int ngr = (b.max_cols + 255) / 256;
+    constexpr int W = Vec<T>::W;
+    const int ngr = (b.max_cols + 64 * W - 1) / (64 * W);
     const int R = b.rows_per_wave > 0 ? b.rows_per_wave : 2;
 #define XRD_BATCH_LAUNCH(NGV, RV)                                                              \
     hipLaunchKernelGGL((batch_stream_kernel<T, NGV, RV, MODE>), dim3(b.grid), dim3(kMatThreads), \
@@ -96,21 +88,27 @@ static hipError_t batch_stream_t(const BatchArgs &b, hipStream_t s) {
     if (ngr <= 1) XRD_BATCH_R(1);
     else if (ngr <= 2) XRD_BATCH_R(2);
     else if (ngr <= 4) XRD_BATCH_R(4);
-    else if (ngr <= 8) XRD_BATCH_R(8);
-    else return hipErrorInvalidValue;
+    else if constexpr (64 * 8 * W <= kMaxMatCols) {
+        if (ngr <= 8) XRD_BATCH_R(8);
+        else return hipErrorInvalidValue;
+    } else return hipErrorInvalidValue;
 #undef XRD_BATCH_R
 #undef XRD_BATCH_LAUNCH
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t batch_stream_mode(const BatchArgs &b, bool mlem, hipStream_t s) {
+    return mlem ? batch_stream_t<T, kColsMlem>(b, s) : batch_stream_t<T, kColsOnes>(b, s);
+}
+
 hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s) {
     switch (b.dtype) {
-        case XRD_DTYPE_F32:
-            return mlem ? batch_stream_t<float, kColsMlem>(b, s) : batch_stream_t<float, kColsOnes>(b, s);
+        case XRD_DTYPE_F32: return batch_stream_mode<float>(b, mlem, s);
         case XRD_DTYPE_BF16:
-            return mlem ? batch_stream_t<bf16x4, kColsMlem>(b, s) : batch_stream_t<bf16x4, kColsOnes>(b, s);
+            return b.vec8 ? batch_stream_mode<bf16x8>(b, mlem, s) : batch_stream_mode<bf16x4>(b, mlem, s);
         case XRD_DTYPE_F16:
-            return mlem ? batch_stream_t<f16x4, kColsMlem>(b, s) : batch_stream_t<f16x4, kColsOnes>(b, s);
+            return b.vec8 ? batch_stream_mode<f16x8>(b, mlem, s) : batch_stream_mode<f16x4>(b, mlem, s);
     }
     return hipErrorInvalidValue;
 }

@@ -20,40 +20,30 @@
 namespace xrd {
 
 // Workgroup b handles rows [b*rpb, min((b+1)*rpb, n_rows)).  Each step the 4
-// waves take R consecutive rows each.  Lane l holds columns 4*(l + 64 g) + i.
+// waves take R consecutive rows each.  Lane l holds columns W (l + 64 g) + i.
 template <typename T, int NG, int R, int MODE>
 __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int64_t n_rows,
                                                              int n_cols, int64_t rpb,
                                                              const float *x, const float *y,
                                                              const float *c, const float *w,
                                                              const float *scale, double *partial) {
-    __shared__ double red[kMatWaves][NG * 256];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int W = Vec<T>::W;
+    __shared__ double red[kMatWaves * NG * 64 * W];
+    const int lane = threadIdx.x & 63;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    const int nq = n_cols >> 2;
 
-    float4 xv[NG];
+    float xv[NG][W];
     bool gv[NG];
-    load_x<NG>(MODE == kColsMlem ? x : nullptr, scale, nq, lane, xv, gv);
-    double acc[NG][4];
+    load_x<W, NG>(MODE == kColsMlem ? x : nullptr, scale, n_cols, lane, xv, gv);
+    double acc[NG][W];
 #pragma unroll
     for (int g = 0; g < NG; g++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) acc[g][i] = 0.0;
+        for (int i = 0; i < W; i++) acc[g][i] = 0.0;
     stream_rows<T, NG, R, MODE>(A, n_cols, r0, r1, xv, gv, y, c, w, acc);
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) red[wid][4 * (lane + 64 * g) + i] = acc[g][i];
-    __syncthreads();
-    for (int j = threadIdx.x; j < n_cols; j += kMatThreads) {
-        double s = red[0][j];
-#pragma unroll
-        for (int q = 1; q < kMatWaves; q++) s += red[q][j];
-        partial[(size_t)blockIdx.x * n_cols + j] = s;
-    }
+    block_column_sums<NG, W>(acc, red, n_cols, partial + (size_t)blockIdx.x * n_cols);
 }
 
 // out[r] = A_r . x  (FP32, one wave per row group)
@@ -62,44 +52,20 @@ __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int
                                                               int n_cols, int64_t rpb,
                                                               const float *x, const float *scale,
                                                               float *out) {
+    constexpr int W = Vec<T>::W;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    const int nq = n_cols >> 2;
-    float4 xv[NG];
+    float xv[NG][W];
     bool gv[NG];
-#pragma unroll
-    for (int g = 0; g < NG; g++) {
-        const int qi = lane + 64 * g;
-        gv[g] = qi < nq;
-        xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (scale && gv[g]) {
-            const float4 sc = *reinterpret_cast<const float4 *>(scale + 4 * qi);
-            xv[g] = make_float4(xv[g].x / sc.x, xv[g].y / sc.y, xv[g].z / sc.z, xv[g].w / sc.w);
-        }
-    }
+    load_x<W, NG>(x, scale, n_cols, lane, xv, gv);
     for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
-        float4 av[R][NG];
-#pragma unroll
-        for (int k = 0; k < R; k++)
-#pragma unroll
-            for (int g = 0; g < NG; g++) {
-                const int64_t r = base + k;
-                av[k][g] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (r < r1 && gv[g]) av[k][g] = Quad<T>::load(quad_ptr<T>(A, r, n_cols, lane + 64 * g));
-            }
+        typename Vec<T>::raw rv[R][NG];
+        load_rows<T, NG, R>(A, n_cols, base, r1, lane, gv, rv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            float d = 0.f;
-#pragma unroll
-            for (int g = 0; g < NG; g++) {
-                d = fmaf(av[k][g].x, xv[g].x, d);
-                d = fmaf(av[k][g].y, xv[g].y, d);
-                d = fmaf(av[k][g].z, xv[g].z, d);
-                d = fmaf(av[k][g].w, xv[g].w, d);
-            }
-            d = wave_sum(d);
+            const float d = row_dot<T, NG>(rv[k], xv);
             const int64_t r = base + k;
             if (lane == 0 && r < r1) out[r] = d;
         }
@@ -152,40 +118,29 @@ __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int6
                                                              const float *x, const float *y,
                                                              const float *c, const float *scale,
                                                              double *partial) {
+    constexpr int W = Vec<T>::W;
     __shared__ double red[kMatWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    const int nq = n_cols >> 2;
-    float4 xv[NG];
+    float xv[NG][W];
     bool gv[NG];
-#pragma unroll
-    for (int g = 0; g < NG; g++) {
-        const int qi = lane + 64 * g;
-        gv[g] = qi < nq;
-        xv[g] = gv[g] ? *reinterpret_cast<const float4 *>(x + 4 * qi) : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (scale && gv[g]) {
-            const float4 sc = *reinterpret_cast<const float4 *>(scale + 4 * qi);
-            xv[g] = make_float4(xv[g].x / sc.x, xv[g].y / sc.y, xv[g].z / sc.z, xv[g].w / sc.w);
-        }
-    }
+    load_x<W, NG>(x, scale, n_cols, lane, xv, gv);
     double ll = 0.0;
-    for (int64_t r = r0 + wid; r < r1; r += kMatWaves) {
-        float d = 0.f;
+    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
+        typename Vec<T>::raw rv[R][NG];
+        load_rows<T, NG, R>(A, n_cols, base, r1, lane, gv, rv);
 #pragma unroll
-        for (int g = 0; g < NG; g++) {
-            if (!gv[g]) continue;
-            const float4 av = Quad<T>::load(quad_ptr<T>(A, r, n_cols, lane + 64 * g));
-            d = fmaf(av.x, xv[g].x, d);
-            d = fmaf(av.y, xv[g].y, d);
-            d = fmaf(av.z, xv[g].z, d);
-            d = fmaf(av.w, xv[g].w, d);
+        for (int k = 0; k < R; k++) {
+            const int64_t r = base + k;
+            const float d = row_dot<T, NG>(rv[k], xv);
+            if (r < r1) {
+                const double f = (double)d + (double)c[r] + 1e-20;
+                const double yr = (double)y[r];
+                ll += (yr > 0.0 ? yr * log(f) : 0.0) - f;
+            }
         }
-        d = wave_sum(d);
-        const double f = (double)d + (double)c[r] + 1e-20;
-        const double yr = (double)y[r];
-        ll += (yr > 0.0 ? yr * log(f) : 0.0) - f;
     }
     if (lane == 0) red[wid] = ll;
     __syncthreads();
@@ -298,24 +253,47 @@ int mat_num_blocks(const MatArgs &m) {
 
 static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb - 1) / nb; }
 
-#define XRD_DISPATCH_NG(NGV, ...)                                  \
+// NG = column groups of 64 W columns covering the row (a power of two <= 8).
+#define XRD_DISPATCH_NG(W, NGV, ...)                               \
     do {                                                           \
-        const int ngr = (m.n_cols + 255) / 256;                    \
+        const int ngr = (m.n_cols + 64 * (W)-1) / (64 * (W));      \
         if (ngr <= 1) { constexpr int NGV = 1; __VA_ARGS__; }      \
         else if (ngr <= 2) { constexpr int NGV = 2; __VA_ARGS__; } \
         else if (ngr <= 4) { constexpr int NGV = 4; __VA_ARGS__; } \
-        else if (ngr <= 8) { constexpr int NGV = 8; __VA_ARGS__; } \
-        else return hipErrorInvalidValue;                          \
+        else if (ngr <= 8) {                                       \
+            if constexpr (64 * 8 * (W) <= kMaxMatCols) {           \
+                constexpr int NGV = 8; __VA_ARGS__;                \
+            } else { return hipErrorInvalidValue; }                \
+        } else return hipErrorInvalidValue;                        \
     } while (0)
+
+// Returns CALL evaluated with T = the storage tag of the matrix: 2-byte matrices
+// use 16-byte loads when every row starts 16-byte aligned.
+#define XRD_DISPATCH_DTYPE(m, CALL)                                                  \
+    do {                                                                             \
+        const bool v8 = (m).n_cols % 8 == 0 && ((uintptr_t)(m).A & 15) == 0;         \
+        switch ((m).dtype) {                                                         \
+            case XRD_DTYPE_F32: { using T = float; return CALL; }                    \
+            case XRD_DTYPE_BF16:                                                     \
+                if (v8) { using T = bf16x8; return CALL; }                           \
+                else { using T = bf16x4; return CALL; }                              \
+            case XRD_DTYPE_F16:                                                      \
+                if (v8) { using T = f16x8; return CALL; }                            \
+                else { using T = f16x4; return CALL; }                               \
+        }                                                                            \
+        return hipErrorInvalidValue;                                                 \
+    } while (0)
+
+// Default rows per wave and step: about 4 KB of a row block in flight per wave
+// per row group.
+template <int NG> constexpr int default_rows() { return NG >= 8 ? 1 : (NG >= 4 ? 2 : 4); }
 
 template <typename T, int NG, int MODE>
 static void colsum_launch_r(const MatArgs &m, int nb, int64_t rpb, const float *x,
                             const float *y, const float *c, const float *w, double *partial,
                             hipStream_t s) {
-    // rows per wave and step: the autotuned value for the MLEM pass, else a default
-    // that keeps 8 KB (FP32) of a row block in flight per wave
-    int r = (MODE == kColsMlem && m.rows_per_wave > 0) ? m.rows_per_wave
-                                                       : (NG >= 8 ? 1 : (NG >= 4 ? 2 : 4));
+    // rows per wave and step: the autotuned value for the MLEM pass, else the default
+    const int r = (MODE == kColsMlem && m.rows_per_wave > 0) ? m.rows_per_wave : default_rows<NG>();
     if (r >= 4 && NG <= 4) {
         hipLaunchKernelGGL((colsum_kernel<T, NG, 4, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
                            m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
@@ -333,19 +311,15 @@ static hipError_t colsum_t(const MatArgs &m, const float *x, const float *y, con
                            const float *w, double *partial, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(NG, { colsum_launch_r<T, NG, MODE>(m, nb, rpb, x, y, c, w, partial, s); });
+    XRD_DISPATCH_NG(Vec<T>::W, NG,
+                    { colsum_launch_r<T, NG, MODE>(m, nb, rpb, x, y, c, w, partial, s); });
     return hipGetLastError();
 }
 
 template <int MODE>
 static hipError_t colsum_dtype(const MatArgs &m, const float *x, const float *y, const float *c,
                                const float *w, double *partial, hipStream_t s) {
-    switch (m.dtype) {
-        case XRD_DTYPE_F32: return colsum_t<float, MODE>(m, x, y, c, w, partial, s);
-        case XRD_DTYPE_BF16: return colsum_t<bf16x4, MODE>(m, x, y, c, w, partial, s);
-        case XRD_DTYPE_F16: return colsum_t<f16x4, MODE>(m, x, y, c, w, partial, s);
-    }
-    return hipErrorInvalidValue;
+    XRD_DISPATCH_DTYPE(m, (colsum_t<T, MODE>(m, x, y, c, w, partial, s)));
 }
 
 hipError_t launch_colsum_partials(const MatArgs &m, const float *w, double *partial,
@@ -363,8 +337,8 @@ template <typename T>
 static hipError_t project_t(const MatArgs &m, const float *x, float *out, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(NG, {
-        constexpr int R = NG >= 8 ? 1 : (NG >= 4 ? 2 : 4);
+    XRD_DISPATCH_NG(Vec<T>::W, NG, {
+        constexpr int R = default_rows<NG>();
         hipLaunchKernelGGL((project_kernel<T, NG, R>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
                            m.n_rows, m.n_cols, rpb, x, m.scale, out);
     });
@@ -372,12 +346,7 @@ static hipError_t project_t(const MatArgs &m, const float *x, float *out, hipStr
 }
 
 hipError_t launch_project(const MatArgs &m, const float *x, float *out, hipStream_t s) {
-    switch (m.dtype) {
-        case XRD_DTYPE_F32: return project_t<float>(m, x, out, s);
-        case XRD_DTYPE_BF16: return project_t<bf16x4>(m, x, out, s);
-        case XRD_DTYPE_F16: return project_t<f16x4>(m, x, out, s);
-    }
-    return hipErrorInvalidValue;
+    XRD_DISPATCH_DTYPE(m, (project_t<T>(m, x, out, s)));
 }
 
 template <typename T>
@@ -385,7 +354,7 @@ static hipError_t loglik_t(const MatArgs &m, const float *x, const float *y, con
                            double *partial, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(NG, {
+    XRD_DISPATCH_NG(Vec<T>::W, NG, {
         hipLaunchKernelGGL((loglik_kernel<T, NG, 1>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
                            m.n_rows, m.n_cols, rpb, x, y, c, m.scale, partial);
     });
@@ -394,12 +363,7 @@ static hipError_t loglik_t(const MatArgs &m, const float *x, const float *y, con
 
 hipError_t launch_loglik_partials(const MatArgs &m, const float *x, const float *y,
                                   const float *c, double *partial, hipStream_t s) {
-    switch (m.dtype) {
-        case XRD_DTYPE_F32: return loglik_t<float>(m, x, y, c, partial, s);
-        case XRD_DTYPE_BF16: return loglik_t<bf16x4>(m, x, y, c, partial, s);
-        case XRD_DTYPE_F16: return loglik_t<f16x4>(m, x, y, c, partial, s);
-    }
-    return hipErrorInvalidValue;
+    XRD_DISPATCH_DTYPE(m, (loglik_t<T>(m, x, y, c, partial, s)));
 }
 
 hipError_t launch_reduce_partials(const double *partial, int nblocks, int n_cols, double *out,

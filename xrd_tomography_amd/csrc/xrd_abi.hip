@@ -478,7 +478,8 @@ int check_matrix(const void *A, int dtype, int64_t n_rows, int32_t n_cols, const
         return fail(XRD_ERR_INVALID_ARG, "%s: unknown dtype %d", fn, dtype);
     if (n_rows <= 0 || n_cols <= 0) return fail(XRD_ERR_DIMS, "%s: non-positive size", fn);
     if (n_cols % 4 != 0) return fail(XRD_ERR_DIMS, "%s: n_cols must be a multiple of 4", fn);
-    if (n_cols > 2048) return fail(XRD_ERR_UNSUPPORTED, "%s: n_cols > 2048", fn);
+    if (n_cols > kMaxMatCols)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: n_cols > %d", fn, kMaxMatCols);
     if (!aligned(A, dtype == XRD_DTYPE_F32 ? 16 : 8))
         return fail(XRD_ERR_ALIGN, "%s: A misaligned", fn);
     return XRD_OK;
@@ -600,6 +601,8 @@ extern "C" int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype
     if (rc) return rc;
     if (!y || !c || !x) return fail(XRD_ERR_NULL_POINTER, "%s: y/c/x is NULL", fn);
     if (!aligned(x, 16)) return fail(XRD_ERR_ALIGN, "%s: x must be 16-byte aligned", fn);
+    if (col_scale && !aligned(col_scale, 16))
+        return fail(XRD_ERR_ALIGN, "%s: col_scale must be 16-byte aligned", fn);
     if (mode != XRD_MLEM_FUSED && mode != XRD_MLEM_UNFUSED)
         return fail(XRD_ERR_INVALID_ARG, "%s: unknown mode %d", fn, mode);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
@@ -984,11 +987,15 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     std::vector<BatchProblem> probs(n_problems);
     std::vector<BatchTile> tiles;
     int max_cols = 0;
+    int vec8 = 1;
     for (int k = 0; k < n_problems; k++) {
         int rc = check_matrix(A[k], a_dtype, n_rows[k], n_cols[k], fn);
         if (rc) return rc;
+        if (n_cols[k] % 8 != 0 || !aligned(A[k], 16)) vec8 = 0;
         if (!y[k] || !c[k] || !x[k]) return fail(XRD_ERR_NULL_POINTER, "%s: y/c/x[%d] is NULL", fn, k);
         if (!aligned(x[k], 16)) return fail(XRD_ERR_ALIGN, "%s: x[%d] misaligned", fn, k);
+        if (col_scale && col_scale[k] && !aligned(col_scale[k], 16))
+            return fail(XRD_ERR_ALIGN, "%s: col_scale[%d] misaligned", fn, k);
         BatchProblem &p = probs[k];
         p.A = A[k];
         p.n_rows = n_rows[k];
@@ -1043,6 +1050,7 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     a.dtype = a_dtype;
     a.grid = batch_grid(b->blocks_per_cu);
     a.rows_per_wave = 0;
+    a.vec8 = vec8;
     a.partial = (double *)b->partial.ptr;
     // sensitivities sum(A_k,1)' once, FP64
     if (e == hipSuccess) e = launch_batch_stream(a, false, s);

@@ -129,6 +129,8 @@ struct MatArgs {
 
 // Largest blocks_per_cu the partial buffers must be sized for.
 constexpr int kMaxBlocksPerCu = 8;
+// Widest matrix row the streaming kernels hold in registers.
+constexpr int kMaxMatCols = 2048;
 
 // Per-column maxima of |A| (FP32 matrix) and power-of-two scales that map each
 // column's maximum to about half the largest finite value of dtype (BF16/F16).
@@ -187,6 +189,7 @@ struct BatchArgs {
     int n_problems, n_tiles, max_cols, stride, dtype;
     int grid;             // persistent workgroups
     int rows_per_wave;    // 0 = default
+    int vec8;             // 2-byte matrices: every n_cols % 8 == 0 and A 16-byte aligned
     double *partial;      // [n_tiles x stride]
 };
 

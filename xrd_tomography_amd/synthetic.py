@@ -360,3 +360,21 @@ def make_coeff_problem(prob: Problem, K: int = 16, full_mu: bool = True,
         beta = np.stack([b1, b2])
     return CoeffProblem(base=prob, coef=coef.astype(np.float32), basis=basis.astype(np.float32),
                         maps=maps.astype(np.float32), beta=beta.astype(np.float32))
+
+
+def make_batch_problems(n: int = 48, seed: int = 4, n_cols: int = 64):
+    """The batched workload: n phantoms with sizes drawn uniformly from
+    {64, 96, 128, 192, 256}^2 voxels over a 512 mm field of view, view counts uniform in
+    90..360, 2..10 materials each, 61 energies, 128 q-bins, n_cols detector pixels.
+    Every phantom has its own seed (seed*1000 + k) for its materials and shapes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.choice([64, 96, 128, 192, 256], n)
+    views = rng.integers(90, 361, n)
+    mats = rng.integers(2, 11, n)
+    out = []
+    for k in range(n):
+        nx = int(sizes[k])
+        out.append(make_problem("fan256", seed=seed * 1000 + k, nx=nx, voxel_mm=512.0 / nx,
+                                n_views=int(views[k]), n_cols=n_cols, n_mat=int(mats[k]),
+                                photons_per_view=1e5 * n_cols / 256.0, mlem_iterations=30))
+    return out
