@@ -47,7 +47,9 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="B,R: fix the fused pass to B workgroups per CU and R rows per wave "
                          "(no autotuning)")
-    ap.add_argument("--cpu-views", type=int, default=12, help="views in the CPU-baseline sample")
+    ap.add_argument("--cpu-views", type=int, default=36, help="views in the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="repeat the CPU-baseline update until this much time has passed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device-to-device copy 10 times (calibrates HBM "
@@ -181,17 +183,24 @@ def main():
         live = xo > 1e-6 * xo.max()
         rel_err = float(np.max(np.abs(xg.cpu().numpy()[live] - xo[live]) / xo[live]))
         threads = O.max_threads()
-        reps = 3
-        t0 = time.perf_counter()
-        O.mlem(A64, y64, c64, x0, reps)
-        t_cpu = (time.perf_counter() - t0) / reps
+        # one update at a time until --cpu-seconds have passed (at least 5); median
+        s64 = O.sensitivity(A64)
+        xc = x0.copy()
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 5 or time.perf_counter() - t_start < args.cpu_seconds:
+            t0 = time.perf_counter()
+            O.mlem_update(A64, y64, c64, s64, xc)
+            times.append(time.perf_counter() - t0)
+        t_cpu = float(np.median(times))
         frac = len(rows) / nrow
         cpu_iters_s = frac / t_cpu
         cpu = {"value": round(cpu_iters_s, 4), "unit": "iters/s", "cores": threads,
                "kind": "port",
                "sample": f"FP64 oracle MLEM update on {nv} of {g.n_views} views "
-                         f"({len(rows)} rows x {ncol} cols, {A64.nbytes / 1e9:.2f} GB FP64), "
-                         f"{reps} updates in {t_cpu * reps:.1f} s, scaled x{1 / frac:.1f} to all rows"}
+                         f"({len(rows)} rows x {ncol} cols, {A64.nbytes / 1e9:.2f} GB FP64): "
+                         f"median of {len(times)} updates ({sum(times):.1f} s in total) "
+                         f"= {t_cpu * 1e3:.1f} ms, scaled x{1 / frac:.1f} to all rows"}
         log(f"[bench] cpu baseline {cpu_iters_s:.3f} iters/s on {threads} threads; "
             f"GPU-vs-FP64 MLEM rel err after {k_err} steps {rel_err:.2e}")
         del A64
@@ -282,21 +291,8 @@ def run_batch(args, dev):
     bytes_it = sum(p[0].shape[0] * p[0].shape[1] * esize + p[0].shape[0] * 8 + p[0].shape[1] * 12
                    for p in stored)
     batch = api.MlemBatch(stored)
-    best = None
-    for b in (1, 2, 3, 4):
-        for r in (1, 2, 4):
-            batch.set_launch(b, r)
-            batch.iterate(1)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            batch.iterate(2)
-            e1.record()
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / 2
-            if best is None or ms < best[0]:
-                best = (ms, b, r)
-    batch.set_launch(best[1], best[2])
+    tuned = batch.autotune(reps=2)
+    log(f"[batch] autotuned launch: {tuned}")
     for p in stored:
         p[3].fill_(10.0)
     batch.iterate(1, use_graph=True)  # capture
@@ -309,8 +305,10 @@ def run_batch(args, dev):
     e1.record()
     torch.cuda.synchronize()
     ms_it = e0.elapsed_time(e1) / n_it
-    log(f"[batch] {args.dtype} batched: {ms_it:.2f} ms per iteration over {len(stored)} phantoms,"
-        f" {bytes_it / ms_it / 1e6:.0f} GB/s, launch {best[1:]}")
+    ms_pass = batch.time_pass(reps=10)
+    log(f"[batch] {args.dtype} batched: {ms_it:.2f} ms per iteration over {len(stored)} phantoms "
+        f"({bytes_it / ms_it / 1e6:.0f} GB/s); streaming pass alone {ms_pass:.2f} ms "
+        f"({bytes_it / ms_pass / 1e6:.0f} GB/s)")
 
     # baselines: the same phantoms one at a time (a) through the fused single-matrix
     # plan on the same stored matrices, (b) through the plain unfused FP32 path
@@ -356,10 +354,12 @@ def run_batch(args, dev):
         "data": "synthetic",
         "config": {"workload": "batch48: 48 phantoms, sizes {64..256}^2 voxels, 90-360 views, "
                                "64 pixels x 61 energies, 128 q-bins x 2-10 materials",
-                   "rows": int(sum(p[0].shape[0] for p in stored)), "launch": list(best[1:])},
-        "roofline": {"bound": "hbm", "achieved": round(bytes_it / ms_it / 1e6, 1),
+                   "rows": int(sum(p[0].shape[0] for p in stored)), "launch": tuned},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_it / ms_pass / 1e6, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(bytes_it / ms_it / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(bytes_it / ms_pass / 1e6 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(ms_pass, 4), "algorithmic_bytes": bytes_it,
+                     "iteration_GBps": round(bytes_it / ms_it / 1e6, 1)},
         "one_at_a_time_fused": {"ms_per_step": round(ms_single_fused, 3),
                                 "speedup": round(ms_single_fused / ms_it, 3)},
         "baseline_one_at_a_time_fp32_unfused": {"ms_per_step": round(ms_base, 3),

@@ -282,21 +282,35 @@ int xrd_mlem_plan_destroy(xrd_mlem_plan *plan);
  * MLEM for n_problems independent explicit matrices (phantoms or slices of uneven
  * size and view count) with one persistent launch per pass over all of them.
  * Host arrays of n_problems device pointers/sizes are read at creation; col_scale
- * may be NULL or hold NULL entries.  Every problem's rows are cut into tiles of
- * rows_per_tile rows (<= 0: 8192); results do not depend on the tile schedule.
- * The sensitivities are computed (FP64) at creation. */
+ * may be NULL or hold NULL entries; reserved must be 0.  The problems' rows are
+ * cut into one contiguous chunk of about equal cost per persistent workgroup, and
+ * each problem is streamed with a row shape picked from its column count.
+ * Results are deterministic for a given launch shape.  The sensitivities are
+ * computed (FP64) at creation.  No reference counterpart: the reference runs one
+ * phantom per MATLAB session (main.m). */
 typedef struct xrd_mlem_batch xrd_mlem_batch;
 
 int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a_dtype,
                           const void *const *A, const int64_t *n_rows, const int32_t *n_cols,
                           const float *const *col_scale, const float *const *y,
-                          const float *const *c, float *const *x, int32_t rows_per_tile,
+                          const float *const *c, float *const *x, int32_t reserved,
                           void *stream, xrd_mlem_batch **out);
 /* n_iter updates of every x[k]; use_graph replays a captured iteration. */
 int xrd_mlem_batch_iterate(xrd_mlem_batch *batch, int32_t n_iter, int use_graph, void *stream);
-/* Launch shape of the persistent pass (0 = default for either field). */
+/* Launch shape of the persistent pass (0 = default for either field): workgroups
+ * per CU (clamped to what fits on a CU at once) and rows per row slot and step.
+ * Waits for the device, then re-partitions the rows. */
 int xrd_mlem_batch_set_launch(xrd_mlem_batch *batch, int32_t blocks_per_cu,
                               int32_t rows_per_wave);
+/* Times the streaming pass (reps launches, x untouched) over blocks_per_cu in
+ * {1,2,3,4} (those that fit on a CU) and rows_per_wave in {1,2,4}, keeps the
+ * fastest and returns it with its mean pass time in ms (outputs may be NULL). */
+int xrd_mlem_batch_autotune(xrd_mlem_batch *batch, int32_t reps, void *stream,
+                            int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
+                            float *out_ms);
+/* Mean duration in ms of reps streaming passes at the current launch shape,
+ * measured with HIP events on stream (x untouched). */
+int xrd_mlem_batch_time_pass(xrd_mlem_batch *batch, int32_t reps, void *stream, float *out_ms);
 int xrd_mlem_batch_destroy(xrd_mlem_batch *batch);
 
 #ifdef __cplusplus

@@ -66,6 +66,9 @@ def lib():
         L.xo_mlem.restype = None
         L.xo_mlem.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                               ctypes.c_int]
+        L.xo_mlem_update.restype = None
+        L.xo_mlem_update.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                     ctypes.c_int]
         L.xo_loglik.restype = ctypes.c_double
         L.xo_loglik.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
         L.xo_max_threads.restype = ctypes.c_int
@@ -194,6 +197,17 @@ def mlem(A, y, c, x0, n_iter, nthreads=0):
     c, cp = _d(c)
     x = np.array(x0, dtype=np.float64, copy=True)
     lib().xo_mlem(Ap, A.shape[0], A.shape[1], yp, cp, x.ctypes.data_as(_dp), n_iter, nthreads)
+    return x
+
+
+def mlem_update(A, y, c, s, x, nthreads=0):
+    """One MLEM update of x (float64 array) in place given s = sum(A, 1)'."""
+    A, Ap = _d(A)
+    y, yp = _d(y)
+    c, cp = _d(c)
+    s, sp = _d(s)
+    assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == (A.shape[1],)
+    lib().xo_mlem_update(Ap, A.shape[0], A.shape[1], yp, cp, sp, x.ctypes.data_as(_dp), nthreads)
     return x
 
 

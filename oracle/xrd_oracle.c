@@ -540,27 +540,30 @@ void xo_sensitivity(const double *A, int64_t n_rows, int n_cols, double *out, in
     xo_colsum(A, n_rows, n_cols, NULL, out, nthreads);
 }
 
-void xo_mlem(const double *A, int64_t n_rows, int n_cols, const double *y, const double *c,
-             double *x, int n_iter, int nthreads) {
+void xo_mlem_update(const double *A, int64_t n_rows, int n_cols, const double *y,
+                    const double *c, const double *s, double *x, int nthreads) {
     double *f = (double *)malloc(sizeof(double) * n_rows);
     double *g = (double *)malloc(sizeof(double) * n_cols);
-    double *s = (double *)malloc(sizeof(double) * n_cols);
-    xo_sensitivity(A, n_rows, n_cols, s, nthreads);
     const int nt = xo_threads(nthreads);
-    for (int it = 0; it < n_iter; it++) {
-        xo_project(A, n_rows, n_cols, x, f, nthreads);
+    xo_project(A, n_rows, n_cols, x, f, nthreads);
 #pragma omp parallel for schedule(static) num_threads(nt)
-        for (int64_t r = 0; r < n_rows; r++) f[r] = (y[r] - 0.0 * c[r]) / (f[r] + c[r] + 1e-20);
-        xo_backproject(A, n_rows, n_cols, f, g, nthreads);
-        for (int j = 0; j < n_cols; j++) {
-            double v = x[j] * g[j] / (s[j] + 1e-7);
-            if (!isfinite(v)) v = 0.0; /* main.m:163 */
-            if (v < 0.0) v = 0.0;      /* main.m:176 */
-            x[j] = v;
-        }
+    for (int64_t r = 0; r < n_rows; r++) f[r] = (y[r] - 0.0 * c[r]) / (f[r] + c[r] + 1e-20);
+    xo_backproject(A, n_rows, n_cols, f, g, nthreads);
+    for (int j = 0; j < n_cols; j++) {
+        double v = x[j] * g[j] / (s[j] + 1e-7);
+        if (!isfinite(v)) v = 0.0; /* main.m:163 */
+        if (v < 0.0) v = 0.0;      /* main.m:176 */
+        x[j] = v;
     }
     free(f);
     free(g);
+}
+
+void xo_mlem(const double *A, int64_t n_rows, int n_cols, const double *y, const double *c,
+             double *x, int n_iter, int nthreads) {
+    double *s = (double *)malloc(sizeof(double) * n_cols);
+    xo_sensitivity(A, n_rows, n_cols, s, nthreads);
+    for (int it = 0; it < n_iter; it++) xo_mlem_update(A, n_rows, n_cols, y, c, s, x, nthreads);
     free(s);
 }
 

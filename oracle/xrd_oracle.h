@@ -91,6 +91,9 @@ void xo_backproject(const double *A, int64_t n_rows, int n_cols, const double *w
                     double *out, int nthreads);
 void xo_sensitivity(const double *A, int64_t n_rows, int n_cols, double *out, int nthreads);
 
+/* One MLEM update of x in place given the sensitivity s = sum(A,1)' (main.m:162-163,176). */
+void xo_mlem_update(const double *A, int64_t n_rows, int n_cols, const double *y,
+                    const double *c, const double *s, double *x, int nthreads);
 /* n_iter MLEM updates of x in place (main.m:159-176). */
 void xo_mlem(const double *A, int64_t n_rows, int n_cols, const double *y, const double *c,
              double *x, int n_iter, int nthreads);

@@ -189,7 +189,8 @@ def test_batched_mlem_matches_single(xrd, oracle, dtype, wide):
         singles.append(xrd.mlem(Ad, yd, cd, 8, col_scale=sc))
         Aeff = Ad.double().cpu().numpy() / (sc.double().cpu().numpy()[None, :] if sc is not None else 1.0)
         refs.append(oracle.mlem(Aeff, y, c, np.full(cc, 10.0), 8))
-    batch = xrd.MlemBatch(probs, rows_per_tile=1000)
+    batch = xrd.MlemBatch(probs)
+    batch.set_launch(1, 1)  # 256 workgroups: several problems share a workgroup's chunk
     batch.iterate(4, use_graph=False)
     batch.iterate(4, use_graph=True)
     torch.cuda.synchronize()

@@ -32,7 +32,8 @@ EXPORTS = (
     "xrd_mlem_plan_destroy", "xrd_forward_coeffs", "xrd_vjp_coeffs", "xrd_poisson_nll",
     "xrd_convert_matrix_scaled", "xrd_mlem_plan_autotune", "xrd_mlem_plan_set_launch",
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
-    "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass",
+    "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
+    "xrd_mlem_batch_time_pass",
 )
 
 
@@ -83,6 +84,11 @@ def _declare(L):
         "xrd_mlem_batch_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),
         "xrd_mlem_batch_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
         "xrd_mlem_batch_destroy": (ctypes.c_int, [_vp]),
+        "xrd_mlem_batch_autotune": (ctypes.c_int, [_vp, _i32, _vp, ctypes.POINTER(_i32),
+                                                   ctypes.POINTER(_i32),
+                                                   ctypes.POINTER(ctypes.c_float)]),
+        "xrd_mlem_batch_time_pass": (ctypes.c_int, [_vp, _i32, _vp,
+                                                    ctypes.POINTER(ctypes.c_float)]),
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),

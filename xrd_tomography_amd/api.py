@@ -397,7 +397,7 @@ class MlemBatch:
     per pass (xrd_mlem_batch_*).  problems: list of (A, y, c, x[, col_scale]); all A share
     one dtype; every x is updated in place."""
 
-    def __init__(self, problems, rows_per_tile: int = 8192, ctx=None, stream=None):
+    def __init__(self, problems, ctx=None, stream=None):
         if not problems:
             raise ValueError("no problems")
         self.ctx = ctx or default_context(problems[0][0].device.index or 0)
@@ -419,7 +419,7 @@ class MlemBatch:
         self._keep = problems
         h = ctypes.c_void_p()
         check(lib().xrd_mlem_batch_create(self.ctx.handle, n, _DTYPES[dts.pop()], A, rows, cols,
-                                          sc, y, c, x, int(rows_per_tile), _stream(stream),
+                                          sc, y, c, x, 0, _stream(stream),
                                           ctypes.byref(h)), "xrd_mlem_batch_create")
         self.handle = h
 
@@ -430,6 +430,21 @@ class MlemBatch:
     def set_launch(self, blocks_per_cu: int = 0, rows_per_wave: int = 0):
         check(lib().xrd_mlem_batch_set_launch(self.handle, blocks_per_cu, rows_per_wave),
               "xrd_mlem_batch_set_launch")
+
+    def autotune(self, reps: int = 2, stream=None) -> dict:
+        """Pick the fastest launch shape of the streaming pass (x is not modified)."""
+        b, r, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        check(lib().xrd_mlem_batch_autotune(self.handle, int(reps), _stream(stream),
+                                            ctypes.byref(b), ctypes.byref(r), ctypes.byref(ms)),
+              "xrd_mlem_batch_autotune")
+        return {"blocks_per_cu": b.value, "rows_per_wave": r.value, "ms_pass": ms.value}
+
+    def time_pass(self, reps: int = 10, stream=None) -> float:
+        """Mean ms of one streaming pass over all problems, HIP events on the stream."""
+        ms = ctypes.c_float()
+        check(lib().xrd_mlem_batch_time_pass(self.handle, int(reps), _stream(stream),
+                                             ctypes.byref(ms)), "xrd_mlem_batch_time_pass")
+        return ms.value
 
     def close(self):
         if getattr(self, "handle", None):

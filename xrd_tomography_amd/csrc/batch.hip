@@ -1,47 +1,52 @@
 // batch.hip -- MLEM for many independent problems (phantoms or slices of uneven
-// size and view count) in one persistent launch per pass.
+// size and view count) with persistent launches over all of them.
 //
-// Each problem's model matrix is cut into tiles of consecutive rows.  A
-// persistent grid walks the global tile list (tile t, t + grid, ...); each tile
-// is streamed exactly like the single-matrix fused pass (matrix_stream.hpp) and
-// its FP64 column sums are written to a per-tile partial slot.  The update
-// kernel then sums, per problem, the partials of that problem's tiles in tile
-// order, so results do not depend on which workgroup processed which tile.
+// Problems are grouped by row span (the column count rounded up to 256, 512,
+// 1024 or 2048, which fixes how many lanes share a row).  For each group the
+// problems' rows are laid end to end and cut into one contiguous chunk per
+// workgroup of about equal cost (bytes of A plus a per-row charge for the row's
+// reduction and ratio); a chunk is a short list of segments, each a row range of
+// one problem.  One persistent launch per group streams every segment and writes
+// its FP64 column sums to the segment's own partial slot; a pass is at most four
+// such launches (one per span present) and one update launch.  The update kernel
+// adds, per problem, the partials of its segments in segment order, so the result
+// depends only on the launch shape, never on timing.  A kernel per span (instead
+// of one kernel branching on the span) keeps each at the registers of its own
+// row shape.
 #include "matrix_stream.hpp"
 #include "xrd_internal.hpp"
 
 namespace xrd {
 
-template <typename T, int NG, int R, int MODE>
+// Rows per row slot and step for a span: at most 2 for rows whose FP64 column
+// sums take 64 registers per lane.
+template <typename T, int SPAN, int R> constexpr int batch_rows() {
+    typedef RowShape<Vec<T>::W, SPAN> Sh;
+    static_assert(Sh::S * SPAN <= kBatchRedDoubles, "block reduction exceeds the LDS buffer");
+    return (Sh::NG * Vec<T>::W >= 32 && R > 2) ? 2 : R;
+}
+
+template <typename T, int SPAN, int R, int MODE>
 __global__ __launch_bounds__(kMatThreads) void batch_stream_kernel(const BatchProblem *probs,
-                                                                   const BatchTile *tiles,
-                                                                   int n_tiles, int stride,
+                                                                   const BatchSegment *segs,
+                                                                   const int *wg_seg, int stride,
                                                                    double *partial) {
-    constexpr int W = Vec<T>::W;
-    __shared__ double red[kMatWaves * NG * 64 * W];
-    const int lane = threadIdx.x & 63;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const BatchTile tl = tiles[t];
-        const BatchProblem pb = probs[tl.problem];
-        float xv[NG][W];
-        bool gv[NG];
-        load_x<W, NG>(MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.n_cols, lane, xv, gv);
-        double acc[NG][W];
-#pragma unroll
-        for (int g = 0; g < NG; g++)
-#pragma unroll
-            for (int i = 0; i < W; i++) acc[g][i] = 0.0;
-        stream_rows<T, NG, R, MODE>(pb.A, pb.n_cols, tl.r0, tl.r1, xv, gv, pb.y, pb.c, nullptr,
-                                    acc);
-        block_column_sums<NG, W>(acc, red, pb.n_cols, partial + (size_t)t * stride);
+    __shared__ double red[RowShape<Vec<T>::W, SPAN>::S * SPAN];
+    const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
+    for (int si = s0; si < s1; si++) {
+        const BatchSegment sg = segs[si];
+        const BatchProblem pb = probs[sg.problem];
+        colsum_block<T, SPAN, batch_rows<T, SPAN, R>(), MODE>(
+            pb.A, pb.n_cols, sg.r0, sg.r1, MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.y,
+            pb.c, nullptr, red, partial + (size_t)si * stride);
         __syncthreads();
     }
 }
 
 // grid (ceil(max_cols/64), n_problems), 1024 lanes: for problem blockIdx.y and 64
-// columns, sum the problem's tile partials in tile order (16 waves take every 16th
-// tile, then the wave sums are added in wave order); then either write the
-// de-scaled sensitivity or apply the MLEM update (main.m:162-163,176).
+// columns, sum the problem's segment partials in segment order (16 waves take
+// every 16th segment, then the wave sums are added in wave order); then either
+// write the de-scaled sensitivity or apply the MLEM update (main.m:162-163,176).
 __global__ __launch_bounds__(1024) void batch_update_kernel(const BatchProblem *probs,
                                                             const double *partial, int stride,
                                                             double *sens_out, int update) {
@@ -51,7 +56,7 @@ __global__ __launch_bounds__(1024) void batch_update_kernel(const BatchProblem *
     const int j = blockIdx.x * 64 + lane;
     double s = 0.0;
     if (j < pb.n_cols)
-        for (int t = pb.tile_begin + wid; t < pb.tile_end; t += 16)
+        for (int t = pb.seg_begin + wid; t < pb.seg_end; t += 16)
             s += partial[(size_t)t * stride + j];
     red[wid][lane] = s;
     __syncthreads();
@@ -71,46 +76,64 @@ __global__ __launch_bounds__(1024) void batch_update_kernel(const BatchProblem *
     }
 }
 
+template <typename T, int SPAN, int MODE>
+static const void *batch_kernel_r(int R) {
+    if (R >= 4) return reinterpret_cast<const void *>(&batch_stream_kernel<T, SPAN, 4, MODE>);
+    if (R >= 2) return reinterpret_cast<const void *>(&batch_stream_kernel<T, SPAN, 2, MODE>);
+    return reinterpret_cast<const void *>(&batch_stream_kernel<T, SPAN, 1, MODE>);
+}
+
 template <typename T, int MODE>
-static hipError_t batch_stream_t(const BatchArgs &b, hipStream_t s) {
-    constexpr int W = Vec<T>::W;
-    const int ngr = (b.max_cols + 64 * W - 1) / (64 * W);
-    const int R = b.rows_per_wave > 0 ? b.rows_per_wave : 2;
-#define XRD_BATCH_LAUNCH(NGV, RV)                                                              \
-    hipLaunchKernelGGL((batch_stream_kernel<T, NGV, RV, MODE>), dim3(b.grid), dim3(kMatThreads), \
-                       0, s, b.probs, b.tiles, b.n_tiles, b.stride, b.partial)
-#define XRD_BATCH_R(NGV)                          \
-    do {                                          \
-        if (R >= 4 && NGV <= 4) XRD_BATCH_LAUNCH(NGV, 4); \
-        else if (R >= 2) XRD_BATCH_LAUNCH(NGV, 2);        \
-        else XRD_BATCH_LAUNCH(NGV, 1);                    \
-    } while (0)
-    if (ngr <= 1) XRD_BATCH_R(1);
-    else if (ngr <= 2) XRD_BATCH_R(2);
-    else if (ngr <= 4) XRD_BATCH_R(4);
-    else if constexpr (64 * 8 * W <= kMaxMatCols) {
-        if (ngr <= 8) XRD_BATCH_R(8);
-        else return hipErrorInvalidValue;
-    } else return hipErrorInvalidValue;
-#undef XRD_BATCH_R
-#undef XRD_BATCH_LAUNCH
-    return hipGetLastError();
+static const void *batch_kernel_span(int span, int R) {
+    switch (span) {
+        case 256: return batch_kernel_r<T, 256, MODE>(R);
+        case 512: return batch_kernel_r<T, 512, MODE>(R);
+        case 1024: return batch_kernel_r<T, 1024, MODE>(R);
+        case 2048: return batch_kernel_r<T, 2048, MODE>(R);
+    }
+    return nullptr;
 }
 
 template <typename T>
-static hipError_t batch_stream_mode(const BatchArgs &b, bool mlem, hipStream_t s) {
-    return mlem ? batch_stream_t<T, kColsMlem>(b, s) : batch_stream_t<T, kColsOnes>(b, s);
+static const void *batch_kernel_mode(bool mlem, int span, int R) {
+    return mlem ? batch_kernel_span<T, kColsMlem>(span, R) : batch_kernel_span<T, kColsOnes>(span, R);
+}
+
+static const void *batch_kernel(const BatchArgs &b, int span, bool mlem) {
+    const int R = b.rows_per_wave > 0 ? b.rows_per_wave : 2;
+    switch (b.dtype) {
+        case XRD_DTYPE_F32: return batch_kernel_mode<float>(mlem, span, R);
+        case XRD_DTYPE_BF16:
+            return b.vec8 ? batch_kernel_mode<bf16x8>(mlem, span, R)
+                          : batch_kernel_mode<bf16x4>(mlem, span, R);
+        case XRD_DTYPE_F16:
+            return b.vec8 ? batch_kernel_mode<f16x8>(mlem, span, R)
+                          : batch_kernel_mode<f16x4>(mlem, span, R);
+    }
+    return nullptr;
+}
+
+hipError_t batch_resident_blocks(const BatchArgs &b, int span, int *blocks_per_cu) {
+    const void *k = batch_kernel(b, span, true);
+    if (!k) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, kMatThreads, 0);
 }
 
 hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s) {
-    switch (b.dtype) {
-        case XRD_DTYPE_F32: return batch_stream_mode<float>(b, mlem, s);
-        case XRD_DTYPE_BF16:
-            return b.vec8 ? batch_stream_mode<bf16x8>(b, mlem, s) : batch_stream_mode<bf16x4>(b, mlem, s);
-        case XRD_DTYPE_F16:
-            return b.vec8 ? batch_stream_mode<f16x8>(b, mlem, s) : batch_stream_mode<f16x4>(b, mlem, s);
+    for (int g = 0; g < b.n_groups; g++) {
+        const BatchGroup &gr = b.groups[g];
+        const void *k = batch_kernel(b, gr.span, mlem);
+        if (!k) return hipErrorInvalidValue;
+        const BatchProblem *probs = b.probs;
+        const BatchSegment *segs = b.segs;
+        const int *wg_seg = b.wg_seg + gr.wg_seg_offset;
+        int stride = b.stride;
+        double *partial = b.partial;
+        void *args[] = {&probs, &segs, &wg_seg, &stride, &partial};
+        hipError_t e = hipLaunchKernel(k, dim3(gr.grid), dim3(kMatThreads), args, 0, s);
+        if (e != hipSuccess) return e;
     }
-    return hipErrorInvalidValue;
+    return hipSuccess;
 }
 
 hipError_t launch_batch_update(const BatchArgs &b, double *sens_out, bool update, hipStream_t s) {

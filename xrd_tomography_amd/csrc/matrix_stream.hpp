@@ -3,9 +3,12 @@
 //
 // Every lane loads W consecutive elements of a row per column group with one
 // 16-byte load (FP32: W = 4; BF16/F16: W = 8), or one 8-byte load for 2-byte
-// matrices whose column count is not a multiple of 8 (W = 4).  A column group
-// is 64 W columns wide; a wave covers NG groups of a row.  Lane l of group g
-// holds columns W (l + 64 g) + i, i < W.
+// matrices whose column count is not a multiple of 8 (W = 4).  L lanes share a
+// row (L = 16, 32 or 64), so a wave works on S = 64/L rows side by side; lane li
+// of its row slot holds columns W (li + L g) + i for column groups g < NG.  A row
+// shape (L, NG) covers SPAN = L NG W columns; narrow rows use small L so that a
+// row's dot product is reduced over fewer lanes and its FP64 ratio is formed for
+// S rows at once.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -112,15 +115,48 @@ template <> struct Vec<f16x8> {
 
 enum { kColsOnes = 0, kColsWeighted = 1, kColsMlem = 2 };
 
+// Row shape of SPAN columns at W elements per load: L lanes per row, NG column
+// groups.  SPAN is a power of two in [256, 2048]; a lane holds 16 columns (32 for
+// SPAN 2048), which bounds its FP64 column sums to 32 (64) registers.
+template <int W, int SPAN> struct RowShape {
+    static constexpr int L = SPAN / 16 < 16 ? 16 : (SPAN / 16 > 64 ? 64 : SPAN / 16);
+    static constexpr int NG = SPAN / (L * W);
+    static constexpr int S = kWave / L;
+    static_assert(L * NG * W == SPAN, "row shape must tile the span");
+};
+
+// v + v of the lane that DPP control CTRL selects (a permutation within rows of
+// 16 lanes).
+template <int CTRL> __device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf,
+                                                          false));
+}
+
+// Sum over the L lanes (16, 32 or 64) of each row slot; every lane of the slot
+// gets the same bits.  Within 16 lanes: swap neighbours (quad_perm 1,0,3,2), swap
+// pairs (quad_perm 2,3,0,1), swap quads (row_half_mirror), swap halves
+// (row_mirror); each step adds two equal-rank partial sums, so both partners get
+// identical results.  Then lanes 16 and 32 apart are added.
+template <int L> __device__ __forceinline__ float group_sum(float v) {
+    static_assert(L == 16 || L == 32 || L == 64, "row slots are 16, 32 or 64 lanes");
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    v = dpp_add<0x141>(v);
+    v = dpp_add<0x140>(v);
+    if (L >= 32) v += __shfl_xor(v, 16, kWave);
+    if (L >= 64) v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
 // x (optionally divided by the per-column storage scale) in the lane's columns
-// W (lane + 64 g) + i; gv[g] marks column groups inside the matrix.  x = NULL
+// W (li + L g) + i; gv[g] marks column groups inside the matrix.  x = NULL
 // leaves zeros.
-template <int W, int NG>
-__device__ __forceinline__ void load_x(const float *x, const float *scale, int n_cols, int lane,
+template <int W, int L, int NG>
+__device__ __forceinline__ void load_x(const float *x, const float *scale, int n_cols, int li,
                                        float (*xv)[W], bool *gv) {
 #pragma unroll
     for (int g = 0; g < NG; g++) {
-        const int j0 = W * (lane + 64 * g);
+        const int j0 = W * (li + L * g);
         gv[g] = j0 < n_cols;
 #pragma unroll
         for (int i = 0; i < W; i++) xv[g][i] = 0.f;
@@ -139,25 +175,27 @@ __device__ __forceinline__ void load_x(const float *x, const float *scale, int n
     }
 }
 
-// R consecutive rows starting at `base`, raw (rows >= r1 and groups outside
+// The lane's part of rows row0 + k S (k < R), raw (rows >= r1 and groups outside
 // the matrix read as zero).
-template <typename T, int NG, int R>
-__device__ __forceinline__ void load_rows(const void *A, int n_cols, int64_t base, int64_t r1,
-                                          int lane, const bool *gv,
+template <typename T, int L, int NG, int R>
+__device__ __forceinline__ void load_rows(const void *A, int n_cols, int64_t row0, int64_t r1,
+                                          int li, const bool *gv,
                                           typename Vec<T>::raw (*rv)[NG]) {
     constexpr int W = Vec<T>::W;
+    constexpr int S = kWave / L;
 #pragma unroll
     for (int k = 0; k < R; k++)
 #pragma unroll
         for (int g = 0; g < NG; g++) {
-            const int64_t r = base + k;
+            const int64_t r = row0 + (int64_t)k * S;
             rv[k][g] = typename Vec<T>::raw{};
-            if (r < r1 && gv[g]) rv[k][g] = Vec<T>::load(A, r * n_cols + W * (lane + 64 * g));
+            if (r < r1 && gv[g]) rv[k][g] = Vec<T>::load(A, r * n_cols + W * (li + L * g));
         }
 }
 
-// FP32 row dot product of one raw row, reduced over the wave.
-template <typename T, int NG>
+// FP32 dot product of the lane's part of one raw row with x, reduced over the
+// row's L lanes.
+template <typename T, int L, int NG>
 __device__ __forceinline__ float row_dot(const typename Vec<T>::raw *rv,
                                          const float (*xv)[Vec<T>::W]) {
     constexpr int W = Vec<T>::W;
@@ -169,73 +207,151 @@ __device__ __forceinline__ float row_dot(const typename Vec<T>::raw *rv,
 #pragma unroll
         for (int i = 0; i < W; i++) d = fmaf(a[i], xv[g][i], d);
     }
-    return wave_sum(d);
+    return group_sum<L>(d);
 }
 
-// Streams rows [r0, r1) of A through the calling workgroup: each step the
-// kMatWaves waves take R consecutive rows each; per row the weight is
-//   kColsMlem:     y_r / (A_r x + c_r + 1e-20)   (main.m:162, FP64 ratio)
+// Streams rows [r0, r1) of A through the calling workgroup: each step every wave
+// takes R S consecutive rows (row slot s of the wave holds rows base + k S + s);
+// per row the weight is
+//   kColsMlem:     y_r / (A_r x + c_r + 1e-20)   (main.m:162)
 //   kColsWeighted: w_r
 //   kColsOnes:     1
-// and the row is accumulated into the lane's FP64 column sums acc.
-template <typename T, int NG, int R, int MODE>
+// and the row is accumulated into the lane's FP64 column sums acc.  Every
+// product is added to acc in FP64 (the MLEM ratio formed in FP64), except in the
+// MLEM pass over BF16 and F16 matrices (storage error 2^-9 / 2^-12): there the
+// ratio is formed in FP32 and the step's R products of a column are added in
+// FP32 before one FP64 addition, which halves the vector instructions per element.
+template <typename T, int L, int NG, int R, int MODE>
 __device__ __forceinline__ void stream_rows(const void *A, int n_cols, int64_t r0, int64_t r1,
                                             const float (*xv)[Vec<T>::W], const bool *gv,
                                             const float *y, const float *c, const float *w,
                                             double (*acc)[Vec<T>::W]) {
     constexpr int W = Vec<T>::W;
+    constexpr int S = kWave / L;
+    constexpr bool kWide = __is_same(T, float) || MODE != kColsMlem;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
+    const int slot = lane / L, li = lane % L;
+    for (int64_t base = r0 + (int64_t)wid * R * S; base < r1;
+         base += (int64_t)kMatWaves * R * S) {
         typename Vec<T>::raw rv[R][NG];
-        load_rows<T, NG, R>(A, n_cols, base, r1, lane, gv, rv);
-        double ratio[R];
+        load_rows<T, L, NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
+        if constexpr (kWide) {
+            double ratio[R];
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            const int64_t r = base + k;
-            if (MODE == kColsMlem) {
-                const float d = row_dot<T, NG>(rv[k], xv);
-                double rt = 0.0;
-                if (r < r1) {
-                    const double cr = (double)c[r];
-                    rt = ((double)y[r] - 0.0 * cr) / ((double)d + cr + 1e-20);
+            for (int k = 0; k < R; k++) {
+                const int64_t r = base + (int64_t)k * S + slot;
+                if (MODE == kColsMlem) {
+                    const float d = row_dot<T, L, NG>(rv[k], xv);
+                    double rt = 0.0;
+                    if (r < r1) {
+                        const double cr = (double)c[r];
+                        rt = ((double)y[r] - 0.0 * cr) / ((double)d + cr + 1e-20);
+                    }
+                    ratio[k] = rt;
+                } else if (MODE == kColsWeighted) {
+                    ratio[k] = (r < r1) ? (double)w[r] : 0.0;
+                } else {
+                    ratio[k] = (r < r1) ? 1.0 : 0.0;
                 }
-                ratio[k] = rt;
-            } else if (MODE == kColsWeighted) {
-                ratio[k] = (r < r1) ? (double)w[r] : 0.0;
-            } else {
-                ratio[k] = (r < r1) ? 1.0 : 0.0;
             }
-        }
 #pragma unroll
-        for (int k = 0; k < R; k++)
+            for (int k = 0; k < R; k++)
+#pragma unroll
+                for (int g = 0; g < NG; g++) {
+                    float a[W];
+                    Vec<T>::unpack(rv[k][g], a);
+#pragma unroll
+                    for (int i = 0; i < W; i++) acc[g][i] = fma((double)a[i], ratio[k], acc[g][i]);
+                }
+        } else {
+            float ratio[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int64_t r = base + (int64_t)k * S + slot;
+                if (MODE == kColsMlem) {
+                    const float d = row_dot<T, L, NG>(rv[k], xv);
+                    float rt = 0.f;
+                    if (r < r1) {
+                        const float cr = c[r];
+                        rt = (y[r] - 0.f * cr) / (d + cr + 1e-20f);
+                    }
+                    ratio[k] = rt;
+                } else if (MODE == kColsWeighted) {
+                    ratio[k] = (r < r1) ? w[r] : 0.f;
+                } else {
+                    ratio[k] = (r < r1) ? 1.f : 0.f;
+                }
+            }
 #pragma unroll
             for (int g = 0; g < NG; g++) {
-                float a[W];
-                Vec<T>::unpack(rv[k][g], a);
+                float s[W];
 #pragma unroll
-                for (int i = 0; i < W; i++) acc[g][i] = fma((double)a[i], ratio[k], acc[g][i]);
+                for (int i = 0; i < W; i++) s[i] = 0.f;
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    float a[W];
+                    Vec<T>::unpack(rv[k][g], a);
+#pragma unroll
+                    for (int i = 0; i < W; i++) s[i] = fmaf(a[i], ratio[k], s[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < W; i++) acc[g][i] += (double)s[i];
             }
+        }
     }
 }
 
-// Sums the kMatWaves waves' column sums through LDS (red holds kMatWaves x
-// NG*64*W doubles) and writes the workgroup's n_cols totals to out.
-template <int NG, int W>
+// Sums the column sums of the workgroup's waves and row slots through LDS and
+// writes the n_cols totals to out.  red holds S * SPAN doubles (S = 64 / L): the
+// waves add their slots into it in wave order, then the slots are added in slot
+// order, so the result does not depend on timing.
+template <int L, int NG, int W>
 __device__ __forceinline__ void block_column_sums(const double (*acc)[W], double *red, int n_cols,
                                                   double *out) {
-    constexpr int span = NG * 64 * W;
+    constexpr int span = L * NG * W;
+    constexpr int S = kWave / L;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int slot = lane / L, li = lane % L;
+    for (int q = 0; q < kMatWaves; q++) {
+        if (wid == q) {
 #pragma unroll
-    for (int g = 0; g < NG; g++)
+            for (int g = 0; g < NG; g++)
 #pragma unroll
-        for (int i = 0; i < W; i++) red[wid * span + W * (lane + 64 * g) + i] = acc[g][i];
-    __syncthreads();
+                for (int i = 0; i < W; i++) {
+                    double *p = red + slot * span + W * (li + L * g) + i;
+                    *p = q == 0 ? acc[g][i] : *p + acc[g][i];
+                }
+        }
+        __syncthreads();
+    }
     for (int j = threadIdx.x; j < n_cols; j += kMatThreads) {
         double s = red[j];
 #pragma unroll
-        for (int q = 1; q < kMatWaves; q++) s += red[q * span + j];
+        for (int k = 1; k < S; k++) s += red[k * span + j];
         out[j] = s;
     }
+}
+
+// Column sums of diag(weight) A over rows [r0, r1) for one workgroup: loads x,
+// streams the rows and writes the workgroup's n_cols FP64 totals to out.
+template <typename T, int SPAN, int R, int MODE>
+__device__ __forceinline__ void colsum_block(const void *A, int n_cols, int64_t r0, int64_t r1,
+                                             const float *x, const float *scale, const float *y,
+                                             const float *c, const float *w, double *red,
+                                             double *out) {
+    constexpr int W = Vec<T>::W;
+    typedef RowShape<W, SPAN> Sh;
+    const int li = (threadIdx.x & 63) % Sh::L;
+    float xv[Sh::NG][W];
+    bool gv[Sh::NG];
+    load_x<W, Sh::L, Sh::NG>(MODE == kColsMlem ? x : nullptr, scale, n_cols, li, xv, gv);
+    double acc[Sh::NG][W];
+#pragma unroll
+    for (int g = 0; g < Sh::NG; g++)
+#pragma unroll
+        for (int i = 0; i < W; i++) acc[g][i] = 0.0;
+    stream_rows<T, Sh::L, Sh::NG, R, MODE>(A, n_cols, r0, r1, xv, gv, y, c, w, acc);
+    block_column_sums<Sh::L, Sh::NG, W>(acc, red, n_cols, out);
 }
 
 }  // namespace xrd

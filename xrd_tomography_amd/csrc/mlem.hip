@@ -19,55 +19,47 @@
 
 namespace xrd {
 
-// Workgroup b handles rows [b*rpb, min((b+1)*rpb, n_rows)).  Each step the 4
-// waves take R consecutive rows each.  Lane l holds columns W (l + 64 g) + i.
-template <typename T, int NG, int R, int MODE>
+// Workgroup b handles rows [b*rpb, min((b+1)*rpb, n_rows)) with row shape
+// RowShape<W, SPAN>; each step a wave takes R row slots' worth of rows.
+template <typename T, int SPAN, int R, int MODE>
 __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int64_t n_rows,
                                                              int n_cols, int64_t rpb,
                                                              const float *x, const float *y,
                                                              const float *c, const float *w,
                                                              const float *scale, double *partial) {
-    constexpr int W = Vec<T>::W;
-    __shared__ double red[kMatWaves * NG * 64 * W];
-    const int lane = threadIdx.x & 63;
+    __shared__ double red[RowShape<Vec<T>::W, SPAN>::S * SPAN];
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-
-    float xv[NG][W];
-    bool gv[NG];
-    load_x<W, NG>(MODE == kColsMlem ? x : nullptr, scale, n_cols, lane, xv, gv);
-    double acc[NG][W];
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int i = 0; i < W; i++) acc[g][i] = 0.0;
-    stream_rows<T, NG, R, MODE>(A, n_cols, r0, r1, xv, gv, y, c, w, acc);
-    block_column_sums<NG, W>(acc, red, n_cols, partial + (size_t)blockIdx.x * n_cols);
+    colsum_block<T, SPAN, R, MODE>(A, n_cols, r0, r1, x, scale, y, c, w, red,
+                                   partial + (size_t)blockIdx.x * n_cols);
 }
 
-// out[r] = A_r . x  (FP32, one wave per row group)
-template <typename T, int NG, int R>
+// out[r] = A_r . x  (FP32, L lanes per row)
+template <typename T, int SPAN, int R>
 __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int64_t n_rows,
                                                               int n_cols, int64_t rpb,
                                                               const float *x, const float *scale,
                                                               float *out) {
     constexpr int W = Vec<T>::W;
+    typedef RowShape<W, SPAN> Sh;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int slot = lane / Sh::L, li = lane % Sh::L;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    float xv[NG][W];
-    bool gv[NG];
-    load_x<W, NG>(x, scale, n_cols, lane, xv, gv);
-    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
-        typename Vec<T>::raw rv[R][NG];
-        load_rows<T, NG, R>(A, n_cols, base, r1, lane, gv, rv);
+    float xv[Sh::NG][W];
+    bool gv[Sh::NG];
+    load_x<W, Sh::L, Sh::NG>(x, scale, n_cols, li, xv, gv);
+    for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
+         base += (int64_t)kMatWaves * R * Sh::S) {
+        typename Vec<T>::raw rv[R][Sh::NG];
+        load_rows<T, Sh::L, Sh::NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            const float d = row_dot<T, NG>(rv[k], xv);
-            const int64_t r = base + k;
-            if (lane == 0 && r < r1) out[r] = d;
+            const float d = row_dot<T, Sh::L, Sh::NG>(rv[k], xv);
+            const int64_t r = base + (int64_t)k * Sh::S + slot;
+            if (li == 0 && r < r1) out[r] = d;
         }
     }
 }
@@ -112,41 +104,44 @@ __global__ void ratio_kernel(int64_t n, const float *f, const float *y, const fl
     }
 }
 
-template <typename T, int NG, int R>
+template <typename T, int SPAN, int R>
 __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int64_t n_rows,
                                                              int n_cols, int64_t rpb,
                                                              const float *x, const float *y,
                                                              const float *c, const float *scale,
                                                              double *partial) {
     constexpr int W = Vec<T>::W;
-    __shared__ double red[kMatWaves];
+    typedef RowShape<W, SPAN> Sh;
+    __shared__ double red[kMatThreads];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int slot = lane / Sh::L, li = lane % Sh::L;
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    float xv[NG][W];
-    bool gv[NG];
-    load_x<W, NG>(x, scale, n_cols, lane, xv, gv);
+    float xv[Sh::NG][W];
+    bool gv[Sh::NG];
+    load_x<W, Sh::L, Sh::NG>(x, scale, n_cols, li, xv, gv);
     double ll = 0.0;
-    for (int64_t base = r0 + (int64_t)wid * R; base < r1; base += (int64_t)kMatWaves * R) {
-        typename Vec<T>::raw rv[R][NG];
-        load_rows<T, NG, R>(A, n_cols, base, r1, lane, gv, rv);
+    for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
+         base += (int64_t)kMatWaves * R * Sh::S) {
+        typename Vec<T>::raw rv[R][Sh::NG];
+        load_rows<T, Sh::L, Sh::NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            const int64_t r = base + k;
-            const float d = row_dot<T, NG>(rv[k], xv);
-            if (r < r1) {
+            const int64_t r = base + (int64_t)k * Sh::S + slot;
+            const float d = row_dot<T, Sh::L, Sh::NG>(rv[k], xv);
+            if (li == 0 && r < r1) {
                 const double f = (double)d + (double)c[r] + 1e-20;
                 const double yr = (double)y[r];
                 ll += (yr > 0.0 ? yr * log(f) : 0.0) - f;
             }
         }
     }
-    if (lane == 0) red[wid] = ll;
+    red[threadIdx.x] = ll;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int q = 0; q < kMatWaves; q++) s += red[q];
+        for (int q = 0; q < kMatThreads; q++) s += red[q];
         partial[blockIdx.x] = s;
     }
 }
@@ -253,18 +248,16 @@ int mat_num_blocks(const MatArgs &m) {
 
 static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb - 1) / nb; }
 
-// NG = column groups of 64 W columns covering the row (a power of two <= 8).
-#define XRD_DISPATCH_NG(W, NGV, ...)                               \
-    do {                                                           \
-        const int ngr = (m.n_cols + 64 * (W)-1) / (64 * (W));      \
-        if (ngr <= 1) { constexpr int NGV = 1; __VA_ARGS__; }      \
-        else if (ngr <= 2) { constexpr int NGV = 2; __VA_ARGS__; } \
-        else if (ngr <= 4) { constexpr int NGV = 4; __VA_ARGS__; } \
-        else if (ngr <= 8) {                                       \
-            if constexpr (64 * 8 * (W) <= kMaxMatCols) {           \
-                constexpr int NGV = 8; __VA_ARGS__;                \
-            } else { return hipErrorInvalidValue; }                \
-        } else return hipErrorInvalidValue;                        \
+// Instantiates the row span covering m.n_cols (256, 512, 1024 or 2048 columns).
+#define XRD_DISPATCH_SPAN(SPANV, ...)                                  \
+    do {                                                               \
+        switch (row_span(m.n_cols)) {                                  \
+            case 256: { constexpr int SPANV = 256; __VA_ARGS__; break; }   \
+            case 512: { constexpr int SPANV = 512; __VA_ARGS__; break; }   \
+            case 1024: { constexpr int SPANV = 1024; __VA_ARGS__; break; } \
+            case 2048: { constexpr int SPANV = 2048; __VA_ARGS__; break; } \
+            default: return hipErrorInvalidValue;                      \
+        }                                                              \
     } while (0)
 
 // Returns CALL evaluated with T = the storage tag of the matrix: 2-byte matrices
@@ -284,24 +277,25 @@ static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb 
         return hipErrorInvalidValue;                                                 \
     } while (0)
 
-// Default rows per wave and step: about 4 KB of a row block in flight per wave
-// per row group.
+// Default rows per row slot and step: about 4 loads of 16 bytes per lane in
+// flight per column group count.
 template <int NG> constexpr int default_rows() { return NG >= 8 ? 1 : (NG >= 4 ? 2 : 4); }
 
-template <typename T, int NG, int MODE>
+template <typename T, int SPAN, int MODE>
 static void colsum_launch_r(const MatArgs &m, int nb, int64_t rpb, const float *x,
                             const float *y, const float *c, const float *w, double *partial,
                             hipStream_t s) {
-    // rows per wave and step: the autotuned value for the MLEM pass, else the default
+    constexpr int NG = RowShape<Vec<T>::W, SPAN>::NG;
+    // rows per row slot and step: the autotuned value for the MLEM pass, else the default
     const int r = (MODE == kColsMlem && m.rows_per_wave > 0) ? m.rows_per_wave : default_rows<NG>();
     if (r >= 4 && NG <= 4) {
-        hipLaunchKernelGGL((colsum_kernel<T, NG, 4, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+        hipLaunchKernelGGL((colsum_kernel<T, SPAN, 4, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
                            m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
     } else if (r >= 2) {
-        hipLaunchKernelGGL((colsum_kernel<T, NG, 2, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+        hipLaunchKernelGGL((colsum_kernel<T, SPAN, 2, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
                            m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
     } else {
-        hipLaunchKernelGGL((colsum_kernel<T, NG, 1, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
+        hipLaunchKernelGGL((colsum_kernel<T, SPAN, 1, MODE>), dim3(nb), dim3(kMatThreads), 0, s,
                            m.A, m.n_rows, m.n_cols, rpb, x, y, c, w, m.scale, partial);
     }
 }
@@ -311,8 +305,7 @@ static hipError_t colsum_t(const MatArgs &m, const float *x, const float *y, con
                            const float *w, double *partial, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(Vec<T>::W, NG,
-                    { colsum_launch_r<T, NG, MODE>(m, nb, rpb, x, y, c, w, partial, s); });
+    XRD_DISPATCH_SPAN(SPAN, { colsum_launch_r<T, SPAN, MODE>(m, nb, rpb, x, y, c, w, partial, s); });
     return hipGetLastError();
 }
 
@@ -337,9 +330,9 @@ template <typename T>
 static hipError_t project_t(const MatArgs &m, const float *x, float *out, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(Vec<T>::W, NG, {
-        constexpr int R = default_rows<NG>();
-        hipLaunchKernelGGL((project_kernel<T, NG, R>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
+    XRD_DISPATCH_SPAN(SPAN, {
+        constexpr int R = default_rows<RowShape<Vec<T>::W, SPAN>::NG>();
+        hipLaunchKernelGGL((project_kernel<T, SPAN, R>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
                            m.n_rows, m.n_cols, rpb, x, m.scale, out);
     });
     return hipGetLastError();
@@ -354,8 +347,8 @@ static hipError_t loglik_t(const MatArgs &m, const float *x, const float *y, con
                            double *partial, hipStream_t s) {
     const int nb = mat_num_blocks(m);
     const int64_t rpb = rows_per_block(m, nb);
-    XRD_DISPATCH_NG(Vec<T>::W, NG, {
-        hipLaunchKernelGGL((loglik_kernel<T, NG, 1>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
+    XRD_DISPATCH_SPAN(SPAN, {
+        hipLaunchKernelGGL((loglik_kernel<T, SPAN, 1>), dim3(nb), dim3(kMatThreads), 0, s, m.A,
                            m.n_rows, m.n_cols, rpb, x, y, c, m.scale, partial);
     });
     return hipGetLastError();

@@ -46,7 +46,7 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t n) {
         if (n <= bytes) return hipSuccess;
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&ptr, n);
@@ -54,7 +54,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -185,14 +185,14 @@ int xrd_context_create(int device, xrd_context **out) {
 
 int xrd_context_destroy(xrd_context *c) {
     if (!c) return XRD_OK;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
                       &c->src_int, &c->tables, &c->partial, &c->small, &c->cm_maps, &c->cm_src,
                       &c->cm_vlist, &c->cm_h})
         b->release();
-    if (c->pinned) hipHostFree(c->pinned);
-    if (c->staged) hipEventDestroy(c->staged);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->staged) (void)hipEventDestroy(c->staged);
     delete c;
     return XRD_OK;
 }
@@ -246,7 +246,7 @@ hipError_t stage_to_device(xrd_context *c, const void *host, size_t bytes, void 
         c->staged_pending = false;
     }
     if (bytes > c->pinned_bytes) {
-        if (c->pinned) hipHostFree(c->pinned);
+        if (c->pinned) (void)hipHostFree(c->pinned);
         c->pinned = nullptr;
         c->pinned_bytes = 0;
         hipError_t e = hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault);
@@ -661,7 +661,7 @@ extern "C" int xrd_mlem_plan_iterate(xrd_mlem_plan *p, int32_t n_iter, int use_g
             if (e2 != hipSuccess) e = e2;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
-        hipStreamDestroy(cs);
+        (void)hipStreamDestroy(cs);
         if (e != hipSuccess) return hip_fail(e, "graph capture");
     }
     for (int i = 0; i < n_iter; i++) HIP_TRY(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
@@ -725,18 +725,18 @@ extern "C" int xrd_mlem_plan_autotune(xrd_mlem_plan *p, int32_t reps, void *stre
         }
         if (err != hipSuccess) break;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (err != hipSuccess) return hip_fail(err, fn);
     p->m.blocks_per_cu = best_b;
     p->m.rows_per_wave = best_r;
     p->nblocks = mat_num_blocks(p->m);
     if (p->exec) {  // the captured iteration used the old launch shape
-        hipGraphExecDestroy(p->exec);
+        (void)hipGraphExecDestroy(p->exec);
         p->exec = nullptr;
     }
     if (p->graph) {
-        hipGraphDestroy(p->graph);
+        (void)hipGraphDestroy(p->graph);
         p->graph = nullptr;
     }
     if (out_blocks_per_cu) *out_blocks_per_cu = best_b;
@@ -791,11 +791,11 @@ extern "C" int xrd_mlem_plan_set_launch(xrd_mlem_plan *p, int32_t blocks_per_cu,
     p->m.rows_per_wave = rows_per_wave;
     p->nblocks = mat_num_blocks(p->m);
     if (p->exec) {
-        hipGraphExecDestroy(p->exec);
+        (void)hipGraphExecDestroy(p->exec);
         p->exec = nullptr;
     }
     if (p->graph) {
-        hipGraphDestroy(p->graph);
+        (void)hipGraphDestroy(p->graph);
         p->graph = nullptr;
     }
     return XRD_OK;
@@ -807,9 +807,9 @@ extern "C" const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *p) {
 
 extern "C" int xrd_mlem_plan_destroy(xrd_mlem_plan *p) {
     if (!p) return XRD_OK;
-    hipSetDevice(p->ctx->device);
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
+    (void)hipSetDevice(p->ctx->device);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     p->sens.release();
     p->partial.release();
     p->f.release();
@@ -940,19 +940,21 @@ struct xrd_mlem_batch {
     xrd_context *ctx = nullptr;
     BatchArgs args{};
     int blocks_per_cu = 2;
-    DevBuf probs, tiles, partial, sens;
+    int esize = 4;
+    std::vector<BatchProblem> host_probs;
+    DevBuf probs, segs, wg_seg, partial, sens;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
-int batch_grid(int blocks_per_cu) {
+int device_cus() {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
         n = 256;
-    return n * blocks_per_cu;
+    return n;
 }
 
 void batch_drop_graph(xrd_mlem_batch *b) {
@@ -960,6 +962,123 @@ void batch_drop_graph(xrd_mlem_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+// Row-cost charge of one row's reduction and FP64 ratio, in bytes of A.
+constexpr double kBatchRowCharge = 64.0;
+
+// Cuts the rows of the problems in `members` (laid end to end) into G contiguous
+// chunks of about equal cost, one per workgroup, at multiples of 16 rows inside a
+// problem; appends the segments to segs and the G + 1 workgroup offsets to wg.
+void partition_group(const std::vector<BatchProblem> &probs, const std::vector<int> &members,
+                     int esize, int G, std::vector<BatchSegment> &segs, std::vector<int> &wg) {
+    const int P = (int)members.size();
+    std::vector<double> cost(P);
+    double total = 0.0;
+    for (int q = 0; q < P; q++) {
+        const BatchProblem &pr = probs[members[q]];
+        cost[q] = (double)pr.n_rows * ((double)pr.n_cols * esize + kBatchRowCharge);
+        total += cost[q];
+    }
+    // boundary w = (member, row) where workgroup w starts
+    std::vector<int> bk(G + 1);
+    std::vector<int64_t> br(G + 1);
+    bk[0] = 0;
+    br[0] = 0;
+    bk[G] = P;
+    br[G] = 0;
+    int k = 0;
+    double cum = 0.0;
+    for (int w = 1; w < G; w++) {
+        const double t = total * (double)w / (double)G;
+        while (k < P && cum + cost[k] <= t) cum += cost[k++];
+        if (k == P) {
+            bk[w] = P;
+            br[w] = 0;
+            continue;
+        }
+        const int64_t n_rows = probs[members[k]].n_rows;
+        int64_t r = (int64_t)((t - cum) / (cost[k] / (double)n_rows));
+        r -= r % 16;
+        if (r > n_rows) r = n_rows;
+        if (bk[w - 1] == k && r < br[w - 1]) r = br[w - 1];  // keep boundaries monotone
+        bk[w] = k;
+        br[w] = r;
+    }
+    for (int w = 0; w < G; w++) {
+        wg.push_back((int)segs.size());
+        int k0 = bk[w];
+        int64_t r0 = br[w];
+        const int k1 = bk[w + 1];
+        const int64_t r1 = br[w + 1];
+        while (k0 < k1 || (k0 == k1 && k0 < P && r0 < r1)) {
+            const int64_t end = k0 < k1 ? probs[members[k0]].n_rows : r1;
+            if (end > r0) {
+                BatchSegment sg{};
+                sg.problem = members[k0];
+                sg.r0 = r0;
+                sg.r1 = end;
+                segs.push_back(sg);
+            }
+            if (k0 == k1) break;
+            k0++;
+            r0 = 0;
+        }
+    }
+    wg.push_back((int)segs.size());
+}
+
+// Groups the problems by row span, sizes each group's persistent grid
+// (blocks_per_cu per CU, never more than fit on a CU at once: a workgroup that
+// waits for a free CU would finish its chunk after all others), partitions every
+// group and uploads the segment lists with the problem descriptors.
+hipError_t batch_set_grid(xrd_mlem_batch *b) {
+    std::vector<BatchProblem> &probs = b->host_probs;
+    BatchArgs &a = b->args;
+    std::vector<BatchSegment> segs;
+    std::vector<int> wg;
+    a.n_groups = 0;
+    const int cus = device_cus();
+    for (int span = 256; span <= kMaxMatCols; span *= 2) {
+        std::vector<int> members;
+        for (int k = 0; k < (int)probs.size(); k++)
+            if (probs[k].span == span) members.push_back(k);
+        if (members.empty()) continue;
+        int resident = 0;
+        hipError_t e = batch_resident_blocks(a, span, &resident);
+        if (e != hipSuccess) return e;
+        int bpc = b->blocks_per_cu;
+        if (resident > 0 && bpc > resident) bpc = resident;
+        BatchGroup &gr = a.groups[a.n_groups++];
+        gr.span = span;
+        gr.grid = cus * bpc;
+        gr.wg_seg_offset = (int)wg.size();
+        partition_group(probs, members, b->esize, gr.grid, segs, wg);
+    }
+    for (auto &p : probs) {
+        p.seg_begin = 0;
+        p.seg_end = 0;
+    }
+    for (int i = (int)segs.size() - 1; i >= 0; i--) probs[segs[i].problem].seg_begin = i;
+    for (int i = 0; i < (int)segs.size(); i++) probs[segs[i].problem].seg_end = i + 1;
+    hipError_t e = b->segs.ensure(segs.size() * sizeof(BatchSegment));
+    if (e == hipSuccess) e = b->wg_seg.ensure(wg.size() * sizeof(int));
+    if (e == hipSuccess) e = b->partial.ensure(segs.size() * (size_t)a.stride * sizeof(double));
+    // descriptors are small: synchronous copies from pageable memory
+    if (e == hipSuccess)
+        e = hipMemcpy(b->segs.ptr, segs.data(), segs.size() * sizeof(BatchSegment),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->wg_seg.ptr, wg.data(), wg.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->probs.ptr, probs.data(), probs.size() * sizeof(BatchProblem),
+                      hipMemcpyHostToDevice);
+    a.probs = (const BatchProblem *)b->probs.ptr;
+    a.segs = (const BatchSegment *)b->segs.ptr;
+    a.wg_seg = (const int *)b->wg_seg.ptr;
+    a.n_segs = (int)segs.size();
+    a.partial = (double *)b->partial.ptr;
+    return e;
 }
 
 hipError_t batch_iteration(xrd_mlem_batch *b, hipStream_t s) {
@@ -974,7 +1093,7 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
                                      const void *const *A, const int64_t *n_rows,
                                      const int32_t *n_cols, const float *const *col_scale,
                                      const float *const *y, const float *const *c,
-                                     float *const *x, int32_t rows_per_tile, void *stream,
+                                     float *const *x, int32_t reserved, void *stream,
                                      xrd_mlem_batch **out) {
     const char *fn = "xrd_mlem_batch_create";
     if (!out) return fail(XRD_ERR_NULL_POINTER, "%s: out is NULL", fn);
@@ -983,9 +1102,8 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     if (n_problems <= 0) return fail(XRD_ERR_DIMS, "%s: n_problems must be > 0", fn);
     if (!A || !n_rows || !n_cols || !y || !c || !x)
         return fail(XRD_ERR_NULL_POINTER, "%s: a required host array is NULL", fn);
-    if (rows_per_tile <= 0) rows_per_tile = 8192;
+    if (reserved != 0) return fail(XRD_ERR_INVALID_ARG, "%s: reserved must be 0", fn);
     std::vector<BatchProblem> probs(n_problems);
-    std::vector<BatchTile> tiles;
     int max_cols = 0;
     int vec8 = 1;
     for (int k = 0; k < n_problems; k++) {
@@ -1000,27 +1118,18 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
         p.A = A[k];
         p.n_rows = n_rows[k];
         p.n_cols = n_cols[k];
+        p.span = row_span(n_cols[k]);
         p.y = y[k];
         p.c = c[k];
         p.x = x[k];
         p.scale = col_scale ? col_scale[k] : nullptr;
-        p.sens_offset = 0;
-        p.tile_begin = (int)tiles.size();
-        for (int64_t r = 0; r < n_rows[k]; r += rows_per_tile) {
-            BatchTile t{};
-            t.problem = k;
-            t.r0 = r;
-            t.r1 = r + rows_per_tile < n_rows[k] ? r + rows_per_tile : n_rows[k];
-            tiles.push_back(t);
-        }
-        p.tile_end = (int)tiles.size();
         if (n_cols[k] > max_cols) max_cols = n_cols[k];
     }
-    if (tiles.size() > (size_t)(1 << 30)) return fail(XRD_ERR_DIMS, "%s: too many tiles", fn);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;
     xrd_mlem_batch *b = new xrd_mlem_batch();
     b->ctx = ctx;
+    b->esize = a_dtype == XRD_DTYPE_F32 ? 4 : 2;
     size_t sens_total = 0;
     for (auto &p : probs) {
         p.sens_offset = (int)sens_total;
@@ -1028,30 +1137,17 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     }
     hipError_t e = b->sens.ensure(sens_total * sizeof(double));
     if (e == hipSuccess) e = b->probs.ensure(probs.size() * sizeof(BatchProblem));
-    if (e == hipSuccess) e = b->tiles.ensure(tiles.size() * sizeof(BatchTile));
-    if (e == hipSuccess)
-        e = b->partial.ensure(tiles.size() * (size_t)max_cols * sizeof(double));
     if (e == hipSuccess)
         for (auto &p : probs) p.sens = (const double *)b->sens.ptr + p.sens_offset;
-    // descriptors are small: synchronous copies from pageable memory
-    if (e == hipSuccess)
-        e = hipMemcpy(b->probs.ptr, probs.data(), probs.size() * sizeof(BatchProblem),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->tiles.ptr, tiles.data(), tiles.size() * sizeof(BatchTile),
-                      hipMemcpyHostToDevice);
+    b->host_probs = probs;
     BatchArgs &a = b->args;
-    a.probs = (const BatchProblem *)b->probs.ptr;
-    a.tiles = (const BatchTile *)b->tiles.ptr;
     a.n_problems = n_problems;
-    a.n_tiles = (int)tiles.size();
     a.max_cols = max_cols;
     a.stride = max_cols;
     a.dtype = a_dtype;
-    a.grid = batch_grid(b->blocks_per_cu);
     a.rows_per_wave = 0;
     a.vec8 = vec8;
-    a.partial = (double *)b->partial.ptr;
+    if (e == hipSuccess) e = batch_set_grid(b);
     // sensitivities sum(A_k,1)' once, FP64
     if (e == hipSuccess) e = launch_batch_stream(a, false, s);
     if (e == hipSuccess) e = launch_batch_update(a, (double *)b->sens.ptr, false, s);
@@ -1098,10 +1194,88 @@ extern "C" int xrd_mlem_batch_set_launch(xrd_mlem_batch *b, int32_t blocks_per_c
         rows_per_wave > 4)
         return fail(XRD_ERR_INVALID_ARG, "xrd_mlem_batch_set_launch: blocks_per_cu in [0,%d], "
                     "rows_per_wave in [0,4]", kMaxBlocksPerCu);
-    b->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 2;
-    b->args.grid = batch_grid(b->blocks_per_cu);
-    b->args.rows_per_wave = rows_per_wave;
+    HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+    // descriptors are replaced below: wait for passes that still read them
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     batch_drop_graph(b);
+    b->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 2;
+    b->args.rows_per_wave = rows_per_wave;
+    HIP_TRY(batch_set_grid(b), "batch partition");
+    return XRD_OK;
+}
+
+namespace {
+
+// Mean duration (ms) of reps streaming passes of the batch at its current shape.
+hipError_t batch_time_stream(xrd_mlem_batch *b, int reps, hipStream_t s, float *ms) {
+    hipEvent_t e0, e1;
+    hipError_t err = hipEventCreate(&e0);
+    if (err != hipSuccess) return err;
+    err = hipEventCreate(&e1);
+    if (err != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return err;
+    }
+    // the streaming pass reads x, y, c and writes the partials only: x is untouched
+    err = launch_batch_stream(b->args, true, s);
+    if (err == hipSuccess) err = hipEventRecord(e0, s);
+    for (int i = 0; i < reps && err == hipSuccess; i++) err = launch_batch_stream(b->args, true, s);
+    if (err == hipSuccess) err = hipEventRecord(e1, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    *ms = t / (float)reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return err;
+}
+
+}  // namespace
+
+extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *stream,
+                                       int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
+                                       float *out_ms) {
+    const char *fn = "xrd_mlem_batch_autotune";
+    if (!b) return fail(XRD_ERR_NULL_POINTER, "%s: batch is NULL", fn);
+    if (reps <= 0) return fail(XRD_ERR_INVALID_ARG, "%s: reps must be > 0", fn);
+    HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    batch_drop_graph(b);
+    const int bpcs[] = {1, 2, 3, 4};
+    const int rows[] = {1, 2, 4};
+    float best = 1e30f;
+    int best_b = 2, best_r = 2;
+    for (int r : rows) {
+        b->args.rows_per_wave = r;
+        for (int bpc : bpcs) {
+            b->blocks_per_cu = bpc;
+            HIP_TRY(batch_set_grid(b), "batch partition");
+            float ms = 0.f;
+            HIP_TRY(batch_time_stream(b, reps, s, &ms), "timed pass");
+            if (ms < best) {
+                best = ms;
+                best_b = bpc;
+                best_r = r;
+            }
+        }
+    }
+    b->blocks_per_cu = best_b;
+    b->args.rows_per_wave = best_r;
+    HIP_TRY(batch_set_grid(b), "batch partition");
+    if (out_blocks_per_cu) *out_blocks_per_cu = best_b;
+    if (out_rows_per_wave) *out_rows_per_wave = best_r;
+    if (out_ms) *out_ms = best;
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_batch_time_pass(xrd_mlem_batch *b, int32_t reps, void *stream,
+                                        float *out_ms) {
+    const char *fn = "xrd_mlem_batch_time_pass";
+    if (!b || !out_ms) return fail(XRD_ERR_NULL_POINTER, "%s: NULL argument", fn);
+    if (reps <= 0) return fail(XRD_ERR_INVALID_ARG, "%s: reps must be > 0", fn);
+    HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+    HIP_TRY(batch_time_stream(b, reps, (hipStream_t)stream, out_ms), "timed pass");
     return XRD_OK;
 }
 
@@ -1110,7 +1284,8 @@ extern "C" int xrd_mlem_batch_destroy(xrd_mlem_batch *b) {
     (void)hipSetDevice(b->ctx->device);
     batch_drop_graph(b);
     b->probs.release();
-    b->tiles.release();
+    b->segs.release();
+    b->wg_seg.release();
     b->partial.release();
     b->sens.release();
     delete b;

@@ -131,6 +131,13 @@ struct MatArgs {
 constexpr int kMaxBlocksPerCu = 8;
 // Widest matrix row the streaming kernels hold in registers.
 constexpr int kMaxMatCols = 2048;
+// Row span of the streaming kernels for n_cols columns: the smallest of 256, 512,
+// 1024 and 2048 that is >= n_cols (0 if wider).
+__host__ __device__ constexpr int row_span(int n_cols) {
+    int span = 256;
+    while (span < n_cols) span *= 2;
+    return span <= kMaxMatCols ? span : 0;
+}
 
 // Per-column maxima of |A| (FP32 matrix) and power-of-two scales that map each
 // column's maximum to about half the largest finite value of dtype (BF16/F16).
@@ -169,31 +176,50 @@ struct BatchProblem {
     const void *A;        // [n_rows x n_cols], dtype of the batch
     int64_t n_rows;
     int n_cols;           // multiple of 4
-    int tile_begin, tile_end;  // this problem's tiles in the global list
+    int span;             // row span (256, 512, 1024 or 2048) >= n_cols
+    int seg_begin, seg_end;  // this problem's segments in the global list
     int sens_offset;      // offset of its sensitivity in the batch's sensitivity buffer
+    int pad;
     const float *y, *c;
     float *x;
     const float *scale;   // per-column storage scale or NULL
     const double *sens;   // device pointer to its FP64 sensitivity
 };
 
-struct BatchTile {
+// Rows [r0, r1) of one problem, streamed by one workgroup.
+struct BatchSegment {
     int problem;
     int pad;
     int64_t r0, r1;
 };
 
+// Problems of one row span, streamed by one persistent launch of `grid`
+// workgroups; workgroup b streams segments wg_seg[off + b] .. wg_seg[off + b + 1]-1.
+struct BatchGroup {
+    int span;
+    int grid;
+    int wg_seg_offset;
+};
+
 struct BatchArgs {
     const BatchProblem *probs;
-    const BatchTile *tiles;
-    int n_problems, n_tiles, max_cols, stride, dtype;
-    int grid;             // persistent workgroups
+    const BatchSegment *segs;
+    const int *wg_seg;    // the groups' workgroup-to-segment offsets, concatenated
+    int n_problems, n_segs, max_cols, stride, dtype;
     int rows_per_wave;    // 0 = default
     int vec8;             // 2-byte matrices: every n_cols % 8 == 0 and A 16-byte aligned
-    double *partial;      // [n_tiles x stride]
+    int n_groups;
+    BatchGroup groups[4];  // spans 256, 512, 1024, 2048 that occur, ascending
+    double *partial;      // [n_segs x stride]
 };
+
+// LDS doubles of the batch kernels' block reduction (the widest row shape).
+constexpr int kBatchRedDoubles = 2048;
 
 hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s);
 hipError_t launch_batch_update(const BatchArgs &b, double *sens_out, bool update, hipStream_t s);
+// Workgroups of the batch kernel for row span `span` (launch shape of b) that fit
+// on one CU at once.
+hipError_t batch_resident_blocks(const BatchArgs &b, int span, int *blocks_per_cu);
 
 }  // namespace xrd
