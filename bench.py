@@ -27,6 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (profiles/mlem_fused_fan256_hbm_traffic.md): FETCH_SIZE scaled by 2.0 and
 # WRITE_SIZE by 1.0, the factors that map each counter onto a 2 GiB copy kernel.
 MEASURED_TRAFFIC = {("fan256", "f32", "fused"): 23_076_117_299}
+ESIZE = {"f32": 4, "bf16": 2, "f16": 2, "bf24": 3}  # bytes per stored element of A
 
 
 def log(*a):
@@ -39,7 +40,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="fan256")
-    ap.add_argument("--dtype", default="f32", choices=["f32", "bf16", "f16"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "bf16", "f16", "bf24"],
+                    help="storage of A: FP32, BF16/F16 with per-column scale, or 3-byte BF24")
     ap.add_argument("--unfused", action="store_true", help="plain two-pass A x then A' r")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -114,7 +116,9 @@ def main():
     y = torch.from_numpy(np.random.default_rng(11).poisson(expected).astype(np.float32)).to(dev)
     del N
     col_scale = None
-    if args.dtype != "f32":
+    if args.dtype == "bf24":
+        Am = api.convert_matrix(A, api.BF24)
+    elif args.dtype != "f32":
         dt = torch.bfloat16 if args.dtype == "bf16" else torch.float16
         Am, col_scale = api.convert_matrix_scaled(A, dt)
     else:
@@ -148,7 +152,7 @@ def main():
     # the dominant kernel alone: the streaming pass over A, timed live with HIP events
     ms_pass = plan.time_pass(reps=max(5, args.steps // 5))
     log(f"[bench] streaming pass alone: {ms_pass:.3f} ms")
-    esize = {"f32": 4, "bf16": 2, "f16": 2}[args.dtype]
+    esize = ESIZE[args.dtype]
     bytes_step = nrow * ncol * esize + nrow * 8 + ncol * 12
     passes = 2 if args.unfused else 1
     achieved = bytes_step * passes / (ms_pass * 1e-3) / 1e9
@@ -171,7 +175,12 @@ def main():
         k_err = 5
         # GPU MLEM on the stored matrix (FP32, or BF16/F16 with its column scale) vs
         # the FP64 oracle on the FP32 matrix
-        As_stored = Am.index_select(0, rows_t).contiguous() if col_scale is not None else As
+        if args.dtype == "bf24":
+            As_stored = api.convert_matrix(As, api.BF24)  # elementwise: same values as Am's rows
+        elif col_scale is not None:
+            As_stored = Am.index_select(0, rows_t).contiguous()
+        else:
+            As_stored = As
         xg = api.mlem(As_stored, ys, cs, k_err, col_scale=col_scale)
         del As_stored
         A64 = As.double().cpu().numpy()
@@ -279,11 +288,14 @@ def run_batch(args, dev):
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     n_it = 30
-    esize = {"f32": 4, "bf16": 2, "f16": 2}[args.dtype]
+    esize = ESIZE[args.dtype]
     stored = []
     for A, y, c in data:
         if args.dtype == "f32":
             stored.append((A, y, c, torch.full((A.shape[1],), 10.0, device=dev)))
+        elif args.dtype == "bf24":
+            stored.append((api.convert_matrix(A, api.BF24), y, c,
+                           torch.full((A.shape[1],), 10.0, device=dev)))
         else:
             dt = torch.bfloat16 if args.dtype == "bf16" else torch.float16
             A16, sc = api.convert_matrix_scaled(A, dt)

@@ -216,6 +216,12 @@ int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts, const floa
 #define XRD_DTYPE_F32  0
 #define XRD_DTYPE_BF16 1
 #define XRD_DTYPE_F16  2
+/* BF24: FP32 rounded to nearest even at 24 bits (sign, 8-bit exponent, 15-bit
+ * mantissa; relative error <= 2^-16), stored as two planes in one buffer of
+ * 3 n_rows n_cols bytes: n_rows*n_cols uint16 holding bits 31..16, then
+ * n_rows*n_cols uint8 holding bits 15..8.  Needs n_cols % 8 == 0 and a 16-byte
+ * aligned buffer.  No reference counterpart (the reference keeps A in FP32). */
+#define XRD_DTYPE_BF24 3
 
 /* out[r] = sum_j A[r,j] x[j]            (the A*Irecon of main.m:162).  */
 int xrd_project(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows, int32_t n_cols,
@@ -227,7 +233,7 @@ int xrd_backproject(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows
 int xrd_sensitivity(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
                     int32_t n_cols, double *out, void *stream);
 
-/* Convert an FP32 matrix to BF16/F16 storage (round to nearest even). */
+/* Convert n FP32 values to BF16/F16/BF24 storage (round to nearest even). */
 int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, int out_dtype, void *out,
                        void *stream);
 
@@ -302,12 +308,19 @@ int xrd_mlem_batch_iterate(xrd_mlem_batch *batch, int32_t n_iter, int use_graph,
  * Waits for the device, then re-partitions the rows. */
 int xrd_mlem_batch_set_launch(xrd_mlem_batch *batch, int32_t blocks_per_cu,
                               int32_t rows_per_wave);
-/* Times the streaming pass (reps launches, x untouched) over blocks_per_cu in
- * {1,2,3,4} (those that fit on a CU) and rows_per_wave in {1,2,4}, keeps the
- * fastest and returns it with its mean pass time in ms (outputs may be NULL). */
+/* For every row-span group (problems are launched per column span 256, 512, 1024
+ * or 2048), times that group's streaming launch (reps launches, x untouched) over
+ * blocks_per_cu in {1,2,3,4} (clamped to what fits on a CU) and rows_per_wave in
+ * {1,2,4} and keeps the fastest.  Returns the shape of the group with the largest
+ * matrices and the mean time of a whole pass in ms (outputs may be NULL). */
 int xrd_mlem_batch_autotune(xrd_mlem_batch *batch, int32_t reps, void *stream,
                             int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
                             float *out_ms);
+/* The row-span groups of the batch: writes up to max_groups entries of span,
+ * persistent grid (workgroups) and rows_per_wave; returns the number of groups
+ * (negative on error). */
+int xrd_mlem_batch_groups(const xrd_mlem_batch *batch, int32_t max_groups, int32_t *out_span,
+                          int32_t *out_grid, int32_t *out_rows_per_wave);
 /* Mean duration in ms of reps streaming passes at the current launch shape,
  * measured with HIP events on stream (x untouched). */
 int xrd_mlem_batch_time_pass(xrd_mlem_batch *batch, int32_t reps, void *stream, float *out_ms);

@@ -142,6 +142,64 @@ def test_reduced_precision_matrix(xrd, dtype, cols, offset):
     assert torch.allclose(x1, x2, rtol=1e-5, atol=1e-6 * x2.abs().max().item())
 
 
+def test_bf24_conversion_rounds_to_16_significant_bits(xrd):
+    """BF24 storage keeps 16 significant bits, rounded to nearest (ties to even): the
+    decoded matrix equals an FP64 rounding of each value to 16 significant bits."""
+    import torch
+    rng = np.random.default_rng(30)
+    A = (rng.standard_normal((300, 64)) * np.exp(rng.uniform(-30, 30, (300, 64)))).astype(np.float32)
+    A[0, :8] = [0.0, -0.0, 1.0, 1.0 + 2.0 ** -16, 1.0 + 3 * 2.0 ** -16, 3.0e38, 1e-40, -2.5]
+    Ab = xrd.convert_matrix(torch.from_numpy(A).cuda(), xrd.BF24)
+    got = Ab.float().cpu().numpy().astype(np.float64)
+    a = A.astype(np.float64)
+    e = np.floor(np.log2(np.abs(np.where(a == 0, 1.0, a))))
+    e = np.maximum(e, -126.0)  # subnormals keep the spacing of the smallest binade
+    q = 2.0 ** (e - 15)
+    want = np.round(a / q) * q  # numpy rounds half to even
+    assert np.array_equal(got, want)
+    assert got[0, 3] == 1.0 and got[0, 4] == 1.0 + 4 * 2.0 ** -16  # ties to even
+
+
+@pytest.mark.parametrize("cols", [512, 1024, 2048])
+def test_bf24_operators_and_mlem(xrd, oracle, cols):
+    """BF24 matrices: A x, A' w, sum(A) and MLEM equal the FP32 operators on the decoded
+    matrix, and MLEM is within 2e-5 of the FP64 oracle on the decoded matrix."""
+    import torch
+    A, y, c = _problem(9, rows=6000, cols=cols)
+    Ab = xrd.convert_matrix(torch.from_numpy(A).cuda(), xrd.BF24)
+    Af = Ab.float().contiguous()
+    x = torch.rand(cols, device="cuda")
+    w = torch.rand(A.shape[0], device="cuda") - 0.5
+    assert torch.allclose(xrd.project(Ab, x), xrd.project(Af, x), rtol=1e-6, atol=0)
+    assert torch.allclose(xrd.sensitivity(Ab), xrd.sensitivity(Af), rtol=1e-12, atol=0)
+    b1, b2 = xrd.backproject(Ab, w), xrd.backproject(Af, w)
+    assert (b1 - b2).abs().max().item() <= 1e-12 * b2.abs().max().item()
+    yd, cd = torch.from_numpy(y).cuda(), torch.from_numpy(c).cuda()
+    xb = xrd.mlem(Ab, yd, cd, 10).cpu().numpy()
+    xo = oracle.mlem(Af.double().cpu().numpy(), y, c, np.full(cols, 10.0), 10)
+    live = xo > 1e-6 * xo.max()
+    rel = np.max(np.abs(xb[live] - xo[live]) / xo[live])
+    xo32 = oracle.mlem(A.astype(np.float64), y, c, np.full(cols, 10.0), 10)
+    storage = np.max(np.abs(xb[live] - xo32[live]) / xo32[live])
+    print(f"\n[bf24 cols={cols}] vs FP64 on stored {rel:.2e}; vs FP64 on the FP32 matrix {storage:.2e}")
+    assert rel <= 2e-5 and storage <= 1e-4
+
+
+def test_bf24_rejects_unsupported_layouts(xrd):
+    import torch
+    from xrd_tomography_amd._lib import XrdError
+    A = torch.rand(64, 516, device="cuda")
+    Ab = xrd.convert_matrix(A, xrd.BF24)  # conversion itself takes any shape
+    with pytest.raises(XrdError):
+        xrd.sensitivity(Ab)  # n_cols % 8 != 0
+    A = torch.rand(64, 512, device="cuda")
+    Ab = xrd.convert_matrix(A, xrd.BF24)
+    off = xrd.Bf24Matrix(torch.empty(Ab.data.numel() + 8, dtype=torch.uint8,
+                                     device="cuda")[8:], 64, 512)
+    with pytest.raises(XrdError):
+        xrd.sensitivity(off)  # 8-byte aligned only
+
+
 @pytest.mark.parametrize("dtype", ["float16", "bfloat16"])
 def test_scaled_reduced_precision_mlem(xrd, oracle, dtype):
     """Per-column power-of-two scaled F16/BF16 storage: MLEM on the stored matrix equals
@@ -169,7 +227,7 @@ def test_scaled_reduced_precision_mlem(xrd, oracle, dtype):
 
 
 @pytest.mark.parametrize("dtype,wide", [("float32", 1024), ("float16", 1024),
-                                        ("float16", 1020), ("bfloat16", 1024)])
+                                        ("float16", 1020), ("bfloat16", 1024), ("bf24", 1024)])
 def test_batched_mlem_matches_single(xrd, oracle, dtype, wide):
     """Problems of uneven rows and columns in one persistent launch: each x equals the
     single-problem plan's x (1e-6) and the FP64 oracle's (2e-5 on the stored matrix).
@@ -181,13 +239,16 @@ def test_batched_mlem_matches_single(xrd, oracle, dtype, wide):
         A, y, c = _problem(20 + i, rows=r, cols=cc)
         Ad = torch.from_numpy(A).cuda()
         sc = None
-        if dtype != "float32":
+        if dtype == "bf24":
+            Ad = xrd.convert_matrix(Ad, xrd.BF24)
+        elif dtype != "float32":
             Ad, sc = xrd.convert_matrix_scaled(Ad, getattr(torch, dtype))
         yd, cd = torch.from_numpy(y).cuda(), torch.from_numpy(c).cuda()
         x = torch.full((cc,), 10.0, device="cuda")
         probs.append((Ad, yd, cd, x, sc) if sc is not None else (Ad, yd, cd, x))
         singles.append(xrd.mlem(Ad, yd, cd, 8, col_scale=sc))
-        Aeff = Ad.double().cpu().numpy() / (sc.double().cpu().numpy()[None, :] if sc is not None else 1.0)
+        Aeff = Ad.float().double().cpu().numpy() / (sc.double().cpu().numpy()[None, :]
+                                                    if sc is not None else 1.0)
         refs.append(oracle.mlem(Aeff, y, c, np.full(cc, 10.0), 8))
     batch = xrd.MlemBatch(probs)
     batch.set_launch(1, 1)  # 256 workgroups: several problems share a workgroup's chunk

@@ -19,7 +19,7 @@ ERRORS = {
     -4: "XRD_ERR_ALIGN", -5: "XRD_ERR_HIP", -6: "XRD_ERR_ALLOC", -7: "XRD_ERR_UNSUPPORTED",
     -8: "XRD_ERR_NONFINITE",
 }
-DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
+DTYPE_F32, DTYPE_BF16, DTYPE_F16, DTYPE_BF24 = 0, 1, 2, 3
 MLEM_FUSED, MLEM_UNFUSED = 0, 1
 
 # Exported symbols (kept in sync with include/xrd_tomo.h; a test checks both).
@@ -33,7 +33,7 @@ EXPORTS = (
     "xrd_convert_matrix_scaled", "xrd_mlem_plan_autotune", "xrd_mlem_plan_set_launch",
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
     "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
-    "xrd_mlem_batch_time_pass",
+    "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups",
 )
 
 
@@ -89,6 +89,8 @@ def _declare(L):
                                                    ctypes.POINTER(ctypes.c_float)]),
         "xrd_mlem_batch_time_pass": (ctypes.c_int, [_vp, _i32, _vp,
                                                     ctypes.POINTER(ctypes.c_float)]),
+        "xrd_mlem_batch_groups": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32),
+                                                 ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),

@@ -21,10 +21,43 @@ import numpy as np
 import torch  # loaded before libxrd_tomo.so so both share one HIP runtime
 
 from . import _lib
-from ._lib import DTYPE_BF16, DTYPE_F16, DTYPE_F32, MLEM_FUSED, MLEM_UNFUSED, check, lib
+from ._lib import (DTYPE_BF16, DTYPE_BF24, DTYPE_F16, DTYPE_F32, MLEM_FUSED, MLEM_UNFUSED,
+                   check, lib)
 from .descriptors import Discretization, Geometry
 
 _DTYPES = {torch.float32: DTYPE_F32, torch.bfloat16: DTYPE_BF16, torch.float16: DTYPE_F16}
+
+#: dtype argument of ``convert_matrix`` selecting the 3-byte BF24 storage.
+BF24 = "bf24"
+
+
+class Bf24Matrix:
+    """A [rows x cols] matrix in BF24 storage (XRD_DTYPE_BF24): FP32 rounded to nearest
+    even at 24 bits, kept as one uint8 buffer of 3*rows*cols bytes -- the uint16 plane
+    of bits 31..16, then the uint8 plane of bits 15..8.  Accepted wherever the matrix
+    operators, MlemPlan and MlemBatch take A."""
+
+    dtype = BF24
+
+    def __init__(self, data: torch.Tensor, n_rows: int, n_cols: int):
+        if data.dtype != torch.uint8 or data.dim() != 1 or data.numel() != 3 * n_rows * n_cols:
+            raise ValueError("BF24 storage is a flat uint8 tensor of 3*rows*cols bytes")
+        self.data = data
+        self.shape = (int(n_rows), int(n_cols))
+        self.device = data.device
+
+    def data_ptr(self) -> int:
+        return self.data.data_ptr()
+
+    def numel(self) -> int:
+        return self.shape[0] * self.shape[1]
+
+    def float(self) -> torch.Tensor:
+        """The stored values as an FP32 tensor (exact)."""
+        n = self.numel()
+        hi = self.data[:2 * n].view(torch.int16).to(torch.int32) & 0xFFFF
+        lo = self.data[2 * n:].to(torch.int32)
+        return ((hi << 16) | (lo << 8)).view(torch.float32).view(self.shape)
 
 
 def _ptr(t: Optional[torch.Tensor]):
@@ -156,7 +189,10 @@ def trace_rays(segments: torch.Tensor, nx: int, ny: int, step: float = 0.25,
     return side, entry, count, tex, ints
 
 
-def _mat(A: torch.Tensor):
+def _mat(A):
+    if isinstance(A, Bf24Matrix):
+        _require_gpu(A.data)
+        return DTYPE_BF24, A.shape[0], A.shape[1]
     _require_gpu(A)
     if A.dim() != 2:
         raise ValueError("A must be 2-D [rows, cols]")
@@ -195,10 +231,18 @@ def sensitivity(A: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
     return out
 
 
-def convert_matrix(A: torch.Tensor, dtype: torch.dtype, ctx=None, stream=None) -> torch.Tensor:
-    """FP32 -> BF16/F16 storage (round to nearest even) with the library's kernel."""
+def convert_matrix(A: torch.Tensor, dtype, ctx=None, stream=None):
+    """FP32 -> BF16/F16 (torch tensor) or BF24 (``dtype=BF24``: a Bf24Matrix) storage,
+    round to nearest even, with the library's kernel."""
     ctx = ctx or default_context(A.device.index or 0)
     _require_gpu(A)
+    if dtype == BF24:
+        if A.dim() != 2:
+            raise ValueError("A must be 2-D [rows, cols]")
+        out = torch.empty(3 * A.numel(), dtype=torch.uint8, device=A.device)
+        check(lib().xrd_convert_matrix(ctx.handle, _ptr(A), A.numel(), DTYPE_BF24,
+                                       _ptr(out), _stream(stream)), "xrd_convert_matrix")
+        return Bf24Matrix(out, A.shape[0], A.shape[1])
     out = torch.empty(A.shape, dtype=dtype, device=A.device)
     check(lib().xrd_convert_matrix(ctx.handle, _ptr(A), A.numel(), _DTYPES[dtype], _ptr(out),
                                    _stream(stream)), "xrd_convert_matrix")
@@ -418,7 +462,9 @@ class MlemBatch:
             rows[k], cols[k] = nr, nc
         self._keep = problems
         h = ctypes.c_void_p()
-        check(lib().xrd_mlem_batch_create(self.ctx.handle, n, _DTYPES[dts.pop()], A, rows, cols,
+        dt = dts.pop()
+        check(lib().xrd_mlem_batch_create(self.ctx.handle, n,
+                                          DTYPE_BF24 if dt == BF24 else _DTYPES[dt], A, rows, cols,
                                           sc, y, c, x, 0, _stream(stream),
                                           ctypes.byref(h)), "xrd_mlem_batch_create")
         self.handle = h
@@ -432,12 +478,21 @@ class MlemBatch:
               "xrd_mlem_batch_set_launch")
 
     def autotune(self, reps: int = 2, stream=None) -> dict:
-        """Pick the fastest launch shape of the streaming pass (x is not modified)."""
-        b, r, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
-        check(lib().xrd_mlem_batch_autotune(self.handle, int(reps), _stream(stream),
-                                            ctypes.byref(b), ctypes.byref(r), ctypes.byref(ms)),
-              "xrd_mlem_batch_autotune")
-        return {"blocks_per_cu": b.value, "rows_per_wave": r.value, "ms_pass": ms.value}
+        """Pick the fastest launch shape of each row-span group's streaming launch (x is
+        not modified); returns the chosen groups and the pass time."""
+        ms = ctypes.c_float()
+        check(lib().xrd_mlem_batch_autotune(self.handle, int(reps), _stream(stream), None, None,
+                                            ctypes.byref(ms)), "xrd_mlem_batch_autotune")
+        return {"groups": self.groups(), "ms_pass": ms.value}
+
+    def groups(self) -> list:
+        """[{span, grid, rows_per_wave}] of the row-span groups, one launch each."""
+        n = 4
+        sp, gr, rw = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
+        k = lib().xrd_mlem_batch_groups(self.handle, n, sp, gr, rw)
+        if k < 0:
+            check(k, "xrd_mlem_batch_groups")
+        return [{"span": sp[i], "grid": gr[i], "rows_per_wave": rw[i]} for i in range(min(k, n))]
 
     def time_pass(self, reps: int = 10, stream=None) -> float:
         """Mean ms of one streaming pass over all problems, HIP events on the stream."""

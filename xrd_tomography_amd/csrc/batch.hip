@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kMatThreads) void batch_stream_kernel(const BatchPr
         const BatchSegment sg = segs[si];
         const BatchProblem pb = probs[sg.problem];
         colsum_block<T, SPAN, batch_rows<T, SPAN, R>(), MODE>(
-            pb.A, pb.n_cols, sg.r0, sg.r1, MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.y,
+            pb.A, pb.n_rows, pb.n_cols, sg.r0, sg.r1, MODE == kColsMlem ? pb.x : nullptr, pb.scale, pb.y,
             pb.c, nullptr, red, partial + (size_t)si * stride);
         __syncthreads();
     }
@@ -99,8 +99,7 @@ static const void *batch_kernel_mode(bool mlem, int span, int R) {
     return mlem ? batch_kernel_span<T, kColsMlem>(span, R) : batch_kernel_span<T, kColsOnes>(span, R);
 }
 
-static const void *batch_kernel(const BatchArgs &b, int span, bool mlem) {
-    const int R = b.rows_per_wave > 0 ? b.rows_per_wave : 2;
+static const void *batch_kernel(const BatchArgs &b, int span, int R, bool mlem) {
     switch (b.dtype) {
         case XRD_DTYPE_F32: return batch_kernel_mode<float>(mlem, span, R);
         case XRD_DTYPE_BF16:
@@ -109,20 +108,22 @@ static const void *batch_kernel(const BatchArgs &b, int span, bool mlem) {
         case XRD_DTYPE_F16:
             return b.vec8 ? batch_kernel_mode<f16x8>(mlem, span, R)
                           : batch_kernel_mode<f16x4>(mlem, span, R);
+        case XRD_DTYPE_BF24: return batch_kernel_mode<bf24x8>(mlem, span, R);
     }
     return nullptr;
 }
 
-hipError_t batch_resident_blocks(const BatchArgs &b, int span, int *blocks_per_cu) {
-    const void *k = batch_kernel(b, span, true);
+hipError_t batch_resident_blocks(const BatchArgs &b, int span, int R, int *blocks_per_cu) {
+    const void *k = batch_kernel(b, span, R, true);
     if (!k) return hipErrorInvalidValue;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, kMatThreads, 0);
 }
 
-hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s) {
+hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s, int group) {
     for (int g = 0; g < b.n_groups; g++) {
+        if (group >= 0 && g != group) continue;
         const BatchGroup &gr = b.groups[g];
-        const void *k = batch_kernel(b, gr.span, mlem);
+        const void *k = batch_kernel(b, gr.span, gr.rows_per_wave, mlem);
         if (!k) return hipErrorInvalidValue;
         const BatchProblem *probs = b.probs;
         const BatchSegment *segs = b.segs;

@@ -32,6 +32,7 @@ struct bf16x4 {};
 struct f16x4 {};
 struct bf16x8 {};
 struct f16x8 {};
+struct bf24x8 {};  // 16-byte load from the uint16 plane + 8-byte load from the uint8 plane
 
 __device__ __forceinline__ void unpack_bf16(unsigned int u, float &lo, float &hi) {
     lo = __uint_as_float(u << 16);
@@ -44,15 +45,15 @@ __device__ __forceinline__ void unpack_f16(unsigned int u, float &lo, float &hi)
     hi = f.y;
 }
 
-// Vec<T>::load(A, e) loads the W elements starting at element offset e as a raw
-// register vector; Vec<T>::unpack(raw, a) widens it to W floats.  Rows are kept
-// raw while their dot products are reduced, which halves the registers a 2-byte
-// row group holds in flight.
+// Vec<T>::load(A, e, n) loads the W elements starting at element offset e of
+// the n-element matrix A as a raw register vector; Vec<T>::unpack(raw, a) widens
+// it to W floats.  Rows are kept raw while their dot products are reduced, which
+// halves the registers a 2-byte row group holds in flight.
 template <typename T> struct Vec;
 template <> struct Vec<float> {
     static constexpr int W = 4;
     typedef f32x4v raw;
-    static __device__ __forceinline__ raw load(const void *A, int64_t e) {
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
             reinterpret_cast<const f32x4v *>(reinterpret_cast<const float *>(A) + e));
     }
@@ -63,7 +64,7 @@ template <> struct Vec<float> {
 template <> struct Vec<bf16x4> {
     static constexpr int W = 4;
     typedef u32x2v raw;
-    static __device__ __forceinline__ raw load(const void *A, int64_t e) {
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
             reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e));
     }
@@ -75,7 +76,7 @@ template <> struct Vec<bf16x4> {
 template <> struct Vec<f16x4> {
     static constexpr int W = 4;
     typedef u32x2v raw;
-    static __device__ __forceinline__ raw load(const void *A, int64_t e) {
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
             reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e));
     }
@@ -87,7 +88,7 @@ template <> struct Vec<f16x4> {
 template <> struct Vec<bf16x8> {
     static constexpr int W = 8;
     typedef u32x4v raw;
-    static __device__ __forceinline__ raw load(const void *A, int64_t e) {
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
             reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e));
     }
@@ -101,7 +102,7 @@ template <> struct Vec<bf16x8> {
 template <> struct Vec<f16x8> {
     static constexpr int W = 8;
     typedef u32x4v raw;
-    static __device__ __forceinline__ raw load(const void *A, int64_t e) {
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
             reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e));
     }
@@ -110,6 +111,33 @@ template <> struct Vec<f16x8> {
         unpack_f16(u.y, a[2], a[3]);
         unpack_f16(u.z, a[4], a[5]);
         unpack_f16(u.w, a[6], a[7]);
+    }
+};
+
+// BF24: element i of a pair is (h16_i << 16) | (l8_i << 8).
+struct bf24raw {
+    u32x4v h;
+    u32x2v l;
+};
+template <> struct Vec<bf24x8> {
+    static constexpr int W = 8;
+    typedef bf24raw raw;
+    static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t n) {
+        const uint8_t *b = reinterpret_cast<const uint8_t *>(A);
+        raw v;
+        v.h = __builtin_nontemporal_load(reinterpret_cast<const u32x4v *>(b + 2 * e));
+        v.l = __builtin_nontemporal_load(reinterpret_cast<const u32x2v *>(b + 2 * n + e));
+        return v;
+    }
+    static __device__ __forceinline__ void unpack(const raw &v, float *a) {
+        const unsigned h[4] = {v.h.x, v.h.y, v.h.z, v.h.w};
+        const unsigned l[2] = {v.l.x, v.l.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const unsigned lb = l[q >> 1] >> (16 * (q & 1));  // bytes of elements 2q, 2q+1
+            a[2 * q] = __uint_as_float((h[q] << 16) | ((lb << 8) & 0xff00u));
+            a[2 * q + 1] = __uint_as_float((h[q] & 0xffff0000u) | (lb & 0xff00u));
+        }
     }
 };
 
@@ -178,8 +206,8 @@ __device__ __forceinline__ void load_x(const float *x, const float *scale, int n
 // The lane's part of rows row0 + k S (k < R), raw (rows >= r1 and groups outside
 // the matrix read as zero).
 template <typename T, int L, int NG, int R>
-__device__ __forceinline__ void load_rows(const void *A, int n_cols, int64_t row0, int64_t r1,
-                                          int li, const bool *gv,
+__device__ __forceinline__ void load_rows(const void *A, int64_t n_elem, int n_cols, int64_t row0,
+                                          int64_t r1, int li, const bool *gv,
                                           typename Vec<T>::raw (*rv)[NG]) {
     constexpr int W = Vec<T>::W;
     constexpr int S = kWave / L;
@@ -189,7 +217,8 @@ __device__ __forceinline__ void load_rows(const void *A, int n_cols, int64_t row
         for (int g = 0; g < NG; g++) {
             const int64_t r = row0 + (int64_t)k * S;
             rv[k][g] = typename Vec<T>::raw{};
-            if (r < r1 && gv[g]) rv[k][g] = Vec<T>::load(A, r * n_cols + W * (li + L * g));
+            if (r < r1 && gv[g])
+                rv[k][g] = Vec<T>::load(A, r * n_cols + W * (li + L * g), n_elem);
         }
 }
 
@@ -218,11 +247,13 @@ __device__ __forceinline__ float row_dot(const typename Vec<T>::raw *rv,
 //   kColsOnes:     1
 // and the row is accumulated into the lane's FP64 column sums acc.  Every
 // product is added to acc in FP64 (the MLEM ratio formed in FP64), except in the
-// MLEM pass over BF16 and F16 matrices (storage error 2^-9 / 2^-12): there the
-// ratio is formed in FP32 and the step's R products of a column are added in
-// FP32 before one FP64 addition, which halves the vector instructions per element.
+// MLEM pass over BF16, F16 and BF24 matrices (storage error 2^-9, 2^-12, 2^-16):
+// there the ratio is formed in FP32 and the step's R products of a column are
+// added in FP32 before one FP64 addition, which halves the vector instructions
+// per element.
 template <typename T, int L, int NG, int R, int MODE>
-__device__ __forceinline__ void stream_rows(const void *A, int n_cols, int64_t r0, int64_t r1,
+__device__ __forceinline__ void stream_rows(const void *A, int64_t n_elem, int n_cols, int64_t r0,
+                                            int64_t r1,
                                             const float (*xv)[Vec<T>::W], const bool *gv,
                                             const float *y, const float *c, const float *w,
                                             double (*acc)[Vec<T>::W]) {
@@ -234,7 +265,7 @@ __device__ __forceinline__ void stream_rows(const void *A, int n_cols, int64_t r
     for (int64_t base = r0 + (int64_t)wid * R * S; base < r1;
          base += (int64_t)kMatWaves * R * S) {
         typename Vec<T>::raw rv[R][NG];
-        load_rows<T, L, NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
+        load_rows<T, L, NG, R>(A, n_elem, n_cols, base + slot, r1, li, gv, rv);
         if constexpr (kWide) {
             double ratio[R];
 #pragma unroll
@@ -335,7 +366,8 @@ __device__ __forceinline__ void block_column_sums(const double (*acc)[W], double
 // Column sums of diag(weight) A over rows [r0, r1) for one workgroup: loads x,
 // streams the rows and writes the workgroup's n_cols FP64 totals to out.
 template <typename T, int SPAN, int R, int MODE>
-__device__ __forceinline__ void colsum_block(const void *A, int n_cols, int64_t r0, int64_t r1,
+__device__ __forceinline__ void colsum_block(const void *A, int64_t n_rows, int n_cols, int64_t r0,
+                                             int64_t r1,
                                              const float *x, const float *scale, const float *y,
                                              const float *c, const float *w, double *red,
                                              double *out) {
@@ -350,7 +382,8 @@ __device__ __forceinline__ void colsum_block(const void *A, int n_cols, int64_t 
     for (int g = 0; g < Sh::NG; g++)
 #pragma unroll
         for (int i = 0; i < W; i++) acc[g][i] = 0.0;
-    stream_rows<T, Sh::L, Sh::NG, R, MODE>(A, n_cols, r0, r1, xv, gv, y, c, w, acc);
+    stream_rows<T, Sh::L, Sh::NG, R, MODE>(A, n_rows * n_cols, n_cols, r0, r1, xv, gv, y, c, w,
+                                           acc);
     block_column_sums<Sh::L, Sh::NG, W>(acc, red, n_cols, out);
 }
 

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kMatThreads) void colsum_kernel(const void *A, int6
     const int64_t r0 = (int64_t)blockIdx.x * rpb;
     int64_t r1 = r0 + rpb;
     if (r1 > n_rows) r1 = n_rows;
-    colsum_block<T, SPAN, R, MODE>(A, n_cols, r0, r1, x, scale, y, c, w, red,
+    colsum_block<T, SPAN, R, MODE>(A, n_rows, n_cols, r0, r1, x, scale, y, c, w, red,
                                    partial + (size_t)blockIdx.x * n_cols);
 }
 
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int
     for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
          base += (int64_t)kMatWaves * R * Sh::S) {
         typename Vec<T>::raw rv[R][Sh::NG];
-        load_rows<T, Sh::L, Sh::NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
+        load_rows<T, Sh::L, Sh::NG, R>(A, n_rows * n_cols, n_cols, base + slot, r1, li, gv, rv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const float d = row_dot<T, Sh::L, Sh::NG>(rv[k], xv);
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int6
     for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
          base += (int64_t)kMatWaves * R * Sh::S) {
         typename Vec<T>::raw rv[R][Sh::NG];
-        load_rows<T, Sh::L, Sh::NG, R>(A, n_cols, base + slot, r1, li, gv, rv);
+        load_rows<T, Sh::L, Sh::NG, R>(A, n_rows * n_cols, n_cols, base + slot, r1, li, gv, rv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const int64_t r = base + (int64_t)k * Sh::S + slot;
@@ -146,24 +146,39 @@ __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int6
     }
 }
 
+// Stores v as element i of an n-element BF16/F16/BF24 matrix (round to nearest
+// even; NaN stays a quiet NaN).
+__device__ __forceinline__ void store_element(float v, int64_t i, int64_t n, int dtype, void *out) {
+    uint32_t u = __float_as_uint(v);
+    const bool nan = (u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu);
+    if (dtype == XRD_DTYPE_BF16) {
+        uint16_t b;
+        if (nan) {
+            b = (uint16_t)((u >> 16) | 0x40u);
+        } else {
+            u += 0x7fffu + ((u >> 16) & 1u);
+            b = (uint16_t)(u >> 16);
+        }
+        reinterpret_cast<uint16_t *>(out)[i] = b;
+    } else if (dtype == XRD_DTYPE_BF24) {
+        uint32_t t;  // the top 24 bits
+        if (nan) {
+            t = (u >> 8) | 0x4000u;
+        } else {
+            u += 0x7fu + ((u >> 8) & 1u);
+            t = u >> 8;
+        }
+        reinterpret_cast<uint16_t *>(out)[i] = (uint16_t)(t >> 8);
+        reinterpret_cast<uint8_t *>(out)[2 * n + i] = (uint8_t)(t & 0xffu);
+    } else {
+        reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
+    }
+}
+
 __global__ void convert_kernel(const float *A, int64_t n, int dtype, void *out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const float v = A[i];
-        if (dtype == XRD_DTYPE_BF16) {
-            uint32_t u = __float_as_uint(v);
-            uint16_t b;
-            if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) {
-                b = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-            } else {
-                u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-                b = (uint16_t)(u >> 16);
-            }
-            reinterpret_cast<uint16_t *>(out)[i] = b;
-        } else {
-            reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
-        }
-    }
+         i += (int64_t)gridDim.x * blockDim.x)
+        store_element(A[i], i, n, dtype, out);
 }
 
 __global__ void convert_scaled_kernel(const float *A, int64_t n_rows, int n_cols, int dtype,
@@ -173,19 +188,7 @@ __global__ void convert_scaled_kernel(const float *A, int64_t n_rows, int n_cols
          i += (int64_t)gridDim.x * blockDim.x) {
         float v = A[i];
         if (scale) v *= scale[i % n_cols];  // power-of-two scale: exact
-        if (dtype == XRD_DTYPE_BF16) {
-            uint32_t u = __float_as_uint(v);
-            uint16_t b;
-            if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) {
-                b = (uint16_t)((u >> 16) | 0x40u);
-            } else {
-                u += 0x7fffu + ((u >> 16) & 1u);
-                b = (uint16_t)(u >> 16);
-            }
-            reinterpret_cast<uint16_t *>(out)[i] = b;
-        } else {
-            reinterpret_cast<__half *>(out)[i] = __float2half_rn(v);
-        }
+        store_element(v, i, n, dtype, out);
     }
 }
 
@@ -273,6 +276,7 @@ static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb 
             case XRD_DTYPE_F16:                                                      \
                 if (v8) { using T = f16x8; return CALL; }                            \
                 else { using T = f16x4; return CALL; }                               \
+            case XRD_DTYPE_BF24: { using T = bf24x8; return CALL; }                  \
         }                                                                            \
         return hipErrorInvalidValue;                                                 \
     } while (0)

@@ -474,14 +474,17 @@ namespace {
 
 int check_matrix(const void *A, int dtype, int64_t n_rows, int32_t n_cols, const char *fn) {
     if (!A) return fail(XRD_ERR_NULL_POINTER, "%s: A is NULL", fn);
-    if (dtype != XRD_DTYPE_F32 && dtype != XRD_DTYPE_BF16 && dtype != XRD_DTYPE_F16)
+    if (dtype != XRD_DTYPE_F32 && dtype != XRD_DTYPE_BF16 && dtype != XRD_DTYPE_F16 &&
+        dtype != XRD_DTYPE_BF24)
         return fail(XRD_ERR_INVALID_ARG, "%s: unknown dtype %d", fn, dtype);
     if (n_rows <= 0 || n_cols <= 0) return fail(XRD_ERR_DIMS, "%s: non-positive size", fn);
     if (n_cols % 4 != 0) return fail(XRD_ERR_DIMS, "%s: n_cols must be a multiple of 4", fn);
+    if (dtype == XRD_DTYPE_BF24 && n_cols % 8 != 0)
+        return fail(XRD_ERR_DIMS, "%s: BF24 needs n_cols a multiple of 8", fn);
     if (n_cols > kMaxMatCols)
         return fail(XRD_ERR_UNSUPPORTED, "%s: n_cols > %d", fn, kMaxMatCols);
-    if (!aligned(A, dtype == XRD_DTYPE_F32 ? 16 : 8))
-        return fail(XRD_ERR_ALIGN, "%s: A misaligned", fn);
+    const int align = dtype == XRD_DTYPE_F32 || dtype == XRD_DTYPE_BF24 ? 16 : 8;
+    if (!aligned(A, align)) return fail(XRD_ERR_ALIGN, "%s: A must be %d-byte aligned", fn, align);
     return XRD_OK;
 }
 
@@ -537,8 +540,8 @@ extern "C" int xrd_convert_matrix(xrd_context *ctx, const float *A, int64_t n, i
     if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
     if (!A || !out) return fail(XRD_ERR_NULL_POINTER, "%s: A/out is NULL", fn);
     if (n < 0) return fail(XRD_ERR_DIMS, "%s: n < 0", fn);
-    if (out_dtype != XRD_DTYPE_BF16 && out_dtype != XRD_DTYPE_F16)
-        return fail(XRD_ERR_INVALID_ARG, "%s: out_dtype must be BF16 or F16", fn);
+    if (out_dtype != XRD_DTYPE_BF16 && out_dtype != XRD_DTYPE_F16 && out_dtype != XRD_DTYPE_BF24)
+        return fail(XRD_ERR_INVALID_ARG, "%s: out_dtype must be BF16, F16 or BF24", fn);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     HIP_TRY(launch_convert(A, n, out_dtype, out, (hipStream_t)stream), "convert kernel");
     return XRD_OK;
@@ -936,10 +939,16 @@ extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
 
 // ---- batched MLEM -----------------------------------------------------------------
 
+// Launch shape of one row-span group (index log2(span / 256)).
+struct BatchShape {
+    int blocks_per_cu = 2;
+    int rows_per_wave = 2;
+};
+
 struct xrd_mlem_batch {
     xrd_context *ctx = nullptr;
     BatchArgs args{};
-    int blocks_per_cu = 2;
+    BatchShape shape[4];
     int esize = 4;
     std::vector<BatchProblem> host_probs;
     DevBuf probs, segs, wg_seg, partial, sens;
@@ -1039,18 +1048,20 @@ hipError_t batch_set_grid(xrd_mlem_batch *b) {
     std::vector<int> wg;
     a.n_groups = 0;
     const int cus = device_cus();
-    for (int span = 256; span <= kMaxMatCols; span *= 2) {
+    for (int si = 0, span = 256; span <= kMaxMatCols; si++, span *= 2) {
         std::vector<int> members;
         for (int k = 0; k < (int)probs.size(); k++)
             if (probs[k].span == span) members.push_back(k);
         if (members.empty()) continue;
+        const BatchShape &sh = b->shape[si];
         int resident = 0;
-        hipError_t e = batch_resident_blocks(a, span, &resident);
+        hipError_t e = batch_resident_blocks(a, span, sh.rows_per_wave, &resident);
         if (e != hipSuccess) return e;
-        int bpc = b->blocks_per_cu;
+        int bpc = sh.blocks_per_cu;
         if (resident > 0 && bpc > resident) bpc = resident;
         BatchGroup &gr = a.groups[a.n_groups++];
         gr.span = span;
+        gr.rows_per_wave = sh.rows_per_wave;
         gr.grid = cus * bpc;
         gr.wg_seg_offset = (int)wg.size();
         partition_group(probs, members, b->esize, gr.grid, segs, wg);
@@ -1129,7 +1140,7 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     hipStream_t s = (hipStream_t)stream;
     xrd_mlem_batch *b = new xrd_mlem_batch();
     b->ctx = ctx;
-    b->esize = a_dtype == XRD_DTYPE_F32 ? 4 : 2;
+    b->esize = a_dtype == XRD_DTYPE_F32 ? 4 : (a_dtype == XRD_DTYPE_BF24 ? 3 : 2);
     size_t sens_total = 0;
     for (auto &p : probs) {
         p.sens_offset = (int)sens_total;
@@ -1145,7 +1156,6 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     a.max_cols = max_cols;
     a.stride = max_cols;
     a.dtype = a_dtype;
-    a.rows_per_wave = 0;
     a.vec8 = vec8;
     if (e == hipSuccess) e = batch_set_grid(b);
     // sensitivities sum(A_k,1)' once, FP64
@@ -1198,16 +1208,20 @@ extern "C" int xrd_mlem_batch_set_launch(xrd_mlem_batch *b, int32_t blocks_per_c
     // descriptors are replaced below: wait for passes that still read them
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     batch_drop_graph(b);
-    b->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 2;
-    b->args.rows_per_wave = rows_per_wave;
+    for (auto &sh : b->shape) {
+        sh.blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 2;
+        sh.rows_per_wave = rows_per_wave > 0 ? rows_per_wave : 2;
+    }
     HIP_TRY(batch_set_grid(b), "batch partition");
     return XRD_OK;
 }
 
 namespace {
 
-// Mean duration (ms) of reps streaming passes of the batch at its current shape.
-hipError_t batch_time_stream(xrd_mlem_batch *b, int reps, hipStream_t s, float *ms) {
+// Mean duration (ms) of reps streaming passes of the batch at its current shape
+// (group >= 0: that group's launch only).
+hipError_t batch_time_stream(xrd_mlem_batch *b, int reps, hipStream_t s, float *ms,
+                             int group = -1) {
     hipEvent_t e0, e1;
     hipError_t err = hipEventCreate(&e0);
     if (err != hipSuccess) return err;
@@ -1217,9 +1231,10 @@ hipError_t batch_time_stream(xrd_mlem_batch *b, int reps, hipStream_t s, float *
         return err;
     }
     // the streaming pass reads x, y, c and writes the partials only: x is untouched
-    err = launch_batch_stream(b->args, true, s);
+    err = launch_batch_stream(b->args, true, s, group);
     if (err == hipSuccess) err = hipEventRecord(e0, s);
-    for (int i = 0; i < reps && err == hipSuccess; i++) err = launch_batch_stream(b->args, true, s);
+    for (int i = 0; i < reps && err == hipSuccess; i++)
+        err = launch_batch_stream(b->args, true, s, group);
     if (err == hipSuccess) err = hipEventRecord(e1, s);
     if (err == hipSuccess) err = hipEventSynchronize(e1);
     float t = 0.f;
@@ -1242,31 +1257,64 @@ extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *st
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     batch_drop_graph(b);
+    // each row-span group is launched on its own, so each gets its own shape
     const int bpcs[] = {1, 2, 3, 4};
     const int rows[] = {1, 2, 4};
-    float best = 1e30f;
-    int best_b = 2, best_r = 2;
-    for (int r : rows) {
-        b->args.rows_per_wave = r;
-        for (int bpc : bpcs) {
-            b->blocks_per_cu = bpc;
-            HIP_TRY(batch_set_grid(b), "batch partition");
-            float ms = 0.f;
-            HIP_TRY(batch_time_stream(b, reps, s, &ms), "timed pass");
-            if (ms < best) {
-                best = ms;
-                best_b = bpc;
-                best_r = r;
+    for (int g = 0; g < b->args.n_groups; g++) {
+        int si = 0;
+        while ((256 << si) != b->args.groups[g].span) si++;
+        BatchShape best_shape = b->shape[si];
+        float best = 1e30f;
+        for (int r : rows) {
+            for (int bpc : bpcs) {
+                b->shape[si].rows_per_wave = r;
+                b->shape[si].blocks_per_cu = bpc;
+                HIP_TRY(batch_set_grid(b), "batch partition");
+                float ms = 0.f;
+                HIP_TRY(batch_time_stream(b, reps, s, &ms, g), "timed pass");
+                if (ms < best) {
+                    best = ms;
+                    best_shape = b->shape[si];
+                }
             }
         }
+        b->shape[si] = best_shape;
     }
-    b->blocks_per_cu = best_b;
-    b->args.rows_per_wave = best_r;
     HIP_TRY(batch_set_grid(b), "batch partition");
-    if (out_blocks_per_cu) *out_blocks_per_cu = best_b;
-    if (out_rows_per_wave) *out_rows_per_wave = best_r;
-    if (out_ms) *out_ms = best;
+    float total = 0.f;
+    HIP_TRY(batch_time_stream(b, reps, s, &total), "timed pass");
+    // reported: the shape of the group with the most rows-bytes (the first group if
+    // there is one only)
+    int main_g = 0;
+    double most = -1.0;
+    for (int g = 0; g < b->args.n_groups; g++) {
+        double cost = 0.0;
+        for (const auto &p : b->host_probs)
+            if (p.span == b->args.groups[g].span) cost += (double)p.n_rows * p.n_cols;
+        if (cost > most) {
+            most = cost;
+            main_g = g;
+        }
+    }
+    int si = 0;
+    while (b->args.n_groups > 0 && (256 << si) != b->args.groups[main_g].span) si++;
+    if (out_blocks_per_cu) *out_blocks_per_cu = b->shape[si].blocks_per_cu;
+    if (out_rows_per_wave) *out_rows_per_wave = b->shape[si].rows_per_wave;
+    if (out_ms) *out_ms = total;
     return XRD_OK;
+}
+
+extern "C" int xrd_mlem_batch_groups(const xrd_mlem_batch *b, int32_t max_groups,
+                                     int32_t *out_span, int32_t *out_grid,
+                                     int32_t *out_rows_per_wave) {
+    if (!b) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_batch_groups: batch is NULL");
+    const int n = b->args.n_groups;
+    for (int g = 0; g < n && g < max_groups; g++) {
+        if (out_span) out_span[g] = b->args.groups[g].span;
+        if (out_grid) out_grid[g] = b->args.groups[g].grid;
+        if (out_rows_per_wave) out_rows_per_wave[g] = b->args.groups[g].rows_per_wave;
+    }
+    return n;
 }
 
 extern "C" int xrd_mlem_batch_time_pass(xrd_mlem_batch *b, int32_t reps, void *stream,

@@ -199,6 +199,7 @@ struct BatchGroup {
     int span;
     int grid;
     int wg_seg_offset;
+    int rows_per_wave;    // rows per row slot and step (1, 2 or 4)
 };
 
 struct BatchArgs {
@@ -206,7 +207,6 @@ struct BatchArgs {
     const BatchSegment *segs;
     const int *wg_seg;    // the groups' workgroup-to-segment offsets, concatenated
     int n_problems, n_segs, max_cols, stride, dtype;
-    int rows_per_wave;    // 0 = default
     int vec8;             // 2-byte matrices: every n_cols % 8 == 0 and A 16-byte aligned
     int n_groups;
     BatchGroup groups[4];  // spans 256, 512, 1024, 2048 that occur, ascending
@@ -216,10 +216,11 @@ struct BatchArgs {
 // LDS doubles of the batch kernels' block reduction (the widest row shape).
 constexpr int kBatchRedDoubles = 2048;
 
-hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s);
+// group < 0: every group in turn; else that group only.
+hipError_t launch_batch_stream(const BatchArgs &b, bool mlem, hipStream_t s, int group = -1);
 hipError_t launch_batch_update(const BatchArgs &b, double *sens_out, bool update, hipStream_t s);
-// Workgroups of the batch kernel for row span `span` (launch shape of b) that fit
+// Workgroups of the batch kernel for row span `span` and rows_per_wave R that fit
 // on one CU at once.
-hipError_t batch_resident_blocks(const BatchArgs &b, int span, int *blocks_per_cu);
+hipError_t batch_resident_blocks(const BatchArgs &b, int span, int R, int *blocks_per_cu);
 
 }  // namespace xrd
