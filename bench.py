@@ -26,7 +26,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the streaming pass, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (profiles/mlem_fused_fan256_hbm_traffic.md): FETCH_SIZE scaled by 2.0 and
 # WRITE_SIZE by 1.0, the factors that map each counter onto a 2 GiB copy kernel.
-MEASURED_TRAFFIC = {("fan256", "f32", "fused"): 23_076_117_299}
+MEASURED_TRAFFIC = {("fan256", "f32", "fused"): 23_076_164_551,
+                    ("fan256", "bf24", "fused"): 17_321_587_470}
 ESIZE = {"f32": 4, "bf16": 2, "f16": 2, "bf24": 3}  # bytes per stored element of A
 
 
