@@ -84,3 +84,56 @@ def test_poisson_nll(xrd):
     ref = np.sum(F - np.where(y > 0, y * np.log(F), 0.0))
     assert abs(loss - ref) <= 1e-12 * abs(ref)
     assert np.allclose(gr.cpu().numpy(), 1 - y / F, rtol=1e-6, atol=1e-7)
+
+
+def _autograd_counts():
+    """torch.autograd.Function over the library's forward and VJP (test harness only):
+    counts = F(coef, maps); backward maps the upstream gradient through xrd_vjp_coeffs."""
+    import torch
+    from xrd_tomography_amd import api
+
+    class XrdCounts(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, coef, maps, spectrum, seg, basis, beta, geometry, disc):
+            ctx.save_for_backward(coef, maps, spectrum, seg, basis, beta)
+            ctx.geometry, ctx.disc = geometry, disc
+            return api.forward_coeffs(spectrum, seg, coef, basis, maps, beta,
+                                      geometry=geometry, disc=disc).reshape(-1)
+
+        @staticmethod
+        def backward(ctx, grad):
+            coef, maps, spectrum, seg, basis, beta = ctx.saved_tensors
+            gc, gm = api.vjp_coeffs(spectrum, seg, coef, basis, maps, beta,
+                                    grad.float().contiguous(), geometry=ctx.geometry,
+                                    disc=ctx.disc, want_coef=ctx.needs_input_grad[0],
+                                    want_maps=ctx.needs_input_grad[1])
+            return (gc.to(coef.dtype) if gc is not None else None,
+                    gm.to(maps.dtype) if gm is not None else None,
+                    None, None, None, None, None, None)
+
+    return XrdCounts
+
+
+def test_autograd_function_poisson_loss(xrd, oracle):
+    """The VJP wrapped as a torch.autograd.Function: the gradient of a Poisson NLL
+    written in torch equals the oracle's J'(1 - y/f) for coefficients and maps."""
+    import torch
+    cp = _case()
+    sp, seg, coef, basis, maps, beta = xrd.coeff_problem_tensors(cp)
+    g, d = cp.base.geometry, cp.base.disc
+    F = _autograd_counts()
+    coef = coef.clone().requires_grad_(True)
+    maps = maps.clone().requires_grad_(True)
+    with torch.no_grad():
+        f0 = F.apply(coef, maps, sp, seg, basis, beta, g, d).double()
+    rng = np.random.default_rng(6)
+    bg = torch.full_like(f0, 0.05 * float(f0.mean()))
+    y = torch.from_numpy(rng.poisson((f0 + bg).cpu().numpy())).cuda().double()
+    f = F.apply(coef, maps, sp, seg, basis, beta, g, d).double() + bg
+    loss = (f - torch.where(y > 0, y * torch.log(f), torch.zeros_like(f))).sum()
+    loss.backward()
+    w = (1.0 - y / f).detach().cpu().numpy()
+    gco, gmo = oracle.vjp_coeffs(cp, w)
+    ec, em = _rel(coef.grad.cpu().numpy(), gco), _rel(maps.grad.cpu().numpy(), gmo)
+    print(f"\n[autograd] coef grad rel err {ec:.2e}, maps grad rel err {em:.2e}")
+    assert ec <= 1e-5 and em <= 1e-5
