@@ -465,19 +465,36 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                 }
             }
             // ---- phase 3: Gaussian resolution terms (integrate_diffraction.cu:298-315)
-            for (int j = 0; j < nk; j++) {
+            // For each source energy of the wave, one ballot gives the chunk's voxels
+            // whose band reaches the wave's coarse and fine bins; the wave then walks
+            // the set bits only (scalar bit scans), adding voxels in ascending order.
+            {
+                const int lane = t & (kWave - 1);
 #pragma unroll
                 for (int i = 0; i < EPT; i++) {
                     const int e = h + G * i;
                     if (e >= E) continue;  // wave-uniform
-                    const float4 tv = tw[j * E + e];
-                    const float band = tb[j * E + e];
-                    if (WITH_A && tv.x + band >= wlo_c && tv.x - band <= whi_c) {
+                    bool in_c = false, in_f = false;
+                    if (lane < nk) {
+                        const float x = tw[lane * E + e].x;
+                        const float band = tb[lane * E + e];
+                        in_c = WITH_A && x + band >= wlo_c && x - band <= whi_c;
+                        in_f = x + band >= wlo_f && x - band <= whi_f;
+                    }
+                    uint64_t mc = __ballot(in_c);
+                    uint64_t mf = __ballot(in_f);
+                    while (mc) {
+                        const int j = __ffsll((unsigned long long)mc) - 1;
+                        mc &= mc - 1;
+                        const float4 tv = tw[j * E + e];
                         const float d = (tv.x - qch) + (tv.y - qcl);
                         const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
                         pc[i] = fmaf(tv.w * dqc, g, pc[i]);
                     }
-                    if (tv.x + band >= wlo_f && tv.x - band <= whi_f) {
+                    while (mf) {
+                        const int j = __ffsll((unsigned long long)mf) - 1;
+                        mf &= mf - 1;
+                        const float4 tv = tw[j * E + e];
                         float acc = 0.f;
 #pragma unroll
                         for (int s = 0; s < FS; s++) {
