@@ -185,6 +185,18 @@ hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStr
 // ---------------------------------------------------------------------------
 // forward accumulation
 
+// exp(-x) for finite x >= 0 to FP32 accuracy (about 2e-7 relative): x log2(e) is
+// split in FP64 into an integer n and a fraction |f| <= 1/2, 2^f is v_exp_f32 of
+// the fraction, and n goes into the exponent.  The results feed FP32 table
+// entries, so the FP64 library exponential is not needed.
+__device__ __forceinline__ double exp_neg(double x) {
+    double y = -x * 1.4426950408889634074;
+    y = y < -400.0 ? -400.0 : y;
+    const double n = rint(y);
+    const float f = (float)(y - n);
+    return (double)ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
 struct VoxShared {
     double cth, s, As, Bs, gR, gp, a2c;
     double lin0, lin1, lout0, lout1;
@@ -239,6 +251,44 @@ __device__ __forceinline__ double f2_kn(const ForwardArgs &a, double e) {
     return a.kn0 + a.kn1 * e + a.kn2 * e * e;
 }
 
+// v[k] of lane `from` for DPP control CTRL (a permutation within 16 lanes).
+template <int CTRL> __device__ __forceinline__ float dpp_get(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Sums 16 values per lane over the wave so that lane l ends up with the wave total
+// of value (l >> 2) & 15.  Each step pairs lanes that differ in one lane bit; the
+// lane with the bit clear keeps the lower half of its values, its partner the upper
+// half, and each adds what the other sends: bits 5 and 4 by shuffles, bit 3 by
+// row_mirror (lane i <-> 15 - i), bit 2 by row_half_mirror (i <-> 7 - i); bits 1
+// and 0 then add the one remaining value (quad_perm swaps).
+__device__ __forceinline__ float transpose_sum16(const float *v, int lane) {
+    float w[8];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float keep = b5 ? v[8 + k] : v[k], send = b5 ? v[k] : v[8 + k];
+        w[k] = keep + __shfl_xor(send, 32, kWave);
+    }
+    float u[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float keep = b4 ? w[4 + k] : w[k], send = b4 ? w[k] : w[4 + k];
+        u[k] = keep + __shfl_xor(send, 16, kWave);
+    }
+    float z[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float keep = b3 ? u[2 + k] : u[k], send = b3 ? u[k] : u[2 + k];
+        z[k] = keep + dpp_get<0x140>(send);
+    }
+    const float keep = b2 ? z[1] : z[0], send = b2 ? z[0] : z[1];
+    float r = keep + dpp_get<0x141>(send);
+    r += dpp_get<0x4E>(r);
+    r += dpp_get<0xB1>(r);
+    return r;
+}
+
 // Block of G*TQ lanes: lane (tq, h) owns coarse bin tq, fine bins tq*FS..tq*FS+FS-1
 // and the source energies e = h + G*i, i < EPT.
 template <int TQ, int G, int EPT, int FS, bool WITH_A>
@@ -258,16 +308,20 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
     VoxShared *vox = reinterpret_cast<VoxShared *>(smem);
     unsigned char *r2 = smem + ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
     float4 *part = reinterpret_cast<float4 *>(r2);                 // [kc][nsplit]
-    float4 *tw = part + NT;                                          // [kc][E]
+    double2 *teo = reinterpret_cast<double2 *>(part + NT);           // [kc][E] Compton E_out bin
+    float4 *tw = reinterpret_cast<float4 *>(teo + kc * E);           // [kc][E]
     float *tb = reinterpret_cast<float *>(tw + kc * E);              // [kc][E] band
-    float *tk = tb + kc * E;                                         // [kc][E] Compton K
-    float *hst = reinterpret_cast<float *>(r2);                      // [E][Q] staging (aliases)
+    float *tk = tb + kc * E;                                         // [kc][E] Compton K / width
+    const int nwin = 2 * a.dn + 1;
+    float *rfc = tk + kc * E;     // [E][nwin] spectrum window of each detector bin
+    float *hst = rfc + E * nwin;  // [E][Q] per-material coarse totals (A only)
     double *red = reinterpret_cast<double *>(r2);                    // final reductions (aliases)
 
-    // ---- per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64
-    PixelGeom pg;
-    {
+    // ---- per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64, in LDS
+    __shared__ PixelGeom pg_s;
+    if (t == 0) {
 #pragma clang fp contract(off)
+        PixelGeom pg;
         const ViewConst vc = a.views[v];
         pg.sp = vc.sp;
         pg.cp = vc.cp;
@@ -282,7 +336,10 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
         pg.nsx = -vc.sp * a.sta;
         pg.nsy = vc.cp * a.sta;
         pg.nsz = -a.cta;
+        pg_s = pg;
     }
+    __syncthreads();
+    const PixelGeom &pg = pg_s;
 
     // ---- per-lane q constants
     float qch = 0.f, qcl = 0.f, dqc = 0.f;
@@ -333,16 +390,26 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
     }
 
     // ---- accumulators
-    // per-chunk partials (at most kc terms each), per-material coarse totals
-    // (FP32, one add per chunk) and FP64 fine totals
-    float pc[EPT], pf[EPT], tc[EPT];
-    double ft[EPT];
+    // per-chunk partials (at most kc terms each); the per-material coarse totals in
+    // LDS (hst[e][tq], FP32, one add per chunk); the fine totals per wave and
+    // energy slot in LDS (ftw[wave][i], FP64, one add per chunk of the chunk's
+    // wave total)
+    static_assert(EPT <= 16, "the fine totals use the 16-value transposed wave sum");
+    __shared__ double ftw[NT / kWave][16];
+    float pc[EPT], pf[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         pc[i] = 0.f;
         pf[i] = 0.f;
-        tc[i] = 0.f;
-        ft[i] = 0.0;
+    }
+    if (t < NT / kWave * 16) (&ftw[0][0])[t] = 0.0;
+    if (WITH_A)
+        for (int idx = t; idx < E * Q; idx += NT) hst[idx] = 0.f;
+    for (int idx = t; idx < E * nwin; idx += NT) {
+        const int ed = idx / nwin, k = idx % nwin;
+        int wlo, whi;
+        src_window(ed, E, a.dn, wlo, whi);
+        rfc[idx] = wlo + k <= whi ? a.spectrum[(wlo + k) + (size_t)E * ed] : 0.f;
     }
     double cacc = 0.0;
     const int ced = t % E, cgrp = t / E;
@@ -368,8 +435,22 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                     const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
                     const Ray ray = make_ray(pg.dx, pg.dy, (double)ix + 0.5, (double)iy + 0.5,
                                              a.nx, a.ny, a.ray_step);
+                    // two samples per step so that both samples' texel loads are in
+                    // flight together; the Kahan sums still add samples in order
                     Kahan s1, s2;
-                    for (int q = r; q < ray.n; q += nsplit) {
+                    int q = r;
+                    for (; q + nsplit < ray.n; q += 2 * nsplit) {
+                        double px0, py0, px1, py1;
+                        ray_sample(ray, a.ray_step, q, px0, py0);
+                        ray_sample(ray, a.ray_step, q + nsplit, px1, py1);
+                        const float2 tv0 = bilinear2(a.mu_padded, a.nx, a.ny, px0, py0);
+                        const float2 tv1 = bilinear2(a.mu_padded, a.nx, a.ny, px1, py1);
+                        s1.add(tv0.x);
+                        s2.add(tv0.y);
+                        s1.add(tv1.x);
+                        s2.add(tv1.y);
+                    }
+                    if (q < ray.n) {
                         double px, py;
                         ray_sample(ray, a.ray_step, q, px, py);
                         const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
@@ -418,7 +499,7 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                 const VoxShared &vs = vox[j];
                 const EnergyConst ec = a.energies[e];
                 const double arg = (vs.lin0 + vs.lout0) * ec.inv_f1 + (vs.lin1 + vs.lout1) * ec.inv_f2;
-                const double pinpout = exp(-arg);
+                const double pinpout = exp_neg(arg);
                 const double sig2 = vs.As + vs.Bs * ec.k2sq;
                 const double w = ec.wcoef * vs.s;
                 const float wh = (float)w;
@@ -435,12 +516,16 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                 const double l2w = log2(wc * sqrt(er + wc * wc) / er);
                 const int nq = lut_index((l2w - a.lut_q_min) / a.dq_lut, a.lut_nq);
                 const double g1 = eout * eout * eout, g2 = f2_kn(a, eout);
-                const double pin = exp(-(vs.lin0 * ec.inv_f1 + vs.lin1 * ec.inv_f2));
-                const double pc_ = exp(-(vs.lout0 / g1 + vs.lout1 / g2));
+                const double pin = exp_neg(vs.lin0 * ec.inv_f1 + vs.lin1 * ec.inv_f2);
+                const double pc_ = exp_neg(vs.lout0 / g1 + vs.lout1 / g2);
                 const double cf = er * er * (er + 1.0 / er - 1.0 + vs.cth * vs.cth) / 2.0;
                 const double K = pin * pc_ * vs.gp * cf * vs.a2c *
                                  (double)a.s2u8[nq + a.lut_nq * vs.na];
-                tk[j * E + e] = (float)K;
+                // Compton-shifted source bin [es1, es2] / er (channel_match.cu): the
+                // detector lanes add (overlap with their bin) * K / (bin width)
+                const double eo1 = ec.es1 / er, eo2 = ec.es2 / er;
+                teo[j * E + e] = make_double2(eo1, eo2);
+                tk[j * E + e] = (float)(K / (eo2 - eo1));
             }
             __syncthreads();
             // ---- Compton accumulation over the detector-energy rows
@@ -450,17 +535,13 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                 const double Ed1 = a.e_min + (double)ced * a.e_width;
                 const double Ed2 = a.e_min + (double)(ced + 1) * a.e_width;
                 for (int j = cgrp; j < nk; j += Gc) {
-                    const double cth = vox[j].cth;
                     for (int e = wlo; e <= whi; e++) {
-                        const EnergyConst ec = a.energies[e];
-                        const double er = 1.0 + ec.ein * (1.0 - cth);
-                        const double Eo1 = ec.es1 / er, Eo2 = ec.es2 / er;
-                        const double lo_e = Eo1 > Ed1 ? Eo1 : Ed1;
-                        const double hi_e = Eo2 < Ed2 ? Eo2 : Ed2;
-                        double cm = hi_e - lo_e;
+                        const double2 eo = teo[j * E + e];
+                        const double lo_e = eo.x > Ed1 ? eo.x : Ed1;
+                        const double hi_e = eo.y < Ed2 ? eo.y : Ed2;
+                        const double cm = hi_e - lo_e;
                         if (!(cm > 0.0)) continue;
-                        cm = cm / (Eo2 - Eo1);
-                        cacc += cm * (double)a.spectrum[e + (size_t)E * ced] * (double)tk[j * E + e];
+                        cacc += cm * (double)rfc[ced * nwin + (e - wlo)] * (double)tk[j * E + e];
                     }
                 }
             }
@@ -505,10 +586,18 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                     }
                 }
             }
+            {
+                const int lane = t & (kWave - 1);
+                float pf16[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) pf16[i] = i < EPT ? pf[i] : 0.f;
+                const float sf = transpose_sum16(pf16, lane);
+                if ((lane & 3) == 0) ftw[t >> 6][(lane >> 2) & 15] += (double)sf;
+            }
 #pragma unroll
             for (int i = 0; i < EPT; i++) {
-                if (WITH_A) tc[i] += pc[i];
-                ft[i] += (double)pf[i];
+                const int e = h + G * i;
+                if (WITH_A && e < E && tq < Q) hst[e * Q + tq] += pc[i];
                 pc[i] = 0.f;
                 pf[i] = 0.f;
             }
@@ -517,13 +606,6 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
         if (WITH_A) {
             // ---- material flush: apply the detector-response window
             // A[nwt, mat, e_det] = sum_{e in window(e_det)} spectrum[e, e_det] * H[e, nwt]
-#pragma unroll
-            for (int i = 0; i < EPT; i++) {
-                const int e = h + G * i;
-                if (e < E && tq < Q) hst[e * Q + tq] = tc[i];
-                tc[i] = 0.f;
-            }
-            __syncthreads();
             for (int ed = h; ed < E; ed += G) {
                 if (tq < Q) {
                     int wlo, whi;
@@ -536,24 +618,23 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
                 }
             }
             __syncthreads();
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = h + G * i;
+                if (e < E && tq < Q) hst[e * Q + tq] = 0.f;
+            }
         }
     }
 
-    // ---- counts: reduce fine totals over the q lanes of each energy group
-    const int lane = t & 63, wid = t >> 6;
+    // ---- counts: add the fine totals of the waves of each energy group
     const int nw = NT / kWave;  // waves; waves [h*TQ/64, (h+1)*TQ/64) hold group h
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-        const double sv = wave_sum(ft[i]);
-        if (lane == 0) red[wid * EPT + i] = sv;
-    }
     __syncthreads();
     double *hf = red + nw * EPT;  // [E]
     double *cr = hf + E;          // [Gc][E]
     if (t < E) {
         const int hh = t % G, i = t / G;
         double sacc = 0.0;
-        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++) sacc += red[w * EPT + i];
+        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++) sacc += ftw[w][i];
         hf[t] = sacc;
     }
     if (compton_lane) cr[cgrp * E + ced] = cacc;
@@ -578,8 +659,8 @@ static hipError_t launch_fwd_t(const ForwardArgs &a, hipStream_t s) {
     const int E = a.n_energy;
     const int Gc = NT / E;
     const size_t vox_bytes = ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
-    size_t r2 = (size_t)NT * 16 + (size_t)kc * E * (16 + 4 + 4);
-    r2 = r2 > (size_t)E * a.n_q * 4 ? r2 : (size_t)E * a.n_q * 4;
+    size_t r2 = (size_t)NT * 16 + (size_t)kc * E * (16 + 16 + 4 + 4) +
+                (size_t)E * (2 * a.dn + 1) * 4 + (WITH_A ? (size_t)E * a.n_q * 4 : 0);
     const size_t redb = ((size_t)(NT / kWave) * EPT + E + (size_t)Gc * E) * 8;
     r2 = r2 > redb ? r2 : redb;
     const size_t shmem = vox_bytes + r2;

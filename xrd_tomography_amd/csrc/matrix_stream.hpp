@@ -153,29 +153,6 @@ template <int W, int SPAN> struct RowShape {
     static_assert(L * NG * W == SPAN, "row shape must tile the span");
 };
 
-// v + v of the lane that DPP control CTRL selects (a permutation within rows of
-// 16 lanes).
-template <int CTRL> __device__ __forceinline__ float dpp_add(float v) {
-    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf,
-                                                          false));
-}
-
-// Sum over the L lanes (16, 32 or 64) of each row slot; every lane of the slot
-// gets the same bits.  Within 16 lanes: swap neighbours (quad_perm 1,0,3,2), swap
-// pairs (quad_perm 2,3,0,1), swap quads (row_half_mirror), swap halves
-// (row_mirror); each step adds two equal-rank partial sums, so both partners get
-// identical results.  Then lanes 16 and 32 apart are added.
-template <int L> __device__ __forceinline__ float group_sum(float v) {
-    static_assert(L == 16 || L == 32 || L == 64, "row slots are 16, 32 or 64 lanes");
-    v = dpp_add<0xB1>(v);
-    v = dpp_add<0x4E>(v);
-    v = dpp_add<0x141>(v);
-    v = dpp_add<0x140>(v);
-    if (L >= 32) v += __shfl_xor(v, 16, kWave);
-    if (L >= 64) v += __shfl_xor(v, 32, kWave);
-    return v;
-}
-
 // x (optionally divided by the per-column storage scale) in the lane's columns
 // W (li + L g) + i; gv[g] marks column groups inside the matrix.  x = NULL
 // leaves zeros.
