@@ -86,7 +86,7 @@ typedef struct xrd_geometry {
  * (Forward_XRD.cu:21-53, integrate_diffraction.cu:138-142, wvt_value.cu:8-23). */
 typedef struct xrd_discretization {
     int32_t n_energy;        /* NRG: source and detector energy bins                    */
-    int32_t energy_halfwidth;/* DNRG: window is 2*DNRG+1 source bins (needs NRG > 2*DNRG) */
+    int32_t energy_halfwidth;/* DNRG: window is 2*DNRG+1 source bins (needs NRG > 2*DNRG, DNRG <= 7) */
     int32_t n_q;             /* NWT: reconstruction q-bins (parabolic axis)             */
     int32_t n_qf;            /* NQF: fine uniform q-bins used for the counts            */
     int32_t n_mat;           /* MAT: materials (segmentation ids 1..MAT, 0 = air)       */

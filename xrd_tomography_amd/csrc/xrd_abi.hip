@@ -220,6 +220,8 @@ int check_geometry(const xrd_geometry *g, const xrd_discretization *d, const cha
     if (d->n_energy <= 0 || d->n_q <= 0 || d->n_qf <= 0 || d->n_mat <= 0 || d->lut_nq <= 0 ||
         d->lut_na <= 0 || d->energy_halfwidth < 0)
         return fail(XRD_ERR_DIMS, "%s: non-positive discretization size", fn);
+    if (d->energy_halfwidth > kMaxEnergyHalfwidth)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: energy_halfwidth > %d", fn, kMaxEnergyHalfwidth);
     if (d->n_energy <= 2 * d->energy_halfwidth)
         return fail(XRD_ERR_DIMS, "%s: n_energy (%d) must exceed 2*energy_halfwidth (%d)", fn,
                     d->n_energy, 2 * d->energy_halfwidth);

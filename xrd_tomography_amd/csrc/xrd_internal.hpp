@@ -31,6 +31,10 @@ struct ViewConst {
     double src_x, src_y;  // source position
 };
 
+// Largest detector-response half width (DNRG) the forward kernels support: the
+// window of 2 DNRG + 1 source bins is unrolled.
+constexpr int kMaxEnergyHalfwidth = 7;
+
 struct ForwardArgs {
     // sizes
     int nx, ny, n_views, n_cols, n_rows, n_det;
