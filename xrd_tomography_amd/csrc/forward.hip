@@ -289,26 +289,88 @@ __device__ __forceinline__ float transpose_sum16(const float *v, int lane) {
     return r;
 }
 
+// Per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64.
+__device__ __forceinline__ PixelGeom pixel_geometry(const ForwardArgs &a, int v, int p) {
+#pragma clang fp contract(off)
+    const int col = p % a.n_cols, row = p / a.n_cols;
+    PixelGeom pg;
+    const ViewConst vc = a.views[v];
+    pg.sp = vc.sp;
+    pg.cp = vc.cp;
+    pg.sx = vc.src_x;
+    pg.sy = vc.src_y;
+    const double rdet = ((double)col - (double)a.n_cols / 2.0 + 0.5) * a.col_pitch;
+    pg.dx = a.center_x - a.r_det * vc.sp + rdet * vc.cp;
+    pg.dy = a.center_y + a.r_det * vc.cp + rdet * vc.sp;
+    pg.dz = a.det_z0 + (double)row * a.row_pitch;
+    pg.ndx = vc.sp;
+    pg.ndy = -vc.cp;
+    pg.nsx = -vc.sp * a.sta;
+    pg.nsy = vc.cp * a.sta;
+    pg.nsz = -a.cta;
+    return pg;
+}
+
+// One thread per (view of the batch, pixel, voxel): the detector leg's line
+// integrals (attenuator.cu:9-109; FP64 sample positions, Kahan FP32 sums) and the
+// FP64 scattering geometry (integrate_diffraction.cu:192-255), written as one
+// record for forward_kernel.  Consecutive threads take consecutive voxels of the
+// material-major list, so their rays run side by side through the map.
+__global__ __launch_bounds__(256) void pixel_voxel_kernel(ForwardArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nvox) return;
+    const int p = blockIdx.y, vb = blockIdx.z;
+    const PixelGeom pg = pixel_geometry(a, a.v0 + vb, p);
+    const VoxelEntry ve = a.voxels[k];
+    const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
+    const Ray ray = make_ray(pg.dx, pg.dy, (double)ix + 0.5, (double)iy + 0.5, a.nx, a.ny,
+                             a.ray_step);
+    Kahan s1, s2;
+    for (int q = 0; q < ray.n; q++) {
+        double px, py;
+        ray_sample(ray, a.ray_step, q, px, py);
+        const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
+        s1.add(tv.x);
+        s2.add(tv.y);
+    }
+    VoxShared vs;
+    voxel_geometry(a, pg, ix, iy, vs);
+    const size_t ri = ((size_t)vb * a.n_det + p) * a.nvox + k;
+    a.pv_s[ri] = vs.s;
+    a.pv_cth[ri] = vs.cth;
+    a.pv_geo[ri] = make_float4((float)vs.As, (float)vs.Bs, (float)vs.gR, (float)vs.gp);
+    a.pv_leg[ri] = make_float2((float)((double)s1.value() * a.ray_step),
+                               (float)((double)s2.value() * a.ray_step));
+}
+
+hipError_t launch_pixel_voxel(const ForwardArgs &a, hipStream_t s) {
+    if (a.nvox <= 0) return hipSuccess;
+    dim3 grid((a.nvox + 255) / 256, a.n_det, a.nvb);
+    hipLaunchKernelGGL(pixel_voxel_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 // Block of G*TQ lanes: lane (tq, h) owns coarse bin tq, fine bins tq*FS..tq*FS+FS-1
 // and the source energies e = h + G*i, i < EPT.
+// Four waves per SIMD: two 512-lane workgroups share a CU, so one can trace,
+// build its tables or wait at a barrier while the other evaluates Gaussians.
 template <int TQ, int G, int EPT, int FS, bool WITH_A>
-__global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) {
+__global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) void forward_kernel(
+    ForwardArgs a, int kc) {
     constexpr int NT = G * TQ;
     extern __shared__ __align__(16) unsigned char smem[];
     const int t = threadIdx.x;
     const int tq = t % TQ, h = t / TQ;
     const int V = a.n_views, E = a.n_energy, Q = a.n_q, QF = a.n_qf, M = a.n_mat;
-    const int v = blockIdx.x % V, p = blockIdx.x / V;
-    const int col = p % a.n_cols, row = p / a.n_cols;
+    const int vb = blockIdx.x % a.nvb, p = blockIdx.x / a.nvb;  // view within the batch, pixel
+    const int v = a.v0 + vb;
     const int npix = a.nx * a.ny;
-    const int nsplit = NT / kc;
     const int Gc = NT / E;  // Compton lane groups
 
     // ---- LDS carve-up
     VoxShared *vox = reinterpret_cast<VoxShared *>(smem);
     unsigned char *r2 = smem + ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
-    float4 *part = reinterpret_cast<float4 *>(r2);                 // [kc][nsplit]
-    double2 *teo = reinterpret_cast<double2 *>(part + NT);           // [kc][E] Compton E_out bin
+    double2 *teo = reinterpret_cast<double2 *>(r2);                  // [kc][E] Compton E_out bin
     float4 *tw = reinterpret_cast<float4 *>(teo + kc * E);           // [kc][E]
     float *tb = reinterpret_cast<float *>(tw + kc * E);              // [kc][E] band
     float *tk = tb + kc * E;                                         // [kc][E] Compton K / width
@@ -316,30 +378,6 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
     float *rfc = tk + kc * E;     // [E][nwin] spectrum window of each detector bin
     float *hst = rfc + E * nwin;  // [E][Q] per-material coarse totals (A only)
     double *red = reinterpret_cast<double *>(r2);                    // final reductions (aliases)
-
-    // ---- per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64, in LDS
-    __shared__ PixelGeom pg_s;
-    if (t == 0) {
-#pragma clang fp contract(off)
-        PixelGeom pg;
-        const ViewConst vc = a.views[v];
-        pg.sp = vc.sp;
-        pg.cp = vc.cp;
-        pg.sx = vc.src_x;
-        pg.sy = vc.src_y;
-        const double rdet = ((double)col - (double)a.n_cols / 2.0 + 0.5) * a.col_pitch;
-        pg.dx = a.center_x - a.r_det * vc.sp + rdet * vc.cp;
-        pg.dy = a.center_y + a.r_det * vc.cp + rdet * vc.sp;
-        pg.dz = a.det_z0 + (double)row * a.row_pitch;
-        pg.ndx = vc.sp;
-        pg.ndy = -vc.cp;
-        pg.nsx = -vc.sp * a.sta;
-        pg.nsy = vc.cp * a.sta;
-        pg.nsz = -a.cta;
-        pg_s = pg;
-    }
-    __syncthreads();
-    const PixelGeom &pg = pg_s;
 
     // ---- per-lane q constants
     float qch = 0.f, qcl = 0.f, dqc = 0.f;
@@ -427,70 +465,33 @@ __global__ __launch_bounds__(G * TQ) void forward_kernel(ForwardArgs a, int kc) 
         }
         for (int k0 = lo; k0 < hi; k0 += kc) {
             const int nk = min(kc, hi - k0);
-            // ---- phase 1: detector legs (split over nsplit lanes per voxel) + geometry
-            {
-                const int j = t % kc, r = t / kc;
-                if (j < nk) {
-                    const VoxelEntry ve = a.voxels[k0 + j];
-                    const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
-                    const Ray ray = make_ray(pg.dx, pg.dy, (double)ix + 0.5, (double)iy + 0.5,
-                                             a.nx, a.ny, a.ray_step);
-                    // two samples per step so that both samples' texel loads are in
-                    // flight together; the Kahan sums still add samples in order
-                    Kahan s1, s2;
-                    int q = r;
-                    for (; q + nsplit < ray.n; q += 2 * nsplit) {
-                        double px0, py0, px1, py1;
-                        ray_sample(ray, a.ray_step, q, px0, py0);
-                        ray_sample(ray, a.ray_step, q + nsplit, px1, py1);
-                        const float2 tv0 = bilinear2(a.mu_padded, a.nx, a.ny, px0, py0);
-                        const float2 tv1 = bilinear2(a.mu_padded, a.nx, a.ny, px1, py1);
-                        s1.add(tv0.x);
-                        s2.add(tv0.y);
-                        s1.add(tv1.x);
-                        s2.add(tv1.y);
-                    }
-                    if (q < ray.n) {
-                        double px, py;
-                        ray_sample(ray, a.ray_step, q, px, py);
-                        const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
-                        s1.add(tv.x);
-                        s2.add(tv.y);
-                    }
-                    part[j * nsplit + r] = make_float4(s1.s, s1.c, s2.s, s2.c);
-                    if (r == 0) {
-                        VoxShared vs;
-                        voxel_geometry(a, pg, ix, iy, vs);
-                        vs.a2c = (double)ve.a2c;
-                        vs.na = lut_index((log2(fabs((double)ve.a1c / (double)ve.a2c)) -
-                                           a.lut_a_min) / a.da_lut,
-                                          a.lut_na);
-                        const float2 ls = a.src_int[(size_t)v * npix + (k0 + j)];
-                        vs.lin0 = ls.x;  // source leg; swapped below if requested
-                        vs.lin1 = ls.y;
-                        vox[j] = vs;
-                    }
-                }
-            }
-            __syncthreads();
+            // ---- the chunk's (view, pixel, voxel) records (pixel_voxel_kernel)
             if (t < nk) {
-                double d1 = 0.0, d2 = 0.0;
-                for (int r = 0; r < nsplit; r++) {
-                    const float4 pv = part[t * nsplit + r];
-                    d1 += (double)(pv.x - pv.y);
-                    d2 += (double)(pv.z - pv.w);
-                }
-                d1 *= a.ray_step;
-                d2 *= a.ray_step;
-                VoxShared &vs = vox[t];
-                const double s1 = vs.lin0, s2 = vs.lin1;
+                const int k = k0 + t;
+                const size_t ri = ((size_t)vb * a.n_det + p) * a.nvox + k;
+                const VoxelEntry ve = a.voxels[k];
+                const float4 gg = a.pv_geo[ri];
+                const float2 ld = a.pv_leg[ri];
+                const float2 ls = a.src_int[(size_t)v * npix + k];
+                VoxShared vs;
+                vs.s = a.pv_s[ri];
+                vs.cth = a.pv_cth[ri];
+                vs.As = gg.x;
+                vs.Bs = gg.y;
+                vs.gR = gg.z;
+                vs.gp = gg.w;
+                vs.a2c = (double)ve.a2c;
+                vs.na = lut_index((log2(fabs((double)ve.a1c / (double)ve.a2c)) - a.lut_a_min) /
+                                      a.da_lut,
+                                  a.lut_na);
                 // integrate_diffraction.cu:259-260 passes the detector leg where the
                 // source leg belongs; reproduced only under XRD_QUIRK_SWAPPED_LEGS.
                 if (swap) {
-                    vs.lin0 = d1; vs.lin1 = d2; vs.lout0 = s1; vs.lout1 = s2;
+                    vs.lin0 = ld.x; vs.lin1 = ld.y; vs.lout0 = ls.x; vs.lout1 = ls.y;
                 } else {
-                    vs.lin0 = s1; vs.lin1 = s2; vs.lout0 = d1; vs.lout1 = d2;
+                    vs.lin0 = ls.x; vs.lin1 = ls.y; vs.lout0 = ld.x; vs.lout1 = ld.y;
                 }
+                vox[t] = vs;
             }
             __syncthreads();
             // ---- phase 2: per-(voxel, source energy) table, FP64
@@ -659,7 +660,8 @@ static hipError_t launch_fwd_t(const ForwardArgs &a, hipStream_t s) {
     const int E = a.n_energy;
     const int Gc = NT / E;
     const size_t vox_bytes = ((kc * sizeof(VoxShared) + 15) & ~size_t(15));
-    size_t r2 = (size_t)NT * 16 + (size_t)kc * E * (16 + 16 + 4 + 4) +
+    // teo (16) | tw (16) | tb (4) | tk (4) per (voxel, energy) | rfc | hst
+    size_t r2 = (size_t)kc * E * (16 + 16 + 4 + 4) +
                 (size_t)E * (2 * a.dn + 1) * 4 + (WITH_A ? (size_t)E * a.n_q * 4 : 0);
     const size_t redb = ((size_t)(NT / kWave) * EPT + E + (size_t)Gc * E) * 8;
     r2 = r2 > redb ? r2 : redb;
@@ -668,7 +670,7 @@ static hipError_t launch_fwd_t(const ForwardArgs &a, hipStream_t s) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
     if (e != hipSuccess) return e;
-    const unsigned nblk = (unsigned)a.n_views * (unsigned)a.n_det;
+    const unsigned nblk = (unsigned)a.nvb * (unsigned)a.n_det;
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(NT), shmem, s, a, kc);
     return hipGetLastError();
 }

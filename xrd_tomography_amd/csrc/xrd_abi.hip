@@ -65,7 +65,7 @@ struct DevBuf {
 struct xrd_context {
     int device = 0;
     // forward workspace
-    DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables;
+    DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables, pv;
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
@@ -73,6 +73,8 @@ struct xrd_context {
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // pinned word for small device-to-host reads (the non-air voxel count)
+    int *pinned_word = nullptr;
     hipEvent_t staged = nullptr;
     bool staged_pending = false;
 };
@@ -188,10 +190,11 @@ int xrd_context_destroy(xrd_context *c) {
     (void)hipSetDevice(c->device);
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
-                      &c->src_int, &c->tables, &c->partial, &c->small, &c->cm_maps, &c->cm_src,
-                      &c->cm_vlist, &c->cm_h})
+                      &c->src_int, &c->tables, &c->pv, &c->partial, &c->small, &c->cm_maps,
+                      &c->cm_src, &c->cm_vlist, &c->cm_h})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->pinned_word) (void)hipHostFree(c->pinned_word);
     if (c->staged) (void)hipEventDestroy(c->staged);
     delete c;
     return XRD_OK;
@@ -201,8 +204,8 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     if (!c) return 0;
     int64_t s = 0;
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
-                            &c->offsets, &c->src_int, &c->tables, &c->partial, &c->small,
-                            &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h})
+                            &c->offsets, &c->src_int, &c->tables, &c->pv, &c->partial,
+                            &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -439,10 +442,34 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
     fa.out_c = out_compton;
     fa.out_a = out_A;
     HIP_TRY(launch_source_integrals(fa, (float2 *)ctx->src_int.ptr, s), "source-leg kernel");
-    hipError_t e = launch_forward(fa, s);
-    if (e == hipErrorInvalidValue)
-        return fail(XRD_ERR_UNSUPPORTED, "%s: no kernel instantiation for these sizes", fn);
-    HIP_TRY(e, "forward kernel");
+    // the non-air voxel count sizes the per-(view, pixel, voxel) records: views are
+    // processed in batches whose records fit in kPixelVoxelBudget bytes
+    if (!ctx->pinned_word)
+        HIP_TRY(hipHostMalloc((void **)&ctx->pinned_word, 64, hipHostMallocDefault),
+                "pinned alloc");
+    HIP_TRY(hipMemcpyAsync(ctx->pinned_word, (const int *)ctx->offsets.ptr + d->n_mat, sizeof(int),
+                           hipMemcpyDeviceToHost, s),
+            "voxel count copy");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const int nvox = *ctx->pinned_word;
+    const size_t per_view = (size_t)fa.n_det * (size_t)(nvox > 0 ? nvox : 1) * kPixelVoxelRecordBytes;
+    int nvb = (int)std::min<size_t>((size_t)V, std::max<size_t>(1, kPixelVoxelBudget / per_view));
+    const size_t nrec = (size_t)nvb * fa.n_det * (size_t)(nvox > 0 ? nvox : 1);
+    HIP_TRY(ctx->pv.ensure(nrec * kPixelVoxelRecordBytes), "workspace alloc");
+    fa.nvox = nvox;
+    fa.pv_s = (double *)ctx->pv.ptr;
+    fa.pv_cth = fa.pv_s + nrec;
+    fa.pv_geo = (float4 *)(fa.pv_cth + nrec);
+    fa.pv_leg = (float2 *)(fa.pv_geo + nrec);
+    for (int v0 = 0; v0 < V; v0 += nvb) {
+        fa.v0 = v0;
+        fa.nvb = std::min(nvb, V - v0);
+        HIP_TRY(launch_pixel_voxel(fa, s), "pixel-voxel kernel");
+        hipError_t e = launch_forward(fa, s);
+        if (e == hipErrorInvalidValue)
+            return fail(XRD_ERR_UNSUPPORTED, "%s: no kernel instantiation for these sizes", fn);
+        HIP_TRY(e, "forward kernel");
+    }
     return XRD_OK;
 }
 

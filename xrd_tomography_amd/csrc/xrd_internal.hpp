@@ -61,6 +61,12 @@ struct ForwardArgs {
     const double *q_coarse;     // [2*Q]: centres then widths (FP64)
     // outputs
     float *out_n, *out_c, *out_a;
+    // per-(view, pixel, voxel) records of the current batch of views [v0, v0 + nvb)
+    // (pixel_voxel_kernel), index (vb * n_det + pixel) * nvox + voxel
+    int v0, nvb, nvox;
+    double *pv_s, *pv_cth;  // sin(theta/2), cos(theta)
+    float4 *pv_geo;         // resolution terms As, Bs and geometry weights gR, gp
+    float2 *pv_leg;         // detector-leg integrals (a1, a2)
 };
 
 struct PrepArgs {
@@ -80,6 +86,11 @@ struct PrepArgs {
 
 hipError_t launch_prepare(const PrepArgs &a, hipStream_t s);
 hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStream_t s);
+hipError_t launch_pixel_voxel(const ForwardArgs &a, hipStream_t s);
+// Bytes of one (view, pixel, voxel) record, and the workspace the records of one
+// batch of views may take.
+constexpr size_t kPixelVoxelRecordBytes = 2 * sizeof(double) + sizeof(float4) + sizeof(float2);
+constexpr size_t kPixelVoxelBudget = size_t(4) << 30;
 // Returns hipErrorInvalidValue when no kernel instantiation fits the sizes.
 hipError_t launch_forward(const ForwardArgs &a, hipStream_t s);
 hipError_t launch_trace_rays(int nx, int ny, double step, int64_t n, const double *seg,
