@@ -15,12 +15,49 @@
 // respect to the coefficients are accumulated per voxel in registers
 // (deterministic); gradients with respect to the maps are scattered along the
 // rays with FP64 atomics (order-dependent only at the 1e-16 level).
+#include <climits>
+
 #include "xrd_device.hpp"
 #include "xrd_internal.hpp"
 
 namespace xrd {
 
 namespace {
+
+template <int CTRL> __device__ __forceinline__ float dpp_get(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Sums 16 values per lane over the wave so that lane l ends up with the wave total
+// of value (l >> 2) & 15 (pairs of lanes differing in one bit exchange halves:
+// bits 5, 4 by shuffles, 3 by row_mirror, 2 by row_half_mirror; bits 1 and 0 add
+// the remaining value).
+__device__ __forceinline__ float transpose_sum16(const float *v, int lane) {
+    float w[8];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float keep = b5 ? v[8 + k] : v[k], send = b5 ? v[k] : v[8 + k];
+        w[k] = keep + __shfl_xor(send, 32, kWave);
+    }
+    float u[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float keep = b4 ? w[4 + k] : w[k], send = b4 ? w[k] : w[4 + k];
+        u[k] = keep + __shfl_xor(send, 16, kWave);
+    }
+    float z[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float keep = b3 ? u[2 + k] : u[k], send = b3 ? u[k] : u[2 + k];
+        z[k] = keep + dpp_get<0x140>(send);
+    }
+    const float keep = b2 ? z[1] : z[0], send = b2 ? z[0] : z[1];
+    float r = keep + dpp_get<0x141>(send);
+    r += dpp_get<0x4E>(r);
+    r += dpp_get<0xB1>(r);
+    return r;
+}
 
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kRs2pi = 0.39894228040143267794;
@@ -359,51 +396,90 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
     }
 }
 
-// Scatter gamma[b] * step * (bilinear weights) of every sample of a ray into grad[b][pix].
+// Adds gamma[b] * step * (bilinear weight) of every sample q in [q0, q1) of a ray
+// to grad[b][texel] (the transpose of the line integral, attenuator.cu:52-58).
+// Consecutive samples in the same texel cell are merged first, so each cell the
+// ray crosses costs one FP64 atomic add per corner and map; texels outside the
+// map (the zero border) receive nothing.
 __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, const double *gam,
-                                            int r, int sp) {
+                                            int q0, int q1) {
     const int nx = a.f.nx, ny = a.f.ny, B = a.n_basis;
     const size_t npix = (size_t)nx * ny;
-    for (int q = r; q < ray.n; q += sp) {
+    int ci = INT_MIN, cj = INT_MIN;
+    double w00 = 0.0, w10 = 0.0, w01 = 0.0, w11 = 0.0;
+    auto flush = [&]() {
+        if (ci == INT_MIN) return;
+        const int ii[4] = {ci, ci + 1, ci, ci + 1}, jj[4] = {cj, cj, cj + 1, cj + 1};
+        const double ww[4] = {w00, w10, w01, w11};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (ii[c] < 0 || jj[c] < 0 || ii[c] >= nx || jj[c] >= ny) continue;
+            const size_t pix = (size_t)ii[c] + (size_t)nx * jj[c];
+            const double ws = ww[c] * a.f.ray_step;
+            for (int b = 0; b < B; b++)
+                if (gam[b] != 0.0) unsafeAtomicAdd(a.grad_maps + (size_t)b * npix + pix, gam[b] * ws);
+        }
+    };
+    for (int q = q0; q < q1; q++) {
         double px, py;
         ray_sample(ray, a.f.ray_step, q, px, py);
         const double xt = px - 0.5, yt = py - 0.5;
         const double fx = floor(xt), fy = floor(yt);
         const double ax = xt - fx, ay = yt - fy;
         const int i0 = (int)fx, j0 = (int)fy;
-        const int ci[4] = {i0, i0 + 1, i0, i0 + 1}, cj[4] = {j0, j0, j0 + 1, j0 + 1};
-        const double wgt[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (ci[c] < 0 || cj[c] < 0 || ci[c] >= nx || cj[c] >= ny) continue;
-            const size_t pix = (size_t)ci[c] + (size_t)nx * cj[c];
-            const double ws = wgt[c] * a.f.ray_step;
-            for (int b = 0; b < B; b++)
-                if (gam[b] != 0.0) unsafeAtomicAdd(a.grad_maps + (size_t)b * npix + pix, gam[b] * ws);
+        if (i0 != ci || j0 != cj) {
+            flush();
+            ci = i0;
+            cj = j0;
+            w00 = w10 = w01 = w11 = 0.0;
         }
+        w00 += (1 - ax) * (1 - ay);
+        w10 += ax * (1 - ay);
+        w01 += (1 - ax) * ay;
+        w11 += ax * ay;
     }
+    flush();
 }
 
-// VJP: one workgroup per chunk of KC voxels, looping over every (view, pixel).
-__global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a) {
+// Lane r of sp lanes that share a ray takes a contiguous run of its samples.
+__device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1) {
+    const int per = (n + sp - 1) / sp;
+    q0 = min(n, r * per);
+    q1 = min(n, q0 + per);
+}
+
+// VJP: one workgroup per (chunk of KC voxels, group of views), looping over the
+// views of the group and every pixel.  Per pixel it recomputes the detector legs
+// and the forward terms of the chunk, and
+//   * adds W(e) P(e) G_e(q_n) dq_n to the lane's FP64 totals h_vox(q_n) (the
+//     coefficient gradient is basis . h), written per view group to hpart;
+//   * when map gradients are wanted, forms S_e = sum_n G_e(q_n) dq_n F_vox(q_n)
+//     from the same Gaussians (16-value transposed wave sums), the ray weights
+//     gamma_b = -sum_e beta[b,e] W(e) P(e) S_e, and scatters them along the
+//     detector legs; the source legs carry the per-view sums of gamma.
+__global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
     __shared__ VoxC vox[KC];
     __shared__ float2 part[NT];
     __shared__ float Ld[KC * MAXB];
     __shared__ float Fq[KC * TQ];
     __shared__ float4 tw[KC * 64];
     __shared__ float tb[KC * 64];
-    __shared__ double wps[KC * 64];
+    __shared__ double pw[KC * 64];        // W(e) P(e)
+    __shared__ float sp_[NT / kWave][KC][16];  // per-wave partial S
     __shared__ double gam[KC * MAXB];
     __shared__ double gsrc[KC * MAXB];
     __shared__ double wt[64];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
+    const int lane = t & (kWave - 1), wid = t >> 6;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q, B = a.n_basis, P = f.n_det;
     const size_t npix = (size_t)f.nx * f.ny;
     const int nv = a.nvox[0];
     const int k0 = blockIdx.x * KC;
     if (k0 >= nv) return;  // uniform per block
     const int nk = min(KC, nv - k0);
+    const int va = blockIdx.y * views_per_group, vb = min(V, va + views_per_group);
+    const bool maps = a.grad_maps != nullptr;
     float qch = 1e30f, qcl = 0.f, dqc = 0.f;
     if (tq < Q) {
         const double q = f.q_coarse[tq];
@@ -420,20 +496,18 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a) {
     // coefficients of the chunk are fixed: F_vox(q_n) once
     for (int idx = t; idx < nk * TQ; idx += NT) {
         const int j = idx / TQ, n = idx % TQ;
-        float s = 0.f;
+        float sv = 0.f;
         if (n < Q) {
             const int pix = a.vlist[k0 + j];
             const float *cf = a.coef + (size_t)pix * a.n_coef;
-            for (int kk = 0; kk < a.n_coef; kk++) s = fmaf(cf[kk], a.basis[kk * Q + n], s);
+            for (int kk = 0; kk < a.n_coef; kk++) sv = fmaf(cf[kk], a.basis[kk * Q + n], sv);
         }
-        Fq[j * TQ + n] = s;
+        Fq[j * TQ + n] = sv;
     }
     double ht[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) ht[j] = 0.0;
-    const int nbc = (B + 1) / 2, sp = NT / (KC * nbc);
-    (void)nbc;
-    for (int v = 0; v < V; v++) {
+    for (int v = va; v < vb; v++) {
         for (int idx = t; idx < KC * MAXB; idx += NT) gsrc[idx] = 0.0;
         for (int p = 0; p < P; p++) {
             const PixG pg = pixel_setup(a, v, p);
@@ -448,20 +522,19 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a) {
             }
             if (t < E) {
                 // transpose of the detector-response window
-                double s = 0.0;
+                double sw = 0.0;
                 for (int ed = 0; ed < E; ed++) {
                     int wlo, whi;
                     src_window(ed, E, f.dn, wlo, whi);
                     if (t >= wlo && t <= whi)
-                        s += (double)f.spectrum[t + (size_t)E * ed] *
-                             (double)a.w[(size_t)v + (size_t)V * ((size_t)p + (size_t)P * ed)];
+                        sw += (double)f.spectrum[t + (size_t)E * ed] *
+                              (double)a.w[(size_t)v + (size_t)V * ((size_t)p + (size_t)P * ed)];
                 }
-                wt[t] = s;
+                wt[t] = sw;
             }
             __syncthreads();
             trace_chunk(a, pg, vox, nk, part, Ld);
             __syncthreads();
-            // table + S_e per (voxel, energy)
             for (int idx = t; idx < nk * E; idx += NT) {
                 const int j = idx / E, e = idx % E;
                 float4 tv;
@@ -469,88 +542,121 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a) {
                 double Pe;
                 table_entry(a, vox[j], a.src_ints + ((size_t)v * npix + vox[j].k) * a.n_basis,
                             Ld + j * MAXB, e, tv, band, Pe);
-                float s = 0.f;
-                for (int n = 0; n < Q; n++) {
-                    const double q = f.q_coarse[n];
-                    const float qh = (float)q, ql = (float)(q - (double)qh);
-                    const float d = (tv.x - qh) + (tv.y - ql);
-                    s = fmaf((float)f.q_coarse[Q + n] * Fq[j * TQ + n],
-                             __builtin_amdgcn_exp2f(d * d * tv.z), s);
-                }
-                wps[j * 64 + e] = wt[e] * Pe * (double)s;
-                tv.w = (float)(wt[e] * Pe);  // phase 3 weight
+                pw[j * 64 + e] = wt[e] * Pe;
+                tv.w = (float)(wt[e] * Pe);
                 tw[j * 64 + e] = tv;
                 tb[j * 64 + e] = band;
             }
             __syncthreads();
-            // gamma_b = -sum_e beta[b,e] W(e) P(e) S(e)
-            for (int idx = t; idx < nk * B; idx += NT) {
-                const int j = idx / B, b = idx % B;
-                double g = 0.0;
-                for (int e = 0; e < E; e++) g -= (double)a.beta[b * E + e] * wps[j * 64 + e];
-                gam[j * MAXB + b] = g;
-                gsrc[j * MAXB + b] += g;
-            }
-            // phase 3: h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n
+            // h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n; S_e partials over this wave's bins
 #pragma unroll
             for (int j = 0; j < KC; j++) {
                 if (j >= nk) break;
+                const float fq = Fq[j * TQ + (tq < Q ? tq : 0)] * dqc;
                 float hp = 0.f;
-                for (int i = 0; i < EPT; i++) {
+                float sv[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    sv[i] = 0.f;
                     const int e = h + G * i;
-                    if (e >= E) break;
+                    if (i >= EPT || e >= E) continue;
                     const float4 tv = tw[j * 64 + e];
                     const float band = tb[j * 64 + e];
                     if (tv.x + band >= lo && tv.x - band <= hi) {
                         const float d = (tv.x - qch) + (tv.y - qcl);
-                        hp = fmaf(tv.w, __builtin_amdgcn_exp2f(d * d * tv.z), hp);
+                        const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
+                        hp = fmaf(tv.w, g, hp);
+                        sv[i] = g * fq;
                     }
                 }
                 ht[j] += (double)(hp * dqc);
+                if (maps) {
+                    const float st = transpose_sum16(sv, lane);
+                    if ((lane & 3) == 0) sp_[wid][j][(lane >> 2) & 15] = st;
+                }
             }
             __syncthreads();
-            // scatter gamma along the detector legs
-            if (a.grad_maps) {
+            if (maps) {
+                // gamma_b = -sum_e beta[b,e] W(e) P(e) S_e; group h's waves are
+                // h*(TQ/64) .. h*(TQ/64) + TQ/64 - 1
+                for (int idx = t; idx < nk * B; idx += NT) {
+                    const int j = idx / B, b = idx % B;
+                    double g = 0.0;
+                    for (int e = 0; e < E; e++) {
+                        const int hh = e % G, i = e / G;
+                        double se = 0.0;
+                        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++)
+                            se += (double)sp_[w][j][i];
+                        g -= (double)a.beta[b * E + e] * pw[j * 64 + e] * se;
+                    }
+                    gam[j * MAXB + b] = g;
+                    gsrc[j * MAXB + b] += g;
+                }
+                __syncthreads();
+                // scatter gamma along the detector legs
                 const int j = t % KC, r = t / KC;
                 if (j < nk) {
                     const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
                                              (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
-                    scatter_ray(a, ray, gam + j * MAXB, r, NT / KC);
+                    int q0, q1;
+                    lane_run(ray.n, r, NT / KC, q0, q1);
+                    scatter_ray(a, ray, gam + j * MAXB, q0, q1);
                 }
             }
             __syncthreads();
         }
         // source legs of view v carry the sum of gamma over the pixels
-        if (a.grad_maps) {
+        if (maps) {
             const int j = t % KC, r = t / KC;
             if (j < nk) {
                 const ViewConst vc = f.views[v];
                 const int pix = a.vlist[k0 + j];
                 const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
                                          (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
-                scatter_ray(a, ray, gsrc + j * MAXB, r, NT / KC);
+                int q0, q1;
+                lane_run(ray.n, r, NT / KC, q0, q1);
+                scatter_ray(a, ray, gsrc + j * MAXB, q0, q1);
             }
         }
         __syncthreads();
     }
-    (void)sp;
-    // reduce h over the G energy groups (through a per-block global scratch), then
-    // contract with the basis: grad_coef[vox,k] = sum_n basis[k,n] h_vox(q_n)
-    for (int j = 0; j < nk; j++)
-        a.hscratch[((size_t)blockIdx.x * G + h) * KC * TQ + (size_t)j * TQ + tq] = ht[j];
-    __syncthreads();
-    if (a.grad_coef) {
-        for (int idx = t; idx < nk * a.n_coef; idx += NT) {
-            const int j = idx / a.n_coef, kk = idx % a.n_coef;
-            double s = 0.0;
-            for (int n = 0; n < Q; n++) {
-                double hv = 0.0;
-                for (int hh = 0; hh < G; hh++)
-                    hv += a.hscratch[((size_t)blockIdx.x * G + hh) * KC * TQ + (size_t)j * TQ + n];
-                s += (double)a.basis[kk * Q + n] * hv;
+    // this view group's h: add the G energy groups of each bin in order (the table
+    // tw is free now and holds KC*TQ doubles), then store; coeff_grad_kernel adds
+    // the view groups and contracts with the basis
+    static_assert(sizeof(tw) >= sizeof(double) * KC * TQ, "LDS reuse");
+    double *hred = reinterpret_cast<double *>(tw);
+    for (int hh = 0; hh < G; hh++) {
+        if (h == hh) {
+#pragma unroll
+            for (int j = 0; j < KC; j++) {
+                if (j < nk) {
+                    double *slot = &hred[j * TQ + tq];
+                    *slot = (hh == 0 ? 0.0 : *slot) + ht[j];
+                }
             }
-            a.grad_coef[(size_t)a.vlist[k0 + j] * a.n_coef + kk] = s;
         }
+        __syncthreads();
+    }
+    if (a.grad_coef)
+        for (int idx = t; idx < nk * TQ; idx += NT)
+            a.hpart[((size_t)blockIdx.y * a.nvox_cap + k0) * TQ + idx] = hred[idx];
+}
+
+// One workgroup of TQ lanes per voxel: h(q_n) = sum of the view groups' partials in
+// group order, then grad_coef[vox, k] = sum_n basis[k, n] h(q_n).
+__global__ __launch_bounds__(TQ) void coeff_grad_kernel(CoeffArgs a, int n_groups) {
+    __shared__ double hv[TQ];
+    const int k = blockIdx.x, n = threadIdx.x;
+    if (k >= a.nvox[0]) return;  // uniform per block
+    const int Q = a.f.n_q;
+    double s = 0.0;
+    for (int g = 0; g < n_groups; g++) s += a.hpart[((size_t)g * a.nvox_cap + k) * TQ + n];
+    hv[n] = n < Q ? s : 0.0;
+    __syncthreads();
+    if (n < a.n_coef) {
+        double r = 0.0;
+        for (int i = 0; i < Q; i++) r += (double)a.basis[n * Q + i] * hv[i];
+        a.grad_coef[(size_t)a.vlist[k] * a.n_coef + n] = r;
     }
 }
 
@@ -609,14 +715,29 @@ hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-int coeff_vjp_blocks(int npix) { return (npix + KC - 1) / KC; }
-size_t coeff_hscratch_doubles(int npix) {
-    return (size_t)coeff_vjp_blocks(npix) * G * KC * TQ;
+// View groups of the VJP grid: up to 32 groups of consecutive views, fewer when
+// the per-group FP64 partials of h would pass 4 GiB.
+int coeff_vjp_groups(int n_views, int npix) {
+    const size_t per_group = (size_t)npix * TQ * sizeof(double);
+    size_t ng = ((size_t)4 << 30) / per_group;
+    if (ng > 32) ng = 32;
+    if (ng > (size_t)n_views) ng = (size_t)n_views;
+    return ng < 1 ? 1 : (int)ng;
+}
+
+size_t coeff_hpart_doubles(int n_views, int npix) {
+    return (size_t)coeff_vjp_groups(n_views, npix) * (size_t)npix * TQ;
 }
 
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
-    const int nb = coeff_vjp_blocks(a.f.nx * a.f.ny);
-    hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb), dim3(NT), 0, s, a);
+    const int ng = coeff_vjp_groups(a.f.n_views, a.nvox_cap);
+    const int vpg = (a.f.n_views + ng - 1) / ng;
+    const int nb = (a.nvox_cap + KC - 1) / KC;
+    if (nb > 0) {
+        hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb, ng), dim3(NT), 0, s, a, vpg);
+        if (a.grad_coef)
+            hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
+    }
     return hipGetLastError();
 }
 

@@ -873,7 +873,6 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     HIP_TRY(ctx->cm_maps.ensure(npad * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_src.ensure((size_t)g->n_views * npix * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_vlist.ensure((npix + 4) * sizeof(int)), "workspace alloc");
-    HIP_TRY(ctx->cm_h.ensure(coeff_hscratch_doubles((int)npix) * sizeof(double)), "workspace alloc");
     ForwardArgs fa;
     rc = setup_forward_common(ctx, g, d, spectrum, fa, s);
     if (rc) return rc;
@@ -890,7 +889,7 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     ca.src_ints = (float *)ctx->cm_src.ptr;
     ca.vlist = (int *)ctx->cm_vlist.ptr;
     ca.nvox = (int *)ctx->cm_vlist.ptr + npix;
-    ca.hscratch = (double *)ctx->cm_h.ptr;
+    ca.nvox_cap = (int)npix;
     HIP_TRY(launch_coeff_prepare(ca, s), "coefficient-model preparation");
     return XRD_OK;
 }
@@ -936,7 +935,12 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
     ca.w = grad_counts;
     ca.grad_coef = grad_coef;
     ca.grad_maps = grad_maps;
-    if (grad_coef) HIP_TRY(launch_zero_f64(grad_coef, npix * n_coef, s), "zero kernel");
+    if (grad_coef) {
+        HIP_TRY(ctx->cm_h.ensure(coeff_hpart_doubles(g->n_views, (int)npix) * sizeof(double)),
+                "workspace alloc");
+        ca.hpart = (double *)ctx->cm_h.ptr;
+        HIP_TRY(launch_zero_f64(grad_coef, npix * n_coef, s), "zero kernel");
+    }
     if (grad_maps) HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
     return XRD_OK;

@@ -116,13 +116,14 @@ struct CoeffArgs {
     float *out_counts;       // [rows]
     double *grad_coef;       // [nx*ny x K] or NULL
     double *grad_maps;       // [B x nx*ny] or NULL (zeroed by the caller)
-    double *hscratch;        // coeff_hscratch_doubles(nx*ny)
+    int nvox_cap;            // nx*ny: bound of the voxel list
+    double *hpart;           // coeff_hpart_doubles(V, nx*ny) when grad_coef is set
 };
 
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
-size_t coeff_hscratch_doubles(int npix);
+size_t coeff_hpart_doubles(int n_views, int npix);
 hipError_t launch_zero_f64(double *p, size_t n, hipStream_t s);
 hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, const float *y,
                               float *grad, double *partial, int nblocks, hipStream_t s);
