@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="repeat the CPU-baseline update until this much time has passed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--vjp-views", type=int, default=360,
+                    help="views of the vjp256 workload (the coefficient-model VJP)")
+    ap.add_argument("--vjp-basis", default="full", choices=["full", "2"],
+                    help="vjp256 attenuation: the full 61-energy map or the two-basis form")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device-to-device copy 10 times (calibrates HBM "
                          "byte counters: 2 GiB read + 2 GiB written per launch)")
@@ -88,6 +92,9 @@ def main():
         return
     if args.workload == "batch48":
         run_batch(args, dev)
+        return
+    if args.workload == "vjp256":
+        run_vjp(args, dev)
         return
     prob = make_problem(args.workload)
     g, d = prob.geometry, prob.disc
@@ -379,6 +386,87 @@ def run_batch(args, dev):
                                                 "speedup": round(ms_base / ms_it, 3)},
         "rel_err_vs_fp64": worst, "rel_err_sample": "4 smallest phantoms, 30 updates",
         "a_build_s": round(build_s, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_vjp(args, dev):
+    """Coefficient model on the 256x256-voxel phantom (16 coefficients per voxel on a
+    Gaussian-bump basis, attenuation as the 256^2 x 61 mu(E) map): one forward, the
+    Poisson NLL, the VJP with respect to the coefficients alone and with respect to
+    coefficients and maps.  Rate = detector rays (view, pixel, voxel) per second; each
+    ray's line integrals are recomputed.  Full-size check: <J x, u> = <x, J' u> (the
+    counts are linear in the coefficients)."""
+    import numpy as np
+    import torch
+    from xrd_tomography_amd import api
+    from xrd_tomography_amd.synthetic import make_coeff_problem, make_problem
+    from oracle import oracle as O
+
+    full = args.vjp_basis == "full"
+    prob = make_problem("fan256", n_views=args.vjp_views)
+    cp = make_coeff_problem(prob, K=16, full_mu=full)
+    g, d = prob.geometry, prob.disc
+    nvox = int((prob.segmentation > 0).sum())
+    rays = g.n_views * g.n_cols * g.n_rows * nvox
+    sp, seg, coef, basis, maps, beta = api.coeff_problem_tensors(cp)
+    kw = dict(geometry=g, disc=d)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0.record()
+        out = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return out, e0.elapsed_time(e1) / 1000.0
+
+    log(f"[vjp] fan256 {g.n_views} views, {nvox} voxels, B={cp.B}: {rays} detector rays")
+    y, t_fwd = timed(lambda: api.forward_coeffs(sp, seg, coef, basis, maps, beta, **kw))
+    log(f"[vjp] forward {t_fwd:.2f} s")
+    yobs = torch.poisson(y.double() + 0.05 * float(y.mean())).float().reshape(-1)
+    (loss, w), t_nll = timed(lambda: api.poisson_nll(y.reshape(-1).contiguous(), yobs))
+    (gc, _), t_coef = timed(lambda: api.vjp_coeffs(sp, seg, coef, basis, maps, beta, w,
+                                                   want_maps=False, **kw))
+    log(f"[vjp] VJP w.r.t. coefficients {t_coef:.2f} s")
+    (gc2, gm), t_all = timed(lambda: api.vjp_coeffs(sp, seg, coef, basis, maps, beta, w, **kw))
+    log(f"[vjp] VJP w.r.t. coefficients and maps {t_all:.2f} s")
+    # dot-product identity at full size: counts are linear in the coefficients
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    xr = torch.rand(coef.shape, generator=gen).to(dev) * (seg.reshape(-1, 1) > 0)
+    jx = api.forward_coeffs(sp, seg, xr.float().contiguous(), basis, maps, beta, **kw)
+    a = float((jx.double().reshape(-1) * w.double()).sum())
+    b = float((xr.double() * gc.double()).sum())
+    dot_rel = abs(a - b) / max(abs(a), abs(b))
+    log(f"[vjp] dot test <Jx,w> {a:.9e} <x,J'w> {b:.9e} rel {dot_rel:.2e}")
+
+    # CPU baseline: the FP64 oracle on 2 central pixels of one view, rays/s
+    cpu = None
+    if not args.no_cpu:
+        ps = make_problem("fan256", n_views=1, n_cols=2)
+        cps = make_coeff_problem(ps, K=16, full_mu=full)
+        ws = np.random.default_rng(1).normal(size=ps.n_rows)
+        t0 = time.perf_counter()
+        O.vjp_coeffs(cps, ws)
+        dt = time.perf_counter() - t0
+        rs = ps.geometry.n_cols * int((ps.segmentation > 0).sum())
+        cpu = {"value": round(rs / dt, 1), "unit": "detector rays/s (VJP w.r.t. coefficients "
+               "and maps)", "cores": os.cpu_count(), "kind": "port",
+               "sample": f"FP64 oracle VJP, fan256 phantom, 1 view x 2 central pixels "
+                         f"({rs} rays, {dt:.1f} s)"}
+        log(f"[vjp] CPU oracle {rs / dt:.0f} rays/s")
+    out = {
+        "metric": "coefficient-model VJP (w.r.t. 16 coefficients/voxel and the mu(E) map) "
+                  "detector rays/s @256²×360 views",
+        "value": round(rays / t_all, 1), "unit": "detector rays/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(t_all * 1000.0, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"vjp256: 256x256 voxels, {g.n_views} views, 256 pixels x 61 "
+                               f"energies, 128 q-bins, K=16, B={cp.B}", "rays": rays},
+        "forward_s": round(t_fwd, 3), "vjp_coef_s": round(t_coef, 3),
+        "vjp_coef_maps_s": round(t_all, 3), "poisson_nll_s": round(t_nll, 4),
+        "coef_grad_same_in_both": bool(torch.equal(gc, gc2)),
+        "dot_test_rel": dot_rel, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -2,7 +2,8 @@
 
 Bars: counts <= 1e-5 relative per row against the FP64 oracle; VJP with respect
 to the coefficients and to the per-energy attenuation maps <= 1e-5 relative
-(norm-wise, and elementwise above 1e-6 of the maximum); dot-product test
+norm-wise, and elementwise <= 1e-5 of J'|w| (the element's sum of term
+magnitudes, the scale FP32 rounding acts on); dot-product test
 <J x, u> = <x, J' u> on the GPU to 1e-6.  The oracle VJP itself is pinned by
 FP64 central differences in tests/test_oracle.py.
 """
@@ -47,11 +48,17 @@ def test_vjp_vs_oracle(xrd, oracle, full_mu):
                             disc=cp.base.disc)
     torch.cuda.synchronize()
     gco, gmo = oracle.vjp_coeffs(cp, w.astype(np.float64))
-    ec, em = _rel(gc.cpu().numpy(), gco), _rel(gm.cpu().numpy(), gmo)
-    big_c = np.abs(gco) > 1e-6 * np.abs(gco).max()
-    ec_el = np.max(np.abs(gc.cpu().numpy()[big_c] - gco[big_c]) / np.abs(gco[big_c]))
-    print(f"\n[vjp B={cp.B}] coef rel err {ec:.2e} (elementwise {ec_el:.2e}); maps {em:.2e}")
-    assert ec <= 1e-5 and ec_el <= 1e-4 and em <= 1e-5
+    # each element's own scale: J' |w|, the sum of the magnitudes of its terms (J has
+    # one sign throughout), which random-sign w cancels by up to ~1e4 in places
+    sco, smo = oracle.vjp_coeffs(cp, np.abs(w).astype(np.float64))
+    gc, gm = gc.cpu().numpy(), gm.cpu().numpy()
+    ec, em = _rel(gc, gco), _rel(gm, gmo)
+    ec_el = float(np.max(np.abs(gc - gco) / np.maximum(np.abs(sco), 1e-300)))
+    em_el = float(np.max(np.abs(gm - gmo) / np.maximum(np.abs(smo), 1e-300)))
+    print(f"\n[vjp B={cp.B}] coef rel err {ec:.2e} (elementwise vs J'|w| {ec_el:.2e}); "
+          f"maps {em:.2e} (elementwise {em_el:.2e})")
+    assert ec <= 1e-5 and em <= 1e-5
+    assert ec_el <= 1e-5 and em_el <= 1e-5
 
 
 def test_vjp_dot_product(xrd):

@@ -132,8 +132,8 @@ __device__ __forceinline__ void geom_c(const ForwardArgs &a, const PixG &pg, int
 }
 
 // Bilinear sample of padded planar map b at (px,py): zero border by index clamping.
-__device__ __forceinline__ float bil_planar(const float *__restrict__ pm, int nx, int ny,
-                                            double px, double py) {
+__device__ __forceinline__ float bil_planar(const float *__restrict__ base, unsigned plane_off,
+                                            int nx, int ny, double px, double py) {
     const double xt = px - 0.5, yt = py - 0.5;
     const double fx = floor(xt), fy = floor(yt);
     const float ax = (float)(xt - fx), ay = (float)(yt - fy);
@@ -142,8 +142,11 @@ __device__ __forceinline__ float bil_planar(const float *__restrict__ pm, int nx
     const int i0c = min(max(i0, -1), nx) + 1, i1c = min(max(i0 + 1, -1), nx) + 1;
     const int j0c = min(max(j0, -1), ny) + 1, j1c = min(max(j0 + 1, -1), ny) + 1;
     const float bx = 1.f - ax, by = 1.f - ay;
-    return by * (bx * pm[i0c + pw * j0c] + ax * pm[i1c + pw * j0c]) +
-           ay * (bx * pm[i0c + pw * j1c] + ax * pm[i1c + pw * j1c]);
+    const float *__restrict__ pm = base;
+    return by * (bx * pm[plane_off + (unsigned)(i0c + pw * j0c)] +
+                 ax * pm[plane_off + (unsigned)(i1c + pw * j0c)]) +
+           ay * (bx * pm[plane_off + (unsigned)(i0c + pw * j1c)] +
+                 ax * pm[plane_off + (unsigned)(i1c + pw * j1c)]);
 }
 
 // Phase 1 for a chunk: lane t = j + KC*(bc + nbc*r) marches samples q = r (mod sp) of
@@ -158,14 +161,28 @@ __device__ __forceinline__ void trace_chunk(const CoeffArgs &a, const PixG &pg, 
         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
                                  a.f.nx, a.f.ny, a.f.ray_step);
         const int b0 = 2 * bc, b1 = 2 * bc + 1;
-        const float *m0 = a.maps_padded + (size_t)b0 * plane;
-        const float *m1 = a.maps_padded + (size_t)(b1 < B ? b1 : b0) * plane;
+        const unsigned o0 = (unsigned)(b0 * plane), o1 = (unsigned)((b1 < B ? b1 : b0) * plane);
         Kahan k0, k1;
-        for (int q = r; q < ray.n; q += sp) {
-            double px, py;
-            ray_sample(ray, a.f.ray_step, q, px, py);
-            k0.add(bil_planar(m0, a.f.nx, a.f.ny, px, py));
-            if (b1 < B) k1.add(bil_planar(m1, a.f.nx, a.f.ny, px, py));
+        // four samples' loads in flight before any is used; samples past the end
+        // repeat the lane's last one and are not added
+        constexpr int U = 4;
+        const int nq = ray.n > r ? (ray.n - r + sp - 1) / sp : 0;
+        for (int i0 = 0; i0 < nq; i0 += U) {
+            float v0[U], v1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                double px, py;
+                ray_sample(ray, a.f.ray_step, r + sp * min(i0 + u, nq - 1), px, py);
+                v0[u] = bil_planar(a.maps_padded, o0, a.f.nx, a.f.ny, px, py);
+                v1[u] = bil_planar(a.maps_padded, o1, a.f.nx, a.f.ny, px, py);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + u >= nq) break;
+                k0.add(v0[u]);
+                if (b1 < B) k1.add(v1[u]);
+            }
         }
         part[t] = make_float2(k0.value(), k1.value());
     }
@@ -182,14 +199,99 @@ __device__ __forceinline__ void trace_chunk(const CoeffArgs &a, const PixG &pg, 
     }
 }
 
+// Texel-major layout for many maps: maps_tm[(i+1) + (nx+2)(j+1)][b] holds map b at
+// texel (i, j), 64 floats per texel (zero for b >= B and on the border).  One wave
+// per ray, one lane per map: each sample is four coalesced 256-byte loads, where
+// the planar layout would touch one cache line per map.
+constexpr int kTexelMaps = 64;
+
+__device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sample q0 + lane of a ray (the last sample for lanes past the end): the cell
+// (i0, j0) of its bilinear stencil and the fractions (ax, ay).
+__device__ __forceinline__ void lane_sample(const Ray &ray, double step, int q, int &i0, int &j0,
+                                            double &ax, double &ay) {
+    double px, py;
+    ray_sample(ray, step, q, px, py);
+    const double xt = px - 0.5, yt = py - 0.5;
+    const double fx = floor(xt), fy = floor(yt);
+    ax = xt - fx;
+    ay = yt - fy;
+    i0 = (int)fx;
+    j0 = (int)fy;
+}
+
+__device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
+                                               int nk, float *Ld) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int nx = a.f.nx, ny = a.f.ny, pw = nx + 2;
+    const float *__restrict__ m = a.maps_tm;
+    for (int j = wid; j < nk; j += NT / kWave) {
+        const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
+                                 nx, ny, a.f.ray_step);
+        const int n = __builtin_amdgcn_readfirstlane(ray.n);
+        Kahan k;
+        for (int q0 = 0; q0 < n; q0 += kWave) {
+            // lane l prepares sample q0 + l: corner offsets and FP32 fractions
+            int i0, j0;
+            double dax, day;
+            lane_sample(ray, a.f.ray_step, min(q0 + lane, n - 1), i0, j0, dax, day);
+            const float fax = (float)dax, fay = (float)day;
+            // samples lie on the segment from the grid entry to a voxel centre, so
+            // i0 is in [-1, nx-1] and j0 in [-1, ny-1] (the clamp only guards the
+            // address) and the stencil's other corners sit at +1 texel and +1 row
+            const int o00 = ((min(max(i0, -1), nx - 1) + 1) + pw * (min(max(j0, -1), ny - 1) + 1)) *
+                            kTexelMaps;
+            const int cnt = min(kWave, n - q0);
+            // every lane then walks the 64 samples for its own map, the loads of
+            // four samples in flight at a time; each four are added in FP32 and
+            // their sum enters the compensated total
+            for (int u0 = 0; u0 < cnt; u0 += 4) {
+                float pv[4][4], wx[4], wy[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int u = min(u0 + kk, cnt - 1);
+                    wx[kk] = read_lane(fax, u);
+                    wy[kk] = read_lane(fay, u);
+                    const float *p0 = m + (unsigned)(read_lane(o00, u) + lane);
+                    const float *p1 = p0 + pw * kTexelMaps;
+                    pv[kk][0] = p0[0];
+                    pv[kk][1] = p0[kTexelMaps];
+                    pv[kk][2] = p1[0];
+                    pv[kk][3] = p1[kTexelMaps];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+                float s4 = 0.f;
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    if (u0 + kk >= cnt) break;
+                    const float ax = wx[kk], ay = wy[kk], bx = 1.f - ax, by = 1.f - ay;
+                    s4 += by * (bx * pv[kk][0] + ax * pv[kk][1]) + ay * (bx * pv[kk][2] + ax * pv[kk][3]);
+                }
+                k.add(s4);
+            }
+        }
+        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)((double)k.value() * a.f.ray_step);
+    }
+}
+
 // Per-(voxel, energy) table: hi/lo wvt, c2 = -log2e/(2 sigma^2), prefactor P, band.
+// Ls, Ld: the voxel's source and detector line integrals; bet: beta [B x NRG]
+// (all three staged in LDS).
 __device__ __forceinline__ void table_entry(const CoeffArgs &a, const VoxC &vs, const float *Ls,
-                                            const float *Ld, int e, float4 &tw, float &band,
-                                            double &P) {
+                                            const float *Ld, const float *bet, int e, float4 &tw,
+                                            float &band, double &P) {
     const EnergyConst ec = a.f.energies[e];
     double mu = 0.0;
     for (int b = 0; b < a.n_basis; b++)
-        mu += (double)a.beta[b * a.f.n_energy + e] * ((double)Ls[b] + (double)Ld[b]);
+        mu += (double)bet[b * a.f.n_energy + e] * ((double)Ls[b] + (double)Ld[b]);
     const double sig2 = vs.As + vs.Bs * ec.k2sq;
     const double w = ec.wcoef * vs.s;
     const float wh = (float)w;
@@ -250,6 +352,33 @@ __global__ void coeff_pad_kernel(CoeffArgs a) {
     }
 }
 
+// maps_tm[(px) + (nx+2) py][b]: texel-major copy of the maps with the zero border.
+__global__ void coeff_pad_tm_kernel(CoeffArgs a) {
+    const int pw = a.f.nx + 2, ph = a.f.ny + 2;
+    const size_t n = (size_t)pw * ph * kTexelMaps;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i % kTexelMaps);
+        const int q = (int)(i / kTexelMaps);
+        const int px = q % pw, py = q / pw;
+        float v = 0.f;
+        if (b < a.n_basis && px >= 1 && px <= a.f.nx && py >= 1 && py <= a.f.ny)
+            v = a.maps[(size_t)b * a.f.nx * a.f.ny + (px - 1) + (size_t)a.f.nx * (py - 1)];
+        a.maps_tm[i] = v;
+    }
+}
+
+// grad_maps[b][pix] = grad_tm[pix][b]
+__global__ void coeff_untranspose_kernel(CoeffArgs a) {
+    const size_t npix = (size_t)a.f.nx * a.f.ny;
+    const size_t n = npix * a.n_basis;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = i / npix, pix = i % npix;
+        a.grad_maps[i] = a.grad_tm[pix * kTexelMaps + b];
+    }
+}
+
 // Source legs: one thread per (view, voxel), all B maps.
 __global__ __launch_bounds__(256) void coeff_src_kernel(CoeffArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
@@ -269,8 +398,8 @@ __global__ __launch_bounds__(256) void coeff_src_kernel(CoeffArgs a) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
                 if (b0 + u < a.n_basis)
-                    ks[u].add(bil_planar(a.maps_padded + (size_t)(b0 + u) * plane, a.f.nx, a.f.ny,
-                                         px, py));
+                    ks[u].add(bil_planar(a.maps_padded, (unsigned)((b0 + u) * plane), a.f.nx,
+                                         a.f.ny, px, py));
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -289,6 +418,8 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ float4 tw[KC * 64];
     __shared__ float tb[KC * 64];
     __shared__ double red[(NT / kWave) * EPT + 64];
+    __shared__ float bet[MAXB * 64];
+    __shared__ float Lsrc[KC * MAXB];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
@@ -316,6 +447,7 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
         ft[i] = 0.0;
     }
     const int nv = a.nvox[0];
+    for (int idx = t; idx < a.n_basis * E; idx += NT) bet[idx] = a.beta[idx];
     for (int k0 = 0; k0 < nv; k0 += KC) {
         const int nk = min(KC, nv - k0);
         if (t < nk) {
@@ -327,8 +459,15 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
             geom_c(f, pg, vs.ix, vs.iy, vs);
             vox[t] = vs;
         }
+        for (int idx = t; idx < nk * a.n_basis; idx += NT) {
+            const int j = idx / a.n_basis, b = idx % a.n_basis;
+            Lsrc[j * MAXB + b] = a.src_ints[((size_t)v * npix + k0 + j) * a.n_basis + b];
+        }
         __syncthreads();
-        trace_chunk(a, pg, vox, nk, part, Ld);
+        if (a.maps_tm)
+            trace_chunk_tm(a, pg, vox, nk, Ld);
+        else
+            trace_chunk(a, pg, vox, nk, part, Ld);
         // F_vox(q_n) for the chunk
         for (int idx = t; idx < nk * TQ; idx += NT) {
             const int j = idx / TQ, n = idx % TQ;
@@ -345,8 +484,7 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
             float4 tv;
             float band;
             double P;
-            table_entry(a, vox[j], a.src_ints + ((size_t)v * npix + vox[j].k) * a.n_basis,
-                        Ld + j * MAXB, e, tv, band, P);
+            table_entry(a, vox[j], Lsrc + j * MAXB, Ld + j * MAXB, bet, e, tv, band, P);
             tw[j * 64 + e] = tv;
             tb[j * 64 + e] = band;
         }
@@ -441,6 +579,56 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
     flush();
 }
 
+// Texel-major form of scatter_ray for one wave.  Lanes first take one sample each
+// (64 at a time) and add the bilinear weights of each run of consecutive samples in
+// one cell with a segmented wave scan; then, per run, lane b adds gamma[b] times the
+// run's weights to grad_tm[texel][b]: four coalesced atomic instructions per cell.
+__device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
+                                               int lane) {
+    const int nx = a.f.nx, ny = a.f.ny;
+    const double g = lane < a.n_basis ? gam[lane] : 0.0;
+    const int n = __builtin_amdgcn_readfirstlane(ray.n);
+    for (int q0 = 0; q0 < n; q0 += kWave) {
+        const int cnt = min(kWave, n - q0);
+        int i0, j0;
+        double ax, ay;
+        lane_sample(ray, a.f.ray_step, min(q0 + lane, n - 1), i0, j0, ax, ay);
+        const bool act = lane < cnt;
+        const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
+        const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
+        const bool first = lane == 0 || pi != i0 || pj != j0;
+        const bool last = act && (lane == cnt - 1 || ni != i0 || nj != j0);
+        const unsigned long long starts = __ballot(first);
+        // first lane of this lane's run: highest start at or below it
+        const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int rs = 63 - __clzll(below);
+        double w[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double up = __shfl_up(w[c], d, kWave);
+                if (lane - d >= rs) w[c] += up;
+            }
+        }
+        unsigned long long ends = __ballot(last);
+        while (ends) {
+            const int l = __builtin_amdgcn_readfirstlane(__ffsll(ends) - 1);
+            ends &= ends - 1;
+            const int ci = read_lane(i0, l), cj = read_lane(j0, l);
+            const int ii[4] = {ci, ci + 1, ci, ci + 1}, jj[4] = {cj, cj, cj + 1, cj + 1};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double wc = read_lane(w[c], l);
+                if (ii[c] < 0 || jj[c] < 0 || ii[c] >= nx || jj[c] >= ny) continue;
+                const size_t pix = (size_t)ii[c] + (size_t)nx * jj[c];
+                if (g != 0.0)
+                    unsafeAtomicAdd(a.grad_tm + pix * kTexelMaps + lane, g * (wc * a.f.ray_step));
+            }
+        }
+    }
+}
+
 // Lane r of sp lanes that share a ray takes a contiguous run of its samples.
 __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1) {
     const int per = (n + sp - 1) / sp;
@@ -469,6 +657,8 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     __shared__ double gam[KC * MAXB];
     __shared__ double gsrc[KC * MAXB];
     __shared__ double wt[64];
+    __shared__ float bet[MAXB * 64];
+    __shared__ float Lsrc[KC * MAXB];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int lane = t & (kWave - 1), wid = t >> 6;
@@ -507,8 +697,13 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     double ht[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) ht[j] = 0.0;
+    for (int idx = t; idx < B * E; idx += NT) bet[idx] = a.beta[idx];
     for (int v = va; v < vb; v++) {
         for (int idx = t; idx < KC * MAXB; idx += NT) gsrc[idx] = 0.0;
+        for (int idx = t; idx < nk * B; idx += NT) {
+            const int j = idx / B, b = idx % B;
+            Lsrc[j * MAXB + b] = a.src_ints[((size_t)v * npix + k0 + j) * B + b];
+        }
         for (int p = 0; p < P; p++) {
             const PixG pg = pixel_setup(a, v, p);
             if (t < nk) {
@@ -533,15 +728,17 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
                 wt[t] = sw;
             }
             __syncthreads();
-            trace_chunk(a, pg, vox, nk, part, Ld);
+            if (a.maps_tm)
+                trace_chunk_tm(a, pg, vox, nk, Ld);
+            else
+                trace_chunk(a, pg, vox, nk, part, Ld);
             __syncthreads();
             for (int idx = t; idx < nk * E; idx += NT) {
                 const int j = idx / E, e = idx % E;
                 float4 tv;
                 float band;
                 double Pe;
-                table_entry(a, vox[j], a.src_ints + ((size_t)v * npix + vox[j].k) * a.n_basis,
-                            Ld + j * MAXB, e, tv, band, Pe);
+                table_entry(a, vox[j], Lsrc + j * MAXB, Ld + j * MAXB, bet, e, tv, band, Pe);
                 pw[j * 64 + e] = wt[e] * Pe;
                 tv.w = (float)(wt[e] * Pe);
                 tw[j * 64 + e] = tv;
@@ -587,35 +784,52 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
                         double se = 0.0;
                         for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++)
                             se += (double)sp_[w][j][i];
-                        g -= (double)a.beta[b * E + e] * pw[j * 64 + e] * se;
+                        g -= (double)bet[b * E + e] * pw[j * 64 + e] * se;
                     }
                     gam[j * MAXB + b] = g;
                     gsrc[j * MAXB + b] += g;
                 }
                 __syncthreads();
                 // scatter gamma along the detector legs
-                const int j = t % KC, r = t / KC;
-                if (j < nk) {
-                    const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
-                                             (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
-                    int q0, q1;
-                    lane_run(ray.n, r, NT / KC, q0, q1);
-                    scatter_ray(a, ray, gam + j * MAXB, q0, q1);
+                if (a.grad_tm) {
+                    for (int j = wid; j < nk; j += NT / kWave) {
+                        const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
+                                                 (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
+                        scatter_ray_tm(a, ray, gam + j * MAXB, lane);
+                    }
+                } else {
+                    const int j = t % KC, r = t / KC;
+                    if (j < nk) {
+                        const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
+                                                 (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
+                        int q0, q1;
+                        lane_run(ray.n, r, NT / KC, q0, q1);
+                        scatter_ray(a, ray, gam + j * MAXB, q0, q1);
+                    }
                 }
             }
             __syncthreads();
         }
         // source legs of view v carry the sum of gamma over the pixels
         if (maps) {
-            const int j = t % KC, r = t / KC;
-            if (j < nk) {
-                const ViewConst vc = f.views[v];
-                const int pix = a.vlist[k0 + j];
-                const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
-                                         (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
-                int q0, q1;
-                lane_run(ray.n, r, NT / KC, q0, q1);
-                scatter_ray(a, ray, gsrc + j * MAXB, q0, q1);
+            const ViewConst vc = f.views[v];
+            if (a.grad_tm) {
+                for (int j = wid; j < nk; j += NT / kWave) {
+                    const int pix = a.vlist[k0 + j];
+                    const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
+                                             (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
+                    scatter_ray_tm(a, ray, gsrc + j * MAXB, lane);
+                }
+            } else {
+                const int j = t % KC, r = t / KC;
+                if (j < nk) {
+                    const int pix = a.vlist[k0 + j];
+                    const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
+                                             (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
+                    int q0, q1;
+                    lane_run(ray.n, r, NT / KC, q0, q1);
+                    scatter_ray(a, ray, gsrc + j * MAXB, q0, q1);
+                }
             }
         }
         __syncthreads();
@@ -705,6 +919,7 @@ hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s) {
     unsigned blocks = (unsigned)((n + 255) / 256);
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(coeff_pad_kernel, dim3(blocks), dim3(256), 0, s, a);
+    if (a.maps_tm) hipLaunchKernelGGL(coeff_pad_tm_kernel, dim3(8192), dim3(256), 0, s, a);
     dim3 grid((a.f.nx * a.f.ny + 255) / 256, a.f.n_views);
     hipLaunchKernelGGL(coeff_src_kernel, grid, dim3(256), 0, s, a);
     return hipGetLastError();
@@ -725,6 +940,10 @@ int coeff_vjp_groups(int n_views, int npix) {
     return ng < 1 ? 1 : (int)ng;
 }
 
+bool coeff_texel_major(int n_basis) { return n_basis > 4; }
+
+size_t coeff_texel_floats(int nx, int ny) { return (size_t)(nx + 2) * (ny + 2) * kTexelMaps; }
+
 size_t coeff_hpart_doubles(int n_views, int npix) {
     return (size_t)coeff_vjp_groups(n_views, npix) * (size_t)npix * TQ;
 }
@@ -737,6 +956,8 @@ hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb, ng), dim3(NT), 0, s, a, vpg);
         if (a.grad_coef)
             hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
+        if (a.grad_maps && a.grad_tm)
+            hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }

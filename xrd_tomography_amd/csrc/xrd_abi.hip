@@ -69,7 +69,7 @@ struct xrd_context {
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
-    DevBuf cm_maps, cm_src, cm_vlist, cm_h;
+    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm;
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -191,7 +191,7 @@ int xrd_context_destroy(xrd_context *c) {
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
                       &c->src_int, &c->tables, &c->pv, &c->partial, &c->small, &c->cm_maps,
-                      &c->cm_src, &c->cm_vlist, &c->cm_h})
+                      &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
@@ -205,7 +205,8 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     int64_t s = 0;
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
                             &c->offsets, &c->src_int, &c->tables, &c->pv, &c->partial,
-                            &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h})
+                            &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h,
+                            &c->cm_tm, &c->cm_gtm})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -867,12 +868,16 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
         return fail(XRD_ERR_UNSUPPORTED, "%s: need 1 <= n_coef <= 32 and 1 <= n_basis <= 64", fn);
     if (d->n_energy > 64 || d->n_q > 128)
         return fail(XRD_ERR_UNSUPPORTED, "%s: need n_energy <= 64 and n_q <= 128", fn);
+    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 >= ((size_t)1 << 31))
+        return fail(XRD_ERR_UNSUPPORTED, "%s: (nx+2)(ny+2)*64 must stay below 2^31", fn);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const size_t npix = (size_t)g->nx * g->ny;
     const size_t npad = (size_t)(g->nx + 2) * (g->ny + 2);
     HIP_TRY(ctx->cm_maps.ensure(npad * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_src.ensure((size_t)g->n_views * npix * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_vlist.ensure((npix + 4) * sizeof(int)), "workspace alloc");
+    const bool tm = coeff_texel_major(n_basis);
+    if (tm) HIP_TRY(ctx->cm_tm.ensure(coeff_texel_floats(g->nx, g->ny) * sizeof(float)), "workspace alloc");
     ForwardArgs fa;
     rc = setup_forward_common(ctx, g, d, spectrum, fa, s);
     if (rc) return rc;
@@ -890,6 +895,7 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     ca.vlist = (int *)ctx->cm_vlist.ptr;
     ca.nvox = (int *)ctx->cm_vlist.ptr + npix;
     ca.nvox_cap = (int)npix;
+    ca.maps_tm = tm ? (float *)ctx->cm_tm.ptr : nullptr;
     HIP_TRY(launch_coeff_prepare(ca, s), "coefficient-model preparation");
     return XRD_OK;
 }
@@ -941,7 +947,16 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
         ca.hpart = (double *)ctx->cm_h.ptr;
         HIP_TRY(launch_zero_f64(grad_coef, npix * n_coef, s), "zero kernel");
     }
-    if (grad_maps) HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
+    if (grad_maps) {
+        if (ca.maps_tm) {
+            // accumulate texel-major, written out by the VJP launch
+            HIP_TRY(ctx->cm_gtm.ensure(npix * 64 * sizeof(double)), "workspace alloc");
+            ca.grad_tm = (double *)ctx->cm_gtm.ptr;
+            HIP_TRY(launch_zero_f64(ca.grad_tm, npix * 64, s), "zero kernel");
+        } else {
+            HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
+        }
+    }
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
     return XRD_OK;
 }

@@ -116,6 +116,8 @@ struct CoeffArgs {
     float *out_counts;       // [rows]
     double *grad_coef;       // [nx*ny x K] or NULL
     double *grad_maps;       // [B x nx*ny] or NULL (zeroed by the caller)
+    float *maps_tm;          // [(nx+2)(ny+2) x 64] texel-major maps, or NULL (B <= 4)
+    double *grad_tm;         // [nx*ny x 64] map gradient while accumulating, or NULL
     int nvox_cap;            // nx*ny: bound of the voxel list
     double *hpart;           // coeff_hpart_doubles(V, nx*ny) when grad_coef is set
 };
@@ -124,6 +126,8 @@ hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
 size_t coeff_hpart_doubles(int n_views, int npix);
+bool coeff_texel_major(int n_basis);
+size_t coeff_texel_floats(int nx, int ny);
 hipError_t launch_zero_f64(double *p, size_t n, hipStream_t s);
 hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, const float *y,
                               float *grad, double *partial, int nblocks, hipStream_t s);
