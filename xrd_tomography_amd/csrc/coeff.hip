@@ -579,14 +579,29 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
     flush();
 }
 
-// Texel-major form of scatter_ray for one wave.  Lanes first take one sample each
-// (64 at a time) and add the bilinear weights of each run of consecutive samples in
-// one cell with a segmented wave scan; then, per run, lane b adds gamma[b] times the
-// run's weights to grad_tm[texel][b]: four coalesced atomic instructions per cell.
+// One run of samples in one cell (ci, cj): element offsets texel*64 of its four
+// stencil corners (-1 outside the map, or when a later run adds this corner's
+// weight) and their summed bilinear weights.
+struct CellRun {
+    int off[4];
+    int ci, cj, pad0, pad1;
+    double w[4];
+};
+
+// Texel-major form of scatter_ray for one wave.  Lanes take one sample each (64 at a
+// time) and add the bilinear weights of each run of consecutive samples in one cell
+// with a segmented wave scan; the run-end lanes write their cells to the wave's
+// CellRun list in LDS.  A stencil corner is shared by at most three consecutive
+// cells of a straight ray, so each run then folds in the weights of the two runs
+// before it for the corners it shares with them and leaves out the corners that
+// one of the next two runs shares: each texel is emitted once per 64 samples.
+// Finally the wave walks the list: for each cell and kept corner, lane b adds
+// gamma[b] times the weight to grad_tm[texel][b] (one coalesced atomic
+// instruction).  The list keeps this walk in vector instructions.
 __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
-                                               int lane) {
+                                               int lane, CellRun *runs) {
     const int nx = a.f.nx, ny = a.f.ny;
-    const double g = lane < a.n_basis ? gam[lane] : 0.0;
+    const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     for (int q0 = 0; q0 < n; q0 += kWave) {
         const int cnt = min(kWave, n - q0);
@@ -611,21 +626,72 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
                 if (lane - d >= rs) w[c] += up;
             }
         }
-        unsigned long long ends = __ballot(last);
-        while (ends) {
-            const int l = __builtin_amdgcn_readfirstlane(__ffsll(ends) - 1);
-            ends &= ends - 1;
-            const int ci = read_lane(i0, l), cj = read_lane(j0, l);
-            const int ii[4] = {ci, ci + 1, ci, ci + 1}, jj[4] = {cj, cj, cj + 1, cj + 1};
+        const unsigned long long ends = __ballot(last);
+        const int nr = __popcll(ends);
+        if (last) {
+            CellRun cr;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const double wc = read_lane(w[c], l);
-                if (ii[c] < 0 || jj[c] < 0 || ii[c] >= nx || jj[c] >= ny) continue;
-                const size_t pix = (size_t)ii[c] + (size_t)nx * jj[c];
-                if (g != 0.0)
-                    unsafeAtomicAdd(a.grad_tm + pix * kTexelMaps + lane, g * (wc * a.f.ray_step));
+                const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
+                const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
+                cr.off[c] = in ? (ti + nx * tj) * kTexelMaps : -1;
+                cr.w[c] = w[c];
+            }
+            cr.ci = i0;
+            cr.cj = j0;
+            cr.pad0 = cr.pad1 = 0;
+            runs[__popcll(ends & ((1ull << lane) - 1ull))] = cr;
+        }
+        // a wave's LDS accesses complete in issue order, so only the compiler must
+        // keep the list reads below the writes (a memory fence would also wait for
+        // every atomic in flight)
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        // lane r < nr settles run r's corners against runs r-2 .. r+2
+        int newoff[4];
+        double neww[4];
+        if (lane < nr) {
+            const CellRun me = runs[lane];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int ti = me.ci + (c & 1), tj = me.cj + (c >> 1);
+                bool later = false;
+                double wsum = me.w[c];
+#pragma unroll
+                for (int k = -2; k <= 2; k++) {
+                    if (k == 0 || lane + k < 0 || lane + k >= nr) continue;
+                    const int di = ti - runs[lane + k].ci, dj = tj - runs[lane + k].cj;
+                    if (di < 0 || di > 1 || dj < 0 || dj > 1) continue;
+                    if (k > 0)
+                        later = true;
+                    else
+                        wsum += runs[lane + k].w[di + 2 * dj];
+                }
+                newoff[c] = later ? -1 : me.off[c];
+                neww[c] = wsum;
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        if (lane < nr) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                runs[lane].off[c] = newoff[c];
+                runs[lane].w[c] = neww[c];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        if (gs != 0.0) {
+            for (int r = 0; r < nr; r++) {
+                const CellRun cr = runs[r];
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (cr.off[c] >= 0) unsafeAtomicAdd(a.grad_tm + cr.off[c] + lane, gs * cr.w[c]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
     }
 }
 
@@ -659,6 +725,7 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     __shared__ double wt[64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
+    __shared__ CellRun cruns[NT / kWave][kWave];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int lane = t & (kWave - 1), wid = t >> 6;
@@ -795,7 +862,7 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
                     for (int j = wid; j < nk; j += NT / kWave) {
                         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
                                                  (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
-                        scatter_ray_tm(a, ray, gam + j * MAXB, lane);
+                        scatter_ray_tm(a, ray, gam + j * MAXB, lane, cruns[wid]);
                     }
                 } else {
                     const int j = t % KC, r = t / KC;
@@ -818,7 +885,7 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
                     const int pix = a.vlist[k0 + j];
                     const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % f.nx) + 0.5,
                                              (double)(pix / f.nx) + 0.5, f.nx, f.ny, f.ray_step);
-                    scatter_ray_tm(a, ray, gsrc + j * MAXB, lane);
+                    scatter_ray_tm(a, ray, gsrc + j * MAXB, lane, cruns[wid]);
                 }
             } else {
                 const int j = t % KC, r = t / KC;
