@@ -440,10 +440,10 @@ def run_vjp(args, dev):
     dot_rel = abs(a - b) / max(abs(a), abs(b))
     log(f"[vjp] dot test <Jx,w> {a:.9e} <x,J'w> {b:.9e} rel {dot_rel:.2e}")
 
-    # CPU baseline: the FP64 oracle on 2 central pixels of one view, rays/s
+    # CPU baseline: the FP64 oracle on the central pixel of one view, rays/s
     cpu = None
     if not args.no_cpu:
-        ps = make_problem("fan256", n_views=1, n_cols=2)
+        ps = make_problem("fan256", n_views=1, n_cols=1)
         cps = make_coeff_problem(ps, K=16, full_mu=full)
         ws = np.random.default_rng(1).normal(size=ps.n_rows)
         t0 = time.perf_counter()
@@ -452,7 +452,7 @@ def run_vjp(args, dev):
         rs = ps.geometry.n_cols * int((ps.segmentation > 0).sum())
         cpu = {"value": round(rs / dt, 1), "unit": "detector rays/s (VJP w.r.t. coefficients "
                "and maps)", "cores": os.cpu_count(), "kind": "port",
-               "sample": f"FP64 oracle VJP, fan256 phantom, 1 view x 2 central pixels "
+               "sample": f"FP64 oracle VJP, fan256 phantom, 1 view x 1 central pixel "
                          f"({rs} rays, {dt:.1f} s)"}
         log(f"[vjp] CPU oracle {rs / dt:.0f} rays/s")
     out = {
