@@ -204,6 +204,27 @@ int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretiza
                    const float *beta, const float *grad_counts, double *grad_coef,
                    double *grad_maps, void *stream);
 
+/* CT transmission channel of the simultaneous CT + XRD scanner (README.md:14).
+ * No reference counterpart to bind: the reference names it only in the
+ * commented-out launches of integrate_src2det / polychromatic_src2det
+ * (Forward_XRD.cu:271-276) and does not ship their kernels.  For view v and
+ * detector column c the primary ray runs from the source to the column's point on
+ * the detector (the XRD pixel positions of row 0); its part inside the grid
+ * (Liang-Barsky clipping, FP64) is sampled like attenuator.cu:44-69 from the
+ * source-side entry, and
+ *   out_t[v + V*(c + COL*ed)] = sum_{e in win(ed)} spectrum[e + NRG*ed]
+ *                               * exp(-sum_b beta[b,e] L_b),
+ * L_b the line integral of maps[b] [nx*ny].  Limits: B <= 64, NRG <= 64. */
+int xrd_forward_ct(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                   const float *spectrum, int32_t n_basis, const float *maps, const float *beta,
+                   float *out_t, void *stream);
+
+/* VJP of xrd_forward_ct with respect to the maps for the upstream gradient
+ * grad_t [V*COL*NRG]: grad_maps [B x nx*ny] (FP64, overwritten). */
+int xrd_vjp_ct(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+               const float *spectrum, int32_t n_basis, const float *maps, const float *beta,
+               const float *grad_t, double *grad_maps, void *stream);
+
 /* Poisson negative log-likelihood sum(f - y log f), f = counts + background
  * (background may be NULL), written to *out_loss_host (synchronises the stream);
  * grad_counts (optional) receives 1 - y/f. */

@@ -280,3 +280,51 @@ def vjp_coeffs(cp, w, nthreads=0, want_coef=True, want_maps=True):
         raise RuntimeError(f"xo_vjp_coeffs failed: {rc}")
     return (gc.reshape(npix, cp.K) if want_coef else None,
             gm.reshape(cp.B, npix) if want_maps else None)
+
+
+def ct_chord(sx, sy, dx, dy, nx, ny):
+    """(hit, entry_x, entry_y, exit_x, exit_y) of the segment clipped to the grid."""
+    L = lib()
+    L.xo_ct_chord.restype = ctypes.c_int
+    L.xo_ct_chord.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [_dp] * 4
+    r = [ctypes.c_double() for _ in range(4)]
+    hit = L.xo_ct_chord(sx, sy, dx, dy, nx, ny, *[ctypes.byref(x) for x in r])
+    return bool(hit), r[0].value, r[1].value, r[2].value, r[3].value
+
+
+def forward_ct(prob, maps, beta, nthreads=0, spectrum=None):
+    """CT transmission T [V*COL*NRG] (v fastest) of maps [B, nx*ny] with beta [B, NRG]."""
+    L = lib()
+    L.xo_forward_ct.restype = ctypes.c_int
+    L.xo_forward_ct.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization), _dp,
+                                ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+    g, d = prob.geometry, prob.disc
+    sp = _d(prob.flat("spectrum") if spectrum is None else np.asarray(spectrum).reshape(-1))
+    maps = np.asarray(maps, np.float64)
+    B = maps.shape[0]
+    mp, be = _d(maps.reshape(-1)), _d(np.asarray(beta).reshape(-1))
+    out = np.zeros(g.n_views * g.n_cols * d.n_energy)
+    rc = L.xo_forward_ct(ctypes.byref(g), ctypes.byref(d), sp[1], B, mp[1], be[1],
+                         out.ctypes.data_as(_dp), nthreads)
+    if rc:
+        raise RuntimeError(f"xo_forward_ct failed: {rc}")
+    return out
+
+
+def vjp_ct(prob, maps, beta, w, nthreads=0):
+    """grad_maps [B, nx*ny] of <w, T(maps)> (xo_vjp_ct)."""
+    L = lib()
+    L.xo_vjp_ct.restype = ctypes.c_int
+    L.xo_vjp_ct.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization), _dp,
+                            ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+    g, d = prob.geometry, prob.disc
+    sp = _d(prob.flat("spectrum"))
+    maps = np.asarray(maps, np.float64)
+    B = maps.shape[0]
+    mp, be, ww = _d(maps.reshape(-1)), _d(np.asarray(beta).reshape(-1)), _d(w)
+    gm = np.zeros(B * g.nx * g.ny)
+    rc = L.xo_vjp_ct(ctypes.byref(g), ctypes.byref(d), sp[1], B, mp[1], be[1], ww[1],
+                     gm.ctypes.data_as(_dp), nthreads)
+    if rc:
+        raise RuntimeError(f"xo_vjp_ct failed: {rc}")
+    return gm.reshape(B, g.nx * g.ny)

@@ -149,12 +149,10 @@ double xo_bilinear(const double *map, int nx, int ny, double x, double y) {
            ax * ay * xo_texel(map, nx, ny, i + 1, j + 1);
 }
 
-/* attenuator.cu:9-109: step * sum over samples of the bilinear (a1,a2,...). */
-void xo_line_integrals(const double *maps, int nb, int nx, int ny, double step, double x1,
-                       double y1, double x2, double y2, double *out) {
+/* Samples of attenuator.cu:44-69 from a given entry point (ex,ey) towards (x2,y2). */
+static void xo_line_integrals_from(const double *maps, int nb, int nx, int ny, double step,
+                                   double ex, double ey, double x2, double y2, double *out) {
     for (int b = 0; b < nb; b++) out[b] = 0.0;
-    double ex, ey;
-    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) return;
     double ux, uy, L;
     xo_ray_dir(ex, ey, x2, y2, &ux, &uy, &L);
     const int32_t n = xo_sample_count(L, step);
@@ -166,6 +164,17 @@ void xo_line_integrals(const double *maps, int nb, int nx, int ny, double step, 
         for (int b = 0; b < nb; b++) out[b] += xo_bilinear(maps + b * plane, nx, ny, px, py);
     }
     for (int b = 0; b < nb; b++) out[b] *= step;
+}
+
+/* attenuator.cu:9-109: step * sum over samples of the bilinear (a1,a2,...). */
+void xo_line_integrals(const double *maps, int nb, int nx, int ny, double step, double x1,
+                       double y1, double x2, double y2, double *out) {
+    double ex, ey;
+    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) {
+        for (int b = 0; b < nb; b++) out[b] = 0.0;
+        return;
+    }
+    xo_line_integrals_from(maps, nb, nx, ny, step, ex, ey, x2, y2, out);
 }
 
 /* find_src_nrg_range.cu:8-25 (the window is shifted, never truncated). */
@@ -591,11 +600,10 @@ double xo_loglik(const double *A, int64_t n_rows, int n_cols, const double *y,
  *   S(e) = sum_n exp(-(wvt_e - q_n)^2 / (2 sigma_e^2)) dq_n F_vox(q_n)
  * The Compton term is a constant background here. */
 
-/* Adds gamma[b]*step*(bilinear weight) of every sample of the ray to grad[b][pix]. */
-static void xo_line_scatter(double *grad, int nb, int nx, int ny, double step, double x1,
-                            double y1, double x2, double y2, const double *gamma) {
-    double ex, ey;
-    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) return;
+/* Adds gamma[b]*step*(bilinear weight) of every sample of the ray from the entry
+ * point (ex,ey) towards (x2,y2) to grad[b][pix]. */
+static void xo_line_scatter_from(double *grad, int nb, int nx, int ny, double step, double ex,
+                                 double ey, double x2, double y2, const double *gamma) {
     double ux, uy, L;
     xo_ray_dir(ex, ey, x2, y2, &ux, &uy, &L);
     const int32_t n = xo_sample_count(L, step);
@@ -615,6 +623,14 @@ static void xo_line_scatter(double *grad, int nb, int nx, int ny, double step, d
             for (int b = 0; b < nb; b++) grad[b * plane + pix] += gamma[b] * step * w[c];
         }
     }
+}
+
+/* xo_line_scatter_from for the attenuator ray (x1,y1)->(x2,y2). */
+static void xo_line_scatter(double *grad, int nb, int nx, int ny, double step, double x1,
+                            double y1, double x2, double y2, const double *gamma) {
+    double ex, ey;
+    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) return;
+    xo_line_scatter_from(grad, nb, nx, ny, step, ex, ey, x2, y2, gamma);
 }
 
 typedef struct {
@@ -833,5 +849,143 @@ int xo_vjp_coeffs(const xrd_geometry *g, const xrd_discretization *d, const doub
     free(hpart);
     free(gpart);
     xo_cm_free(&s);
+    return XRD_OK;
+}
+
+/* ---- CT transmission channel ---------------------------------------------------
+ * The transmitted (CT) signal of the simultaneous CT + XRD scanner (README.md:14).
+ * The reference only names it, in the commented-out launches of integrate_src2det
+ * and polychromatic_src2det (Forward_XRD.cu:271-276); their kernels are not in the
+ * repository, so this definition is new.  For view v and detector column c the
+ * primary ray runs from the source to the column's point on the detector (the 2-D
+ * endpoints of xo_scatter_geometry).  Its part inside [0,nx]x[0,ny] (Liang-Barsky
+ * clipping, FP64, no contraction) is sampled like attenuator.cu:44-69 from the
+ * source-side entry point:
+ *   L_b = step * sum_q bilinear(m_b, entry + u q step),  q*step <= |exit - entry|
+ *   T[v + V (c + COL ed)] = sum_{e in win(ed)} RF[e,ed] exp(-sum_b beta[b,e] L_b). */
+
+int xo_ct_chord(double sx, double sy, double dx, double dy, int nx, int ny, double *ex,
+                double *ey, double *xx, double *xy) {
+    const double ddx = dx - sx, ddy = dy - sy;
+    const double p[4] = {-ddx, ddx, -ddy, ddy};
+    const double q[4] = {sx - 0.0, (double)nx - sx, sy - 0.0, (double)ny - sy};
+    double t0 = 0.0, t1 = 1.0;
+    for (int k = 0; k < 4; k++) {
+        if (p[k] == 0.0) {
+            if (q[k] < 0.0) return 0;
+            continue;
+        }
+        const double r = q[k] / p[k];
+        if (p[k] < 0.0) {
+            if (r > t0) t0 = r;
+        } else {
+            if (r < t1) t1 = r;
+        }
+    }
+    if (!(t0 < t1)) return 0;
+    *ex = sx + ddx * t0;
+    *ey = sy + ddy * t0;
+    *xx = sx + ddx * t1;
+    *xy = sy + ddy * t1;
+    return 1;
+}
+
+/* Per-ray pieces: line integrals L[b] and P[e] = exp(-mu_e L). */
+static void xo_ct_ray(const xrd_geometry *g, const xrd_discretization *d, int B,
+                      const double *maps, const double *beta, int v, int c, double *L, double *P,
+                      double *chord, int *hit) {
+    const int E = d->n_energy;
+    xo_scatter_geom sg;
+    xo_scatter_geometry(g, v, c, 0, 0, 0, &sg);
+    *hit = xo_ct_chord(sg.src_x, sg.src_y, sg.det_x, sg.det_y, g->nx, g->ny, &chord[0],
+                       &chord[1], &chord[2], &chord[3]);
+    if (*hit)
+        xo_line_integrals_from(maps, B, g->nx, g->ny, d->ray_step, chord[0], chord[1], chord[2],
+                               chord[3], L);
+    else
+        for (int b = 0; b < B; b++) L[b] = 0.0;
+    for (int e = 0; e < E; e++) {
+        double mu = 0.0;
+        for (int b = 0; b < B; b++) mu += beta[(long)b * E + e] * L[b];
+        P[e] = exp(-mu);
+    }
+}
+
+int xo_forward_ct(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                  int B, const double *maps, const double *beta, double *out_t, int nthreads) {
+    const int V = g->n_views, COL = g->n_cols, E = d->n_energy, DN = d->energy_halfwidth;
+    if (g->nx <= 0 || g->ny <= 0 || V <= 0 || COL <= 0 || B <= 0 || E <= 2 * DN)
+        return XRD_ERR_DIMS;
+    const int nt = xo_threads(nthreads);
+#pragma omp parallel num_threads(nt)
+    {
+        double *L = (double *)malloc(sizeof(double) * (B + E + 4));
+        double *P = L + B, *chord = P + E;
+#pragma omp for schedule(dynamic, 4)
+        for (long vc = 0; vc < (long)V * COL; vc++) {
+            const int v = (int)(vc % V), c = (int)(vc / V);
+            int hit;
+            xo_ct_ray(g, d, B, maps, beta, v, c, L, P, chord, &hit);
+            for (int ed = 0; ed < E; ed++) {
+                int lo, hi;
+                xo_find_src_nrg_range(ed, E, DN, &lo, &hi);
+                double t = 0.0;
+                for (int e = lo; e <= hi; e++) t += spectrum[e + (long)E * ed] * P[e];
+                out_t[v + (long)V * (c + (long)COL * ed)] = t;
+            }
+        }
+        free(L);
+    }
+    return XRD_OK;
+}
+
+int xo_vjp_ct(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum, int B,
+              const double *maps, const double *beta, const double *w, double *grad_maps,
+              int nthreads) {
+    const int V = g->n_views, COL = g->n_cols, E = d->n_energy, DN = d->energy_halfwidth;
+    if (g->nx <= 0 || g->ny <= 0 || V <= 0 || COL <= 0 || B <= 0 || E <= 2 * DN)
+        return XRD_ERR_DIMS;
+    const long npix = (long)g->nx * g->ny;
+    const int nt = xo_threads(nthreads);
+    double *gpart = (double *)calloc((size_t)nt * B * npix, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *GM = gpart + (size_t)tid * B * npix;
+        double *L = (double *)malloc(sizeof(double) * (2 * B + 2 * E + 4));
+        double *P = L + B, *Wt = P + E, *gam = Wt + E, *chord = gam + B;
+#pragma omp for schedule(dynamic, 4)
+        for (long vc = 0; vc < (long)V * COL; vc++) {
+            const int v = (int)(vc % V), c = (int)(vc / V);
+            int hit;
+            xo_ct_ray(g, d, B, maps, beta, v, c, L, P, chord, &hit);
+            if (!hit) continue;
+            for (int e = 0; e < E; e++) Wt[e] = 0.0;
+            for (int ed = 0; ed < E; ed++) {
+                int lo, hi;
+                xo_find_src_nrg_range(ed, E, DN, &lo, &hi);
+                const double wr = w[v + (long)V * (c + (long)COL * ed)];
+                for (int e = lo; e <= hi; e++) Wt[e] += spectrum[e + (long)E * ed] * wr;
+            }
+            /* dT/dL_b = -sum_e RF P_e beta[b,e] */
+            for (int b = 0; b < B; b++) {
+                double gb = 0.0;
+                for (int e = 0; e < E; e++) gb -= Wt[e] * beta[(long)b * E + e] * P[e];
+                gam[b] = gb;
+            }
+            xo_line_scatter_from(GM, B, g->nx, g->ny, d->ray_step, chord[0], chord[1], chord[2],
+                                 chord[3], gam);
+        }
+        free(L);
+    }
+    for (long i = 0; i < (long)B * npix; i++) {
+        double acc = 0.0;
+        for (int t = 0; t < nt; t++) acc += gpart[(size_t)t * B * npix + i];
+        grad_maps[i] = acc;
+    }
+    free(gpart);
     return XRD_OK;
 }

@@ -117,6 +117,22 @@ int xo_vjp_coeffs(const xrd_geometry *g, const xrd_discretization *d, const doub
                   double *grad_maps, int nthreads);
 
 /* Number of OpenMP threads the oracle would use for nthreads <= 0. */
+/* CT transmission channel (new; named by the commented-out integrate_src2det and
+ * polychromatic_src2det launches, Forward_XRD.cu:271-276).  xo_ct_chord clips the
+ * segment (sx,sy)->(dx,dy) to [0,nx]x[0,ny] (Liang-Barsky, FP64): returns 1 and the
+ * entry (ex,ey) and exit (xx,xy), or 0 if it misses.  xo_forward_ct: for view v and
+ * detector column c, T[v + V (c + COL ed)] = sum_{e in win(ed)} RF[e,ed]
+ * exp(-sum_b beta[b,e] L_b) with L_b the line integral of map b over the clipped
+ * source-to-column ray (attenuator.cu:44-69 sampling from the entry).  xo_vjp_ct:
+ * grad_maps [B x nx*ny] = dT/dmaps transposed applied to w [V*COL*NRG]. */
+int xo_ct_chord(double sx, double sy, double dx, double dy, int nx, int ny, double *ex,
+                double *ey, double *xx, double *xy);
+int xo_forward_ct(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                  int B, const double *maps, const double *beta, double *out_t, int nthreads);
+int xo_vjp_ct(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum, int B,
+              const double *maps, const double *beta, const double *w, double *grad_maps,
+              int nthreads);
+
 int xo_max_threads(void);
 
 #ifdef __cplusplus

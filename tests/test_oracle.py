@@ -455,6 +455,133 @@ def test_vjp_central_differences(oracle):
 
 
 # ---------------------------------------------------------------------------
+# CT transmission channel: chord by bisection, Beer-Lambert recomputed in numpy,
+# the zero-attenuation closed form and FP64 central differences for the VJP
+
+def _ct_case(full_mu=False):
+    from xrd_tomography_amd.synthetic import make_coeff_problem, make_problem
+    p = make_problem("small")
+    cp = make_coeff_problem(p, K=2, full_mu=full_mu)
+    return p, cp.maps.astype(np.float64), cp.beta.astype(np.float64)
+
+
+def _inside(x, y, nx, ny):
+    return (x >= 0) & (x <= nx) & (y >= 0) & (y <= ny)
+
+
+def test_ct_chord_by_bisection(oracle):
+    """Entry and exit of random segments through the grid, located by dense sampling
+    of the segment and bisection of the inside/outside boundary."""
+    rng = np.random.default_rng(21)
+    nx, ny = 37, 23
+    n_hit = 0
+    for _ in range(300):
+        sx, sy = rng.uniform(-60, 100, 2)
+        dx, dy = rng.uniform(-60, 100, 2)
+        hit, ex, ey, xx, xy = oracle.ct_chord(sx, sy, dx, dy, nx, ny)
+        t = np.linspace(0.0, 1.0, 20001)
+        ins = _inside(sx + t * (dx - sx), sy + t * (dy - sy), nx, ny)
+        if not ins.any():
+            assert not hit or math.hypot(xx - ex, xy - ey) < 1e-3
+            continue
+        assert hit
+        n_hit += 1
+        idx = np.nonzero(ins)[0]
+
+        def edge(lo, hi, want_inside_at_hi):
+            for _ in range(80):
+                mid = 0.5 * (lo + hi)
+                if bool(_inside(sx + mid * (dx - sx), sy + mid * (dy - sy), nx, ny)) == want_inside_at_hi:
+                    hi = mid
+                else:
+                    lo = mid
+            return hi
+        t0 = 0.0 if idx[0] == 0 else edge(t[idx[0] - 1], t[idx[0]], True)
+        t1 = 1.0 if idx[-1] == len(t) - 1 else edge(t[idx[-1] + 1], t[idx[-1]], True)
+        assert abs(ex - (sx + t0 * (dx - sx))) < 1e-9 and abs(ey - (sy + t0 * (dy - sy))) < 1e-9
+        assert abs(xx - (sx + t1 * (dx - sx))) < 1e-9 and abs(xy - (sy + t1 * (dy - sy))) < 1e-9
+    assert n_hit > 50
+
+
+def _np_bilinear(m, nx, ny, x, y):
+    """Zero-border bilinear of m [nx*ny] (x fastest) at texel coordinates (x, y)."""
+    xt, yt = x - 0.5, y - 0.5
+    i, j = np.floor(xt).astype(int), np.floor(yt).astype(int)
+    ax, ay = xt - i, yt - j
+    M = np.zeros((ny + 4, nx + 4))
+    M[2:ny + 2, 2:nx + 2] = m.reshape(ny, nx)
+
+    def at(ii, jj):
+        return M[np.clip(jj, -2, ny + 1) + 2, np.clip(ii, -2, nx + 1) + 2]
+    return ((1 - ax) * (1 - ay) * at(i, j) + ax * (1 - ay) * at(i + 1, j)
+            + (1 - ax) * ay * at(i, j + 1) + ax * ay * at(i + 1, j + 1))
+
+
+def test_ct_forward_beer_lambert_numpy(oracle):
+    """T recomputed in numpy: the chord sampled every ray_step from its entry (while
+    q*step <= chord length), numpy bilinear line integrals, exp(-beta L) summed over
+    each detector bin's source-energy window."""
+    p, maps, beta = _ct_case()
+    g, d = p.geometry, p.disc
+    T = oracle.forward_ct(p, maps, beta)
+    V, COL, E, DN = g.n_views, g.n_cols, d.n_energy, d.energy_halfwidth
+    RF = p.flat("spectrum").astype(np.float64).reshape(E, E)  # RF[ed, e] = spectrum[e + E*ed]
+    ref = np.zeros(V * COL * E)
+    for v in range(V):
+        for c in range(COL):
+            sg = oracle.scatter_geometry(g, v, c, 0, 0, 0)
+            hit, ex, ey, xx, xy = oracle.ct_chord(sg["src_x"], sg["src_y"], sg["det_x"],
+                                                  sg["det_y"], g.nx, g.ny)
+            assert hit
+            Lc = math.hypot(xx - ex, xy - ey)
+            q = np.arange(int(Lc / d.ray_step) + 2)
+            q = q[q * d.ray_step <= Lc]
+            px = ex + (xx - ex) / Lc * (q * d.ray_step)
+            py = ey + (xy - ey) / Lc * (q * d.ray_step)
+            L = np.array([d.ray_step * _np_bilinear(maps[b], g.nx, g.ny, px, py).sum()
+                          for b in range(maps.shape[0])])
+            P = np.exp(-(beta * L[:, None]).sum(0))
+            for ed in range(E):
+                lo = max(0, ed - DN)
+                hi = lo + 2 * DN
+                if hi >= E:
+                    hi, lo = E - 1, E - 1 - 2 * DN
+                ref[v + V * (c + COL * ed)] = np.dot(RF[ed, lo:hi + 1], P[lo:hi + 1])
+    assert np.allclose(T, ref, rtol=1e-12, atol=0)
+    # no attenuation: T is the window sum of the response matrix
+    T0 = oracle.forward_ct(p, np.zeros_like(maps), beta)
+    for ed in (0, E // 2, E - 1):
+        lo = min(max(0, ed - DN), E - 1 - 2 * DN)
+        assert np.allclose(T0[V * COL * ed:V * COL * (ed + 1)], RF[ed, lo:lo + 2 * DN + 1].sum(),
+                           rtol=1e-14)
+
+
+@pytest.mark.parametrize("full_mu", [False, True])
+def test_ct_vjp_central_differences(oracle, full_mu):
+    """<w, T(maps)> gradient against FP64 central differences, and along a random
+    direction (the directional derivative must equal <grad, direction>)."""
+    p, maps, beta = _ct_case(full_mu)
+    rng = np.random.default_rng(22)
+    w = rng.normal(size=p.geometry.n_views * p.geometry.n_cols * p.disc.n_energy)
+    gm = oracle.vjp_ct(p, maps, beta, w)
+    # beta = identity makes each map's effect small: a larger step keeps the
+    # difference above rounding (the curvature error stays below 1e-9)
+    hr, floor_ = (1e-4, 1e-2) if full_mu else (1e-6, 1e-3)
+    for b in (0, maps.shape[0] - 1):
+        for pix in np.argsort(-np.abs(gm[b]))[:3]:
+            h = hr * max(abs(maps[b, pix]), floor_)
+            mp, mm = maps.copy(), maps.copy()
+            mp[b, pix] += h
+            mm[b, pix] -= h
+            fd = np.dot(w, oracle.forward_ct(p, mp, beta) - oracle.forward_ct(p, mm, beta)) / (2 * h)
+            assert abs(fd - gm[b, pix]) <= 1e-6 * abs(gm[b, pix]), (b, pix, fd, gm[b, pix])
+    dm = rng.normal(size=maps.shape) * 1e-3 * np.abs(maps).max()
+    h = 1e-4
+    fd = np.dot(w, oracle.forward_ct(p, maps + h * dm, beta) - oracle.forward_ct(p, maps - h * dm, beta)) / (2 * h)
+    assert abs(fd - np.sum(gm * dm)) <= 1e-7 * abs(fd)
+
+
+# ---------------------------------------------------------------------------
 # committed FP64 fixtures (tests/golden/make_golden.py)
 
 @pytest.mark.parametrize("name,flags", [("reference", 3), ("small", 0)])

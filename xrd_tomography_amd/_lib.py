@@ -33,7 +33,7 @@ EXPORTS = (
     "xrd_convert_matrix_scaled", "xrd_mlem_plan_autotune", "xrd_mlem_plan_set_launch",
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
     "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
-    "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups",
+    "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups", "xrd_forward_ct", "xrd_vjp_ct",
 )
 
 
@@ -103,6 +103,8 @@ def _declare(L):
                                           _vp, _vp, _vp, _vp]),
         "xrd_poisson_nll": (ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp,
                                            ctypes.POINTER(_dbl), _vp]),
+        "xrd_forward_ct": (ctypes.c_int, [_vp, G, D, _vp, _i32, _vp, _vp, _vp, _vp]),
+        "xrd_vjp_ct": (ctypes.c_int, [_vp, G, D, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -414,6 +414,34 @@ def vjp_coeffs(spectrum, segmentation, coef, basis, maps, beta, grad_counts, *,
     return gc, gm
 
 
+def forward_ct(spectrum, maps, beta, *, geometry: Geometry, disc: Discretization, ctx=None,
+               stream=None) -> torch.Tensor:
+    """CT transmission [V*COL*NRG] (view fastest) of the attenuation maps [B, nx*ny]
+    with energy coefficients beta [B, NRG] (xrd_forward_ct)."""
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, maps, beta)
+    g, d = geometry, disc
+    out = torch.empty(g.n_views * g.n_cols * d.n_energy, dtype=torch.float32,
+                      device=spectrum.device)
+    check(lib().xrd_forward_ct(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                               maps.shape[0], _ptr(maps), _ptr(beta), _ptr(out),
+                               _stream(stream)), "xrd_forward_ct")
+    return out
+
+
+def vjp_ct(spectrum, maps, beta, grad_t, *, geometry: Geometry, disc: Discretization, ctx=None,
+           stream=None) -> torch.Tensor:
+    """grad_maps [B, nx*ny] float64 of <grad_t, forward_ct(maps)> (xrd_vjp_ct)."""
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, maps, beta, grad_t)
+    gm = torch.empty(maps.shape, dtype=torch.float64, device=spectrum.device)
+    g, d = geometry, disc
+    check(lib().xrd_vjp_ct(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                           maps.shape[0], _ptr(maps), _ptr(beta), _ptr(grad_t), _ptr(gm),
+                           _stream(stream)), "xrd_vjp_ct")
+    return gm
+
+
 def poisson_nll(counts, y, background=None, want_grad=True, ctx=None, stream=None):
     """(loss: float, grad: float32 tensor 1 - y/f or None), f = counts + background."""
     ctx = ctx or default_context(counts.device.index or 0)

@@ -228,57 +228,64 @@ __device__ __forceinline__ void lane_sample(const Ray &ray, double step, int q, 
     j0 = (int)fy;
 }
 
+// Kahan sum, for lane b's map, of the bilinear samples of one ray (the caller
+// multiplies by the step).  Lane l prepares sample q0 + l (FP64 position, stencil
+// offset, FP32 fractions); the wave then walks the 64 samples for every map.
+__device__ __forceinline__ float wave_trace_tm(const float *__restrict__ m, int nx, int ny,
+                                               double step, const Ray &ray, int lane) {
+    const int pw = nx + 2;
+    const int n = __builtin_amdgcn_readfirstlane(ray.n);
+    Kahan k;
+    for (int q0 = 0; q0 < n; q0 += kWave) {
+        int i0, j0;
+        double dax, day;
+        lane_sample(ray, step, min(q0 + lane, n - 1), i0, j0, dax, day);
+        const float fax = (float)dax, fay = (float)day;
+        // samples lie on a segment inside the grid, so i0 is in [-1, nx-1] and j0 in
+        // [-1, ny-1] (the clamp only guards the address) and the stencil's other
+        // corners sit at +1 texel and +1 row
+        const int o00 = ((min(max(i0, -1), nx - 1) + 1) + pw * (min(max(j0, -1), ny - 1) + 1)) *
+                        kTexelMaps;
+        const int cnt = min(kWave, n - q0);
+        // the loads of four samples are in flight at a time; each four are added in
+        // FP32 and their sum enters the compensated total
+        for (int u0 = 0; u0 < cnt; u0 += 4) {
+            float pv[4][4], wx[4], wy[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int u = min(u0 + kk, cnt - 1);
+                wx[kk] = read_lane(fax, u);
+                wy[kk] = read_lane(fay, u);
+                const float *p0 = m + (unsigned)(read_lane(o00, u) + lane);
+                const float *p1 = p0 + pw * kTexelMaps;
+                pv[kk][0] = p0[0];
+                pv[kk][1] = p0[kTexelMaps];
+                pv[kk][2] = p1[0];
+                pv[kk][3] = p1[kTexelMaps];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+            float s4 = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                if (u0 + kk >= cnt) break;
+                const float ax = wx[kk], ay = wy[kk], bx = 1.f - ax, by = 1.f - ay;
+                s4 += by * (bx * pv[kk][0] + ax * pv[kk][1]) + ay * (bx * pv[kk][2] + ax * pv[kk][3]);
+            }
+            k.add(s4);
+        }
+    }
+    return k.value();
+}
+
+// Detector legs of a voxel chunk, one wave per ray, one lane per map.
 __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
                                                int nk, float *Ld) {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
-    const int nx = a.f.nx, ny = a.f.ny, pw = nx + 2;
-    const float *__restrict__ m = a.maps_tm;
     for (int j = wid; j < nk; j += NT / kWave) {
         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
-                                 nx, ny, a.f.ray_step);
-        const int n = __builtin_amdgcn_readfirstlane(ray.n);
-        Kahan k;
-        for (int q0 = 0; q0 < n; q0 += kWave) {
-            // lane l prepares sample q0 + l: corner offsets and FP32 fractions
-            int i0, j0;
-            double dax, day;
-            lane_sample(ray, a.f.ray_step, min(q0 + lane, n - 1), i0, j0, dax, day);
-            const float fax = (float)dax, fay = (float)day;
-            // samples lie on the segment from the grid entry to a voxel centre, so
-            // i0 is in [-1, nx-1] and j0 in [-1, ny-1] (the clamp only guards the
-            // address) and the stencil's other corners sit at +1 texel and +1 row
-            const int o00 = ((min(max(i0, -1), nx - 1) + 1) + pw * (min(max(j0, -1), ny - 1) + 1)) *
-                            kTexelMaps;
-            const int cnt = min(kWave, n - q0);
-            // every lane then walks the 64 samples for its own map, the loads of
-            // four samples in flight at a time; each four are added in FP32 and
-            // their sum enters the compensated total
-            for (int u0 = 0; u0 < cnt; u0 += 4) {
-                float pv[4][4], wx[4], wy[4];
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const int u = min(u0 + kk, cnt - 1);
-                    wx[kk] = read_lane(fax, u);
-                    wy[kk] = read_lane(fay, u);
-                    const float *p0 = m + (unsigned)(read_lane(o00, u) + lane);
-                    const float *p1 = p0 + pw * kTexelMaps;
-                    pv[kk][0] = p0[0];
-                    pv[kk][1] = p0[kTexelMaps];
-                    pv[kk][2] = p1[0];
-                    pv[kk][3] = p1[kTexelMaps];
-                }
-                __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
-                float s4 = 0.f;
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    if (u0 + kk >= cnt) break;
-                    const float ax = wx[kk], ay = wy[kk], bx = 1.f - ax, by = 1.f - ay;
-                    s4 += by * (bx * pv[kk][0] + ax * pv[kk][1]) + ay * (bx * pv[kk][2] + ax * pv[kk][3]);
-                }
-                k.add(s4);
-            }
-        }
-        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)((double)k.value() * a.f.ray_step);
+                                 a.f.nx, a.f.ny, a.f.ray_step);
+        const float sum = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane);
+        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)((double)sum * a.f.ray_step);
     }
 }
 
@@ -941,6 +948,129 @@ __global__ __launch_bounds__(TQ) void coeff_grad_kernel(CoeffArgs a, int n_group
     }
 }
 
+// ---- CT transmission channel ---------------------------------------------------
+// The transmitted signal of the simultaneous CT + XRD scanner (README.md:14).  The
+// reference only names it, in the commented-out launches of integrate_src2det and
+// polychromatic_src2det (Forward_XRD.cu:271-276), whose kernels it does not ship:
+// for view v and detector column c the primary ray runs from the source to the
+// column's point on the detector; its part inside the grid (Liang-Barsky clipping,
+// FP64, no contraction) is sampled like attenuator.cu:44-69 from the source-side
+// entry, and
+//   T[v + V (c + COL ed)] = sum_{e in win(ed)} RF[e,ed] exp(-sum_b beta[b,e] L_b).
+
+// Clips (sx,sy)->(dx,dy) to [0,nx]x[0,ny] and samples the chord from its entry.
+__device__ __forceinline__ bool ct_chord(double sx, double sy, double dx, double dy, int nx, int ny,
+                                         double step, Ray &r) {
+#pragma clang fp contract(off)
+    const double ddx = dx - sx, ddy = dy - sy;
+    const double p[4] = {-ddx, ddx, -ddy, ddy};
+    const double q[4] = {sx - 0.0, (double)nx - sx, sy - 0.0, (double)ny - sy};
+    double t0 = 0.0, t1 = 1.0;
+    bool miss = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (p[k] == 0.0) {
+            if (q[k] < 0.0) miss = true;
+            continue;
+        }
+        const double rr = q[k] / p[k];
+        if (p[k] < 0.0) {
+            if (rr > t0) t0 = rr;
+        } else {
+            if (rr < t1) t1 = rr;
+        }
+    }
+    r.n = 0;
+    r.side = -1;
+    r.ex = r.ey = r.ux = r.uy = 0.0;
+    if (miss || !(t0 < t1)) return false;
+    r.ex = sx + ddx * t0;
+    r.ey = sy + ddy * t0;
+    const double xx = sx + ddx * t1, xy = sy + ddy * t1;
+    const double lx = xx - r.ex, ly = xy - r.ey;
+    const double L = sqrt(lx * lx + ly * ly);
+    r.ux = lx / L;
+    r.uy = ly / L;
+    if (!(L >= 0.0)) return false;
+    int n = (int)floor(L / step) + 1;
+    while ((double)n * step <= L) n++;
+    while (n > 0 && (double)(n - 1) * step > L) n--;
+    r.n = n;
+    r.side = 0;
+    return true;
+}
+
+// Per ray (one wave): lane b's line integral L_b, then lane e's exp(-mu_e).
+__device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int lane, Ray &ray,
+                                           bool &hit) {
+    const int E = a.f.n_energy, B = a.n_basis;
+    const PixG pg = pixel_setup(a, v, c);
+    hit = ct_chord(pg.sx, pg.sy, pg.dx, pg.dy, a.f.nx, a.f.ny, a.f.ray_step, ray);
+    double Lb = 0.0;
+    if (hit) Lb = (double)wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane) *
+                  a.f.ray_step;
+    const int e = min(lane, E - 1);
+    double mu = 0.0;
+    for (int b = 0; b < B; b++) mu += (double)a.beta[b * E + e] * read_lane(Lb, b);
+    return exp(-mu);
+}
+
+// One wave per (view, column) ray; 4 rays per workgroup.
+__global__ __launch_bounds__(256) void ct_forward_kernel(CoeffArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int V = a.f.n_views, COL = a.f.n_cols, E = a.f.n_energy;
+    const int ray_id = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ray_id >= V * COL) return;  // uniform per wave
+    const int v = ray_id % V, c = ray_id / V;
+    Ray ray;
+    bool hit;
+    const double P = ct_ray_p(a, v, c, lane, ray, hit);
+    // lane ed: sum over its source-energy window (all windows have 2 dn + 1 bins)
+    const int ed = min(lane, E - 1);
+    int lo, hi;
+    src_window(ed, E, a.f.dn, lo, hi);
+    double t = 0.0;
+    for (int k = 0; k <= 2 * a.f.dn; k++) {
+        const int e = lo + k;
+        const double pe = __shfl(P, e, kWave);
+        t += (double)a.f.spectrum[e + (size_t)E * ed] * pe;
+    }
+    if (lane < E) a.out_counts[(size_t)v + (size_t)V * ((size_t)c + (size_t)COL * lane)] = (float)t;
+}
+
+// VJP of <w, T> with respect to the maps: per ray gamma_b = -sum_e beta[b,e] P_e
+// sum_{ed: e in win(ed)} RF[e,ed] w[v,c,ed], scattered along the chord.
+__global__ __launch_bounds__(256) void ct_vjp_kernel(CoeffArgs a) {
+    __shared__ double gam[4][kWave];
+    __shared__ CellRun cruns[4][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int V = a.f.n_views, COL = a.f.n_cols, E = a.f.n_energy, B = a.n_basis;
+    const int ray_id = blockIdx.x * 4 + wid;
+    if (ray_id >= V * COL) return;  // uniform per wave
+    const int v = ray_id % V, c = ray_id / V;
+    Ray ray;
+    bool hit;
+    const double P = ct_ray_p(a, v, c, lane, ray, hit);
+    if (!hit) return;
+    double wt = 0.0;
+    if (lane < E)
+        for (int ed = 0; ed < E; ed++) {
+            int lo, hi;
+            src_window(ed, E, a.f.dn, lo, hi);
+            if (lane >= lo && lane <= hi)
+                wt += (double)a.f.spectrum[lane + (size_t)E * ed] *
+                      (double)a.w[(size_t)v + (size_t)V * ((size_t)c + (size_t)COL * ed)];
+        }
+    const double ge = lane < E ? wt * P : 0.0;
+    const int b = min(lane, B - 1);
+    double gb = 0.0;
+    for (int e = 0; e < E; e++) gb -= (double)a.beta[b * E + e] * read_lane(ge, e);
+    gam[wid][lane] = lane < B ? gb : 0.0;
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    scatter_ray_tm(a, ray, gam[wid], lane, cruns[wid]);
+}
+
 __global__ void zero_f64_kernel(double *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -1026,6 +1156,23 @@ hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
         if (a.grad_maps && a.grad_tm)
             hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_ct(const CoeffArgs &a, bool vjp, hipStream_t s) {
+    const int rays = a.f.n_views * a.f.n_cols;
+    const dim3 grid((rays + 3) / 4);
+    if (vjp) {
+        hipLaunchKernelGGL(ct_vjp_kernel, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(ct_forward_kernel, grid, dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ct_prepare(const CoeffArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(coeff_pad_tm_kernel, dim3(8192), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

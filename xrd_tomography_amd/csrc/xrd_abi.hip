@@ -961,6 +961,76 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
     return XRD_OK;
 }
 
+namespace {
+
+int ct_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+             const float *spectrum, int32_t n_basis, const float *maps, const float *beta,
+             CoeffArgs &ca, hipStream_t s, const char *fn) {
+    int rc = check_geometry(g, d, fn);
+    if (rc) return rc;
+    if (!spectrum || !maps || !beta)
+        return fail(XRD_ERR_NULL_POINTER, "%s: a required input pointer is NULL", fn);
+    if (n_basis <= 0 || n_basis > 64)
+        return fail(XRD_ERR_UNSUPPORTED, "%s: need 1 <= n_basis <= 64", fn);
+    if (d->n_energy > 64) return fail(XRD_ERR_UNSUPPORTED, "%s: need n_energy <= 64", fn);
+    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 >= ((size_t)1 << 31))
+        return fail(XRD_ERR_UNSUPPORTED, "%s: (nx+2)(ny+2)*64 must stay below 2^31", fn);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(ctx->cm_tm.ensure(coeff_texel_floats(g->nx, g->ny) * sizeof(float)), "workspace alloc");
+    ForwardArgs fa;
+    rc = setup_forward_common(ctx, g, d, spectrum, fa, s);
+    if (rc) return rc;
+    ca = CoeffArgs{};
+    ca.f = fa;
+    ca.n_basis = n_basis;
+    ca.maps = maps;
+    ca.beta = beta;
+    ca.maps_tm = (float *)ctx->cm_tm.ptr;
+    ca.nvox_cap = g->nx * g->ny;
+    HIP_TRY(launch_ct_prepare(ca, s), "CT map preparation");
+    return XRD_OK;
+}
+
+}  // namespace
+
+extern "C" int xrd_forward_ct(xrd_context *ctx, const xrd_geometry *g,
+                              const xrd_discretization *d, const float *spectrum,
+                              int32_t n_basis, const float *maps, const float *beta, float *out_t,
+                              void *stream) {
+    const char *fn = "xrd_forward_ct";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!out_t) return fail(XRD_ERR_NULL_POINTER, "%s: out_t is NULL", fn);
+    hipStream_t s = (hipStream_t)stream;
+    CoeffArgs ca;
+    int rc = ct_setup(ctx, g, d, spectrum, n_basis, maps, beta, ca, s, fn);
+    if (rc) return rc;
+    ca.out_counts = out_t;
+    HIP_TRY(launch_ct(ca, false, s), "CT forward kernel");
+    return XRD_OK;
+}
+
+extern "C" int xrd_vjp_ct(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                          const float *spectrum, int32_t n_basis, const float *maps,
+                          const float *beta, const float *grad_t, double *grad_maps,
+                          void *stream) {
+    const char *fn = "xrd_vjp_ct";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (!grad_t || !grad_maps)
+        return fail(XRD_ERR_NULL_POINTER, "%s: grad_t or grad_maps is NULL", fn);
+    hipStream_t s = (hipStream_t)stream;
+    CoeffArgs ca;
+    int rc = ct_setup(ctx, g, d, spectrum, n_basis, maps, beta, ca, s, fn);
+    if (rc) return rc;
+    const size_t npix = (size_t)g->nx * g->ny;
+    HIP_TRY(ctx->cm_gtm.ensure(npix * 64 * sizeof(double)), "workspace alloc");
+    ca.grad_tm = (double *)ctx->cm_gtm.ptr;
+    HIP_TRY(launch_zero_f64(ca.grad_tm, npix * 64, s), "zero kernel");
+    ca.w = grad_t;
+    ca.grad_maps = grad_maps;
+    HIP_TRY(launch_ct(ca, true, s), "CT VJP kernel");
+    return XRD_OK;
+}
+
 extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
                                const float *background, const float *y, float *grad_counts,
                                double *out_loss_host, void *stream) {

@@ -127,6 +127,8 @@ hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
 size_t coeff_hpart_doubles(int n_views, int npix);
 bool coeff_texel_major(int n_basis);
+hipError_t launch_ct_prepare(const CoeffArgs &a, hipStream_t s);
+hipError_t launch_ct(const CoeffArgs &a, bool vjp, hipStream_t s);
 size_t coeff_texel_floats(int nx, int ny);
 hipError_t launch_zero_f64(double *p, size_t n, hipStream_t s);
 hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, const float *y,
