@@ -439,6 +439,14 @@ def run_vjp(args, dev):
     b = float((xr.double() * gc.double()).sum())
     dot_rel = abs(a - b) / max(abs(a), abs(b))
     log(f"[vjp] dot test <Jx,w> {a:.9e} <x,J'w> {b:.9e} rel {dot_rel:.2e}")
+    # CT transmission channel on the same maps: forward and map VJP
+    t_ct, t_ctv = None, None
+    if g.n_rows == 1:
+        tct, t_ct = timed(lambda: api.forward_ct(sp, maps, beta, **kw))
+        wct = torch.randn(tct.shape, generator=gen).to(dev)
+        _, t_ctv = timed(lambda: api.vjp_ct(sp, maps, beta, wct, **kw))
+        log(f"[vjp] CT forward {t_ct * 1e3:.2f} ms, CT map VJP {t_ctv * 1e3:.2f} ms "
+            f"({g.n_views * g.n_cols} rays)")
 
     # CPU baseline: the FP64 oracle on the central pixel of one view, rays/s
     cpu = None
@@ -466,7 +474,7 @@ def run_vjp(args, dev):
         "forward_s": round(t_fwd, 3), "vjp_coef_s": round(t_coef, 3),
         "vjp_coef_maps_s": round(t_all, 3), "poisson_nll_s": round(t_nll, 4),
         "coef_grad_same_in_both": bool(torch.equal(gc, gc2)),
-        "dot_test_rel": dot_rel, "cpu_baseline": cpu,
+        "dot_test_rel": dot_rel, "ct_forward_s": t_ct, "ct_vjp_s": t_ctv, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# The builder tests read the reference checkout's physics tables where it exists
+# (they are skipped elsewhere).
+os.environ.setdefault("XRD_REFERENCE_DIR", "/root/reference")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X) and the built HIP library")
