@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=36, help="views in the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="repeat the CPU-baseline update until this much time has passed")
+    ap.add_argument("--cpu-a-cols", type=int, default=8,
+                    help="detector columns of the one-view CPU A-build sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp-views", type=int, default=360,
                     help="views of the vjp256 workload (the coefficient-model VJP)")
@@ -170,6 +172,7 @@ def main():
     # ---- error vs FP64 and CPU baseline on a view subsample (whole views, all rows of them)
     rel_err = None
     cpu = None
+    cpu_a = None
     if not args.no_cpu:
         from oracle import oracle as O
         nv = max(1, min(args.cpu_views, g.n_views))
@@ -221,6 +224,19 @@ def main():
         log(f"[bench] cpu baseline {cpu_iters_s:.3f} iters/s on {threads} threads; "
             f"GPU-vs-FP64 MLEM rel err after {k_err} steps {rel_err:.2e}")
         del A64
+        # A-build baseline: the FP64 oracle forward (N, C and A) on one view and a few
+        # detector columns of the same phantom, scaled to all views and columns (the
+        # cost per (view, column) row is the same)
+        pa = make_problem(args.workload, n_views=1, n_cols=args.cpu_a_cols)
+        t0 = time.perf_counter()
+        O.forward(pa, with_A=True)
+        dt_a = time.perf_counter() - t0
+        k_a = g.n_views * g.n_cols / args.cpu_a_cols
+        cpu_a = {"value": round(dt_a * k_a, 1), "unit": "s per A build (N, C and A)",
+                 "cores": threads, "kind": "port",
+                 "sample": f"FP64 oracle forward on 1 view x {args.cpu_a_cols} detector "
+                           f"columns of the same phantom ({dt_a:.2f} s), scaled x{k_a:.0f}"}
+        log(f"[bench] cpu A build {dt_a * k_a:.0f} s (scaled) against {a_build_s:.2f} s")
 
     out = {
         "metric": METRIC,
@@ -251,6 +267,7 @@ def main():
         "cpu_baseline": cpu,
         "rel_err_vs_fp64": rel_err,
         "a_build_s": round(a_build_s, 3),
+        "cpu_a_build": cpu_a,
         "bytes_per_step": bytes_step * passes,
     }
     plan.close()
