@@ -152,7 +152,10 @@ int xrd_version(void);
  *  out_A        plhs[2]  float [NWT x MAT x SRC x COL x ROW x NRG] model matrix; may be
  *                        NULL to skip it.
  * Row index r = src + SRC*(col + COL*(row + ROW*e_det)); column j = nwt + NWT*mat.
- * Requirements: 16-byte aligned pointers; n_energy > 2*energy_halfwidth.        */
+ * Requirements: 16-byte aligned pointers; n_energy > 2*energy_halfwidth.
+ * Synchronisation: the call waits once on `stream` for the non-air voxel count,
+ * which sizes the view batches of its per-(view, pixel, voxel) workspace (at most
+ * 4 GiB); the kernels after that are enqueued asynchronously.                    */
 int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
                 const float *spectrum, const float *ifine, const int32_t *segmentation,
                 const float *a1, const float *a2, const float *s2_table,
