@@ -86,6 +86,16 @@ def test_forward_reference_geometry(xrd, oracle, flags):
     _compare(f"reference flags={flags}", Ng, Cg, Ag, No, Co, Ao)
 
 
+@pytest.mark.parametrize("name,rows", [("small", 3), ("reference", 2)])
+def test_forward_multi_row_detector(xrd, oracle, name, rows):
+    """ROW > 1 (integrate_diffraction.cu:39-40,100-109): rows stacked in z."""
+    from xrd_tomography_amd.synthetic import make_problem
+    prob = make_problem(name, n_rows=rows, flags=3 if name == "reference" else 0)
+    Ng, Cg, Ag = _run(xrd, prob)
+    No, Co, Ao = oracle.forward(prob, nthreads=4)
+    _compare(f"{name} rows={rows}", Ng, Cg, Ag, No, Co, Ao)
+
+
 def test_forward_fan64_view_subset(xrd, oracle):
     """64x64 voxels, 60 views, 64 pixels, 61 energies, 128 q-bins (3 views checked)."""
     from xrd_tomography_amd.synthetic import make_problem

@@ -221,6 +221,44 @@ def test_geometry_hand_triangle(oracle):
     assert abs(theta - np.arccos(cos_t)) < 0.02  # the half-voxel offset changes theta little
 
 
+def test_geometry_detector_rows(oracle):
+    """Row r of the detector sits at height det_z0 + r*row_pitch
+    (integrate_diffraction.cu:100-109): cos(theta) of each row from hand vectors."""
+    g = _geom(n_rows=4)
+    R, Rd, w = g.r_src, g.r_det, g.wedge_angle
+    col = 5
+    rdet = (col - g.n_cols / 2 + 0.5) * g.col_pitch
+    a = np.array([31.5 - 32.0, 31.5 - (32.0 - R), 0.0])
+    a[2] = -a[1] * np.tan(w / 2)
+    for row in range(4):
+        o = oracle.scatter_geometry(g, 0, col, row, 31, 31)
+        b = np.array([32.0 + rdet - 31.5, 32.0 + Rd - 31.5,
+                      g.det_z0 + row * g.row_pitch - a[2]])
+        cos_t = np.dot(a, b) / np.linalg.norm(a) / np.linalg.norm(b)
+        assert abs(o["cos_theta"] - cos_t) < 1e-14, row
+        assert abs(o["det_x"] - (32.0 + rdet)) < 1e-12 and abs(o["det_y"] - (32.0 + Rd)) < 1e-12
+
+
+def test_forward_rows_layout(oracle):
+    """Row r of a multi-row detector (output row v + V*(col + COL*(row + ROW*e))) is the
+    one-row detector placed at height det_z0 + r*row_pitch."""
+    import dataclasses
+    from xrd_tomography_amd.descriptors import Geometry
+    from xrd_tomography_amd.synthetic import make_problem
+    p1 = make_problem("small")
+    g3 = Geometry(**p1.geometry.as_dict())
+    g3.n_rows = 3
+    N3, C3, A3 = oracle.forward(dataclasses.replace(p1, geometry=g3), nthreads=2)
+    V, COL, E = g3.n_views, g3.n_cols, p1.disc.n_energy
+    for r in range(3):
+        gr = Geometry(**p1.geometry.as_dict())
+        gr.det_z0 = p1.geometry.det_z0 + r * p1.geometry.row_pitch
+        N1, C1, A1 = oracle.forward(dataclasses.replace(p1, geometry=gr), nthreads=2)
+        assert np.array_equal(N3.reshape(E, 3, COL, V)[:, r].reshape(-1), N1)
+        assert np.array_equal(C3.reshape(E, 3, COL, V)[:, r].reshape(-1), C1)
+        assert np.array_equal(A3.reshape(E, 3, COL, V, -1)[:, r].reshape(A1.shape), A1)
+
+
 def test_geometry_rotation_symmetry(oracle):
     """A quarter turn of the scanner (V/4 views) maps voxel offset (dx, dy) to (-dy, dx);
     every scalar of the scattering geometry is unchanged."""

@@ -151,18 +151,21 @@ def compton_table(nq: int, na: int, lut):
 
 def fan_geometry(nx: int, ny: int, voxel_mm: float, n_views: int, n_cols: int,
                  sid_mm: float = 600.0, sdd_mm: float = 1000.0,
-                 theta_min_deg: float = 2.0, theta_max_deg: float = 8.0) -> Geometry:
+                 theta_min_deg: float = 2.0, theta_max_deg: float = 8.0,
+                 n_rows: int = 1) -> Geometry:
     """Fan-beam scanner with the XRD panel opposite the source (the reference's
     arrangement, integrate_diffraction.cu:94-121) at source-isocentre sid_mm and
     source-detector sdd_mm; one row at height z0 and COL columns spanning the
-    lateral offsets for which isocentre rays scatter in [theta_min, theta_max]."""
+    lateral offsets for which isocentre rays scatter in [theta_min, theta_max];
+    further rows (n_rows > 1) stack upwards at a 1 mm pitch, as the reference places
+    its rows (integrate_diffraction.cu:100-109)."""
     iso_det = sdd_mm - sid_mm
     z0 = iso_det * math.tan(math.radians(theta_min_deg))
     rho_max = iso_det * math.tan(math.radians(theta_max_deg))
     r_max = math.sqrt(rho_max ** 2 - z0 ** 2)
     col_pitch = 2.0 * r_max / n_cols
     v = voxel_mm
-    return Geometry(nx=nx, ny=ny, n_views=n_views, n_cols=n_cols, n_rows=1,
+    return Geometry(nx=nx, ny=ny, n_views=n_views, n_cols=n_cols, n_rows=n_rows,
                     center_x=nx / 2.0, center_y=ny / 2.0,
                     r_src=sid_mm / v, r_det=iso_det / v,
                     view_angle0=0.0, view_angle_step=2.0 * math.pi / n_views,
@@ -231,8 +234,8 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
                  (tests).
     ``reference`` : the reference's own scanner literals (integrate_diffraction.cu:81-140)
                  on a 40x40 grid with 3 materials, 4 views, 16 pixels, 16 energies.
-    Keyword overrides replace sizes (nx, n_views, n_cols, n_energy, n_q, n_qf, n_mat,
-    flags, voxel_mm, photons_per_view, mlem_iterations).
+    Keyword overrides replace sizes (nx, n_views, n_cols, n_rows, n_energy, n_q, n_qf,
+    n_mat, flags, voxel_mm, photons_per_view, mlem_iterations).
     """
     presets = {
         "fan64": dict(nx=64, voxel_mm=4.0, n_mat=4, n_views=60, n_cols=64, n_energy=61,
@@ -259,13 +262,15 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
     M = int(p["n_mat"])
 
     if name == "reference":
-        g = Geometry.reference(nx, ny, n_views=int(p["n_views"]), n_cols=int(p["n_cols"]))
+        g = Geometry.reference(nx, ny, n_views=int(p["n_views"]), n_cols=int(p["n_cols"]),
+                               n_rows=int(p.get("n_rows", 1)))
         d = Discretization.reference(n_energy=int(p["n_energy"]), n_q=int(p["n_q"]),
                                      n_qf=int(p["n_qf"]), n_mat=M, flags=flags)
         # the reference phantom occupies [0,NX]^2 while its rotation centre is at
         # (NX,NY) (integrate_diffraction.cu:88-95); keep inclusions inside the grid.
     else:
-        g = fan_geometry(nx, ny, vmm, int(p["n_views"]), int(p["n_cols"]))
+        g = fan_geometry(nx, ny, vmm, int(p["n_views"]), int(p["n_cols"]),
+                         n_rows=int(p.get("n_rows", 1)))
         d = scaled_discretization(M, n_energy=int(p["n_energy"]), n_q=int(p["n_q"]),
                                   n_qf=int(p["n_qf"]), flags=flags)
 
