@@ -332,8 +332,8 @@ int xrd_mlem_batch_iterate(xrd_mlem_batch *batch, int32_t n_iter, int use_graph,
  * Waits for the device, then re-partitions the rows. */
 int xrd_mlem_batch_set_launch(xrd_mlem_batch *batch, int32_t blocks_per_cu,
                               int32_t rows_per_wave);
-/* For every row-span group (problems are launched per column span 256, 512, 1024
- * or 2048), times that group's streaming launch (reps launches, x untouched) over
+/* For every row-span group (problems are launched per column span 256, 384, 512,
+ * 768, 1024, 1536 or 2048), times that group's streaming launch (reps launches, x untouched) over
  * blocks_per_cu in {1,2,3,4} (clamped to what fits on a CU) and rows_per_wave in
  * {1,2,4} and keeps the fastest.  Returns the shape of the group with the largest
  * matrices and the mean time of a whole pass in ms (outputs may be NULL). */

@@ -21,7 +21,7 @@ def _problem(seed=0, rows=20000, cols=512):
     return A, y, c
 
 
-@pytest.mark.parametrize("cols", [48, 512, 1024, 1536])
+@pytest.mark.parametrize("cols", [48, 384, 512, 640, 1024, 1280, 1536])
 def test_project_backproject_sensitivity(xrd, oracle, cols):
     import torch
     A, y, c = _problem(1, rows=3001, cols=cols)
@@ -106,7 +106,7 @@ def test_mlem_deterministic(xrd):
 
 @pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
 @pytest.mark.parametrize("cols,offset", [(512, 0), (1024, 0), (516, 0), (1028, 0), (512, 4),
-                                         (2048, 0)])
+                                         (2048, 0), (768, 0), (1280, 0), (1284, 4)])
 def test_reduced_precision_matrix(xrd, dtype, cols, offset):
     """BF16/F16 storage of A: operators equal the FP32 operators on the rounded matrix.
     Column counts that are multiples of 8 on 16-byte aligned storage take the 16-byte
@@ -160,7 +160,7 @@ def test_bf24_conversion_rounds_to_16_significant_bits(xrd):
     assert got[0, 3] == 1.0 and got[0, 4] == 1.0 + 4 * 2.0 ** -16  # ties to even
 
 
-@pytest.mark.parametrize("cols", [512, 1024, 2048])
+@pytest.mark.parametrize("cols", [512, 640, 1024, 1280, 2048])
 def test_bf24_operators_and_mlem(xrd, oracle, cols):
     """BF24 matrices: A x, A' w, sum(A) and MLEM equal the FP32 operators on the decoded
     matrix, and MLEM is within 2e-5 of the FP64 oracle on the decoded matrix."""
@@ -227,7 +227,9 @@ def test_scaled_reduced_precision_mlem(xrd, oracle, dtype):
 
 
 @pytest.mark.parametrize("dtype,wide", [("float32", 1024), ("float16", 1024),
-                                        ("float16", 1020), ("bfloat16", 1024), ("bf24", 1024)])
+                                        ("float16", 1020), ("bfloat16", 1024), ("bf24", 1024),
+                                        ("float32", 640), ("bf24", 1280), ("float16", 1276),
+                                        ("bf24", 384)])
 def test_batched_mlem_matches_single(xrd, oracle, dtype, wide):
     """Problems of uneven rows and columns in one persistent launch: each x equals the
     single-problem plan's x (1e-6) and the FP64 oracle's (2e-5 on the stored matrix).

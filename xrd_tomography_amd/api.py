@@ -515,7 +515,7 @@ class MlemBatch:
 
     def groups(self) -> list:
         """[{span, grid, rows_per_wave}] of the row-span groups, one launch each."""
-        n = 4
+        n = 8
         sp, gr, rw = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
         k = lib().xrd_mlem_batch_groups(self.handle, n, sp, gr, rw)
         if k < 0:

@@ -1,14 +1,14 @@
 // batch.hip -- MLEM for many independent problems (phantoms or slices of uneven
 // size and view count) with persistent launches over all of them.
 //
-// Problems are grouped by row span (the column count rounded up to 256, 512,
-// 1024 or 2048, which fixes how many lanes share a row).  For each group the
+// Problems are grouped by row span (the column count rounded up to 256, 384, 512,
+// 768, 1024, 1536 or 2048, which fixes how many lanes share a row).  For each group the
 // problems' rows are laid end to end and cut into one contiguous chunk per
 // workgroup of about equal cost (bytes of A plus a per-row charge for the row's
 // reduction and ratio); a chunk is a short list of segments, each a row range of
 // one problem.  One persistent launch per group streams every segment and writes
-// its FP64 column sums to the segment's own partial slot; a pass is at most four
-// such launches (one per span present) and one update launch.  The update kernel
+// its FP64 column sums to the segment's own partial slot; a pass is one such
+// launch per span present and one update launch.  The update kernel
 // adds, per problem, the partials of its segments in segment order, so the result
 // depends only on the launch shape, never on timing.  A kernel per span (instead
 // of one kernel branching on the span) keeps each at the registers of its own
@@ -87,8 +87,11 @@ template <typename T, int MODE>
 static const void *batch_kernel_span(int span, int R) {
     switch (span) {
         case 256: return batch_kernel_r<T, 256, MODE>(R);
+        case 384: return batch_kernel_r<T, 384, MODE>(R);
         case 512: return batch_kernel_r<T, 512, MODE>(R);
+        case 768: return batch_kernel_r<T, 768, MODE>(R);
         case 1024: return batch_kernel_r<T, 1024, MODE>(R);
+        case 1536: return batch_kernel_r<T, 1536, MODE>(R);
         case 2048: return batch_kernel_r<T, 2048, MODE>(R);
     }
     return nullptr;

@@ -147,7 +147,8 @@ enum { kColsOnes = 0, kColsWeighted = 1, kColsMlem = 2 };
 // groups.  SPAN is a power of two in [256, 2048]; a lane holds 16 columns (32 for
 // SPAN 2048), which bounds its FP64 column sums to 32 (64) registers.
 template <int W, int SPAN> struct RowShape {
-    static constexpr int L = SPAN / 16 < 16 ? 16 : (SPAN / 16 > 64 ? 64 : SPAN / 16);
+    // a power of two in [16, 64] lanes per row, about SPAN/16
+    static constexpr int L = SPAN / 16 >= 64 ? 64 : (SPAN / 16 >= 32 ? 32 : 16);
     static constexpr int NG = SPAN / (L * W);
     static constexpr int S = kWave / L;
     static_assert(L * NG * W == SPAN, "row shape must tile the span");

@@ -251,13 +251,17 @@ int mat_num_blocks(const MatArgs &m) {
 
 static int64_t rows_per_block(const MatArgs &m, int nb) { return (m.n_rows + nb - 1) / nb; }
 
-// Instantiates the row span covering m.n_cols (256, 512, 1024 or 2048 columns).
+// Instantiates the row span covering m.n_cols (256, 384, 512, 768, 1024, 1536 or 2048
+// columns).
 #define XRD_DISPATCH_SPAN(SPANV, ...)                                  \
     do {                                                               \
         switch (row_span(m.n_cols)) {                                  \
             case 256: { constexpr int SPANV = 256; __VA_ARGS__; break; }   \
+            case 384: { constexpr int SPANV = 384; __VA_ARGS__; break; }   \
             case 512: { constexpr int SPANV = 512; __VA_ARGS__; break; }   \
+            case 768: { constexpr int SPANV = 768; __VA_ARGS__; break; }   \
             case 1024: { constexpr int SPANV = 1024; __VA_ARGS__; break; } \
+            case 1536: { constexpr int SPANV = 1536; __VA_ARGS__; break; } \
             case 2048: { constexpr int SPANV = 2048; __VA_ARGS__; break; } \
             default: return hipErrorInvalidValue;                      \
         }                                                              \

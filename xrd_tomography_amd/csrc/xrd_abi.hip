@@ -1066,7 +1066,7 @@ struct BatchShape {
 struct xrd_mlem_batch {
     xrd_context *ctx = nullptr;
     BatchArgs args{};
-    BatchShape shape[4];
+    BatchShape shape[kNumSpans];
     int esize = 4;
     std::vector<BatchProblem> host_probs;
     DevBuf probs, segs, wg_seg, partial, sens;
@@ -1166,7 +1166,8 @@ hipError_t batch_set_grid(xrd_mlem_batch *b) {
     std::vector<int> wg;
     a.n_groups = 0;
     const int cus = device_cus();
-    for (int si = 0, span = 256; span <= kMaxMatCols; si++, span *= 2) {
+    for (int si = 0; si < kNumSpans; si++) {
+        const int span = span_of(si);
         std::vector<int> members;
         for (int k = 0; k < (int)probs.size(); k++)
             if (probs[k].span == span) members.push_back(k);
@@ -1379,8 +1380,7 @@ extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *st
     const int bpcs[] = {1, 2, 3, 4};
     const int rows[] = {1, 2, 4};
     for (int g = 0; g < b->args.n_groups; g++) {
-        int si = 0;
-        while ((256 << si) != b->args.groups[g].span) si++;
+        const int si = span_index(b->args.groups[g].span);
         BatchShape best_shape = b->shape[si];
         float best = 1e30f;
         for (int r : rows) {
@@ -1414,8 +1414,7 @@ extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *st
             main_g = g;
         }
     }
-    int si = 0;
-    while (b->args.n_groups > 0 && (256 << si) != b->args.groups[main_g].span) si++;
+    const int si = b->args.n_groups > 0 ? span_index(b->args.groups[main_g].span) : 0;
     if (out_blocks_per_cu) *out_blocks_per_cu = b->shape[si].blocks_per_cu;
     if (out_rows_per_wave) *out_rows_per_wave = b->shape[si].rows_per_wave;
     if (out_ms) *out_ms = total;

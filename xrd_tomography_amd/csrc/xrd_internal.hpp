@@ -155,10 +155,24 @@ constexpr int kMaxBlocksPerCu = 8;
 constexpr int kMaxMatCols = 2048;
 // Row span of the streaming kernels for n_cols columns: the smallest of 256, 512,
 // 1024 and 2048 that is >= n_cols (0 if wider).
+// Row spans of the streaming kernels (the span fixes the lanes per row and the
+// columns per lane): a matrix uses the smallest span that holds its columns.  The
+// 384, 768 and 1536 spans keep the lanes of a matrix whose column count falls
+// between two powers of two mostly busy.
+constexpr int kNumSpans = 7;
+__host__ __device__ constexpr int span_of(int i) {
+    return i == 0 ? 256 : i == 1 ? 384 : i == 2 ? 512 : i == 3 ? 768 : i == 4 ? 1024
+                                                        : i == 5 ? 1536 : 2048;
+}
 __host__ __device__ constexpr int row_span(int n_cols) {
-    int span = 256;
-    while (span < n_cols) span *= 2;
-    return span <= kMaxMatCols ? span : 0;
+    for (int i = 0; i < kNumSpans; i++)
+        if (n_cols <= span_of(i)) return span_of(i);
+    return 0;
+}
+__host__ __device__ constexpr int span_index(int span) {
+    for (int i = 0; i < kNumSpans; i++)
+        if (span_of(i) == span) return i;
+    return -1;
 }
 
 // Per-column maxima of |A| (FP32 matrix) and power-of-two scales that map each
@@ -198,7 +212,7 @@ struct BatchProblem {
     const void *A;        // [n_rows x n_cols], dtype of the batch
     int64_t n_rows;
     int n_cols;           // multiple of 4
-    int span;             // row span (256, 512, 1024 or 2048) >= n_cols
+    int span;             // row span (span_of(i)) >= n_cols
     int seg_begin, seg_end;  // this problem's segments in the global list
     int sens_offset;      // offset of its sensitivity in the batch's sensitivity buffer
     int pad;
@@ -231,7 +245,7 @@ struct BatchArgs {
     int n_problems, n_segs, max_cols, stride, dtype;
     int vec8;             // 2-byte matrices: every n_cols % 8 == 0 and A 16-byte aligned
     int n_groups;
-    BatchGroup groups[4];  // spans 256, 512, 1024, 2048 that occur, ascending
+    BatchGroup groups[kNumSpans];  // the spans that occur, ascending
     double *partial;      // [n_segs x stride]
 };
 
