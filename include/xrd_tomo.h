@@ -162,6 +162,24 @@ int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
                 double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
                 float *out_counts, float *out_compton, float *out_A, void *stream);
 
+/* Slice `slice` of a volume: segmentation, a1 and a2 are [nx x ny x n_slices]
+ * (x fastest, then y, then the slice).  The slice's voxels scatter exactly as in
+ * xrd_forward.  The attenuation legs are 3-D segments through the whole volume: the
+ * source leg from the focal spot at the slice centre (height slice + 0.5 voxels) to
+ * the voxel at its wedge height (integrate_diffraction.cu:199-205), the detector leg
+ * from the pixel (slice centre + det_z0 + row*row_pitch) to the voxel.  Each is
+ * sampled at the attenuator's 2-D sample positions (attenuator.cu:44-69), at the
+ * segment's height there, trilinear with zero outside the volume.  No reference
+ * counterpart: the reference scans one slice with attenuation constant in z, which
+ * is n_slices = 1 here (identical to xrd_forward).  Outputs as xrd_forward, for the
+ * one slice.  Limit: (nx+2)(ny+2)(n_slices+2) <= 2^31. */
+int xrd_forward_slice(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                      const float *spectrum, const float *ifine, const int32_t *segmentation,
+                      const float *a1, const float *a2, int32_t n_slices, int32_t slice,
+                      const float *s2_table, double lut_q_min, double lut_q_max,
+                      double lut_a_min, double lut_a_max, float *out_counts, float *out_compton,
+                      float *out_A, void *stream);
+
 /* ---- ray traversal (the GetIntersection + attenuator sampling, on its own) ----
  * For `n` segments (x1,y1)->(x2,y2) given in `seg` as 4 doubles each, computes
  * in FP64 the entry point into [0,nx]x[0,ny] with GetIntersection's rules

@@ -130,7 +130,7 @@ def wvt_value(n, n_q, q_min, q_max, dq0):
 
 
 GEOM_FIELDS = ("src_x", "src_y", "det_x", "det_y", "cos_theta", "sin_half", "anorm2",
-               "bnorm2", "res2", "geom", "rayleigh")
+               "bnorm2", "res2", "geom", "rayleigh", "vox_z", "det_z")
 
 
 def scatter_geometry(g: Geometry, view, col, row, ix, iy):
@@ -328,3 +328,48 @@ def vjp_ct(prob, maps, beta, w, nthreads=0):
     if rc:
         raise RuntimeError(f"xo_vjp_ct failed: {rc}")
     return gm.reshape(B, g.nx * g.ny)
+
+
+def line_integrals_3d(vols, nx, ny, nz, step, x1, y1, z1, x2, y2, z2):
+    """Line integrals of volumes [nb, nx*ny*nz] (x fastest) along a 3-D segment."""
+    L = lib()
+    L.xo_line_integrals_3d.restype = None
+    L.xo_line_integrals_3d.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_double] + [ctypes.c_double] * 6 + [_dp]
+    v = np.ascontiguousarray(np.asarray(vols, np.float64).reshape(-1))
+    nb = v.size // (nx * ny * nz)
+    out = np.zeros(nb)
+    L.xo_line_integrals_3d(v.ctypes.data_as(_dp), nb, nx, ny, nz, step, x1, y1, z1, x2, y2, z2,
+                           out.ctypes.data_as(_dp))
+    return out
+
+
+def forward_slice(vp, n_slices, slice_, views=None, with_A=True, nthreads=0):
+    """Oracle forward model of slice slice_ of a volume problem (segmentation, a1, a2
+    [nx, ny, n_slices]).  Returns (N, C, A) like forward()."""
+    g, d = vp.geometry, vp.disc
+    vl = np.arange(g.n_views, dtype=np.int32) if views is None else np.asarray(views, np.int32)
+    nvl = len(vl)
+    rows = nvl * g.n_cols * g.n_rows * d.n_energy
+    N = np.zeros(rows)
+    C = np.zeros(rows)
+    A = np.zeros(rows * d.n_q * d.n_mat) if with_A else None
+    sp, spp = _d(vp.flat("spectrum"))
+    fi, fip = _d(vp.flat("ifine"))
+    sg, sgp = _i(vp.flat("segmentation"))
+    a1, a1p = _d(vp.flat("a1"))
+    a2, a2p = _d(vp.flat("a2"))
+    s2, s2p = _d(vp.flat("s2_table"))
+    vv, vvp = _i(vl)
+    L = lib()
+    L.xo_forward_slice.restype = ctypes.c_int
+    rc = L.xo_forward_slice(ctypes.byref(g), ctypes.byref(d), spp, fip, sgp, a1p, a2p,
+                            ctypes.c_int(n_slices), ctypes.c_int(slice_), s2p,
+                            *[ctypes.c_double(float(v)) for v in vp.lut], vvp, ctypes.c_int(nvl),
+                            N.ctypes.data_as(_dp), C.ctypes.data_as(_dp),
+                            A.ctypes.data_as(_dp) if with_A else None, ctypes.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"xo_forward_slice failed: {rc}")
+    if with_A:
+        A = A.reshape(rows, d.n_q * d.n_mat)
+    return N, C, A

@@ -177,6 +177,43 @@ void xo_line_integrals(const double *maps, int nb, int nx, int ny, double step, 
     xo_line_integrals_from(maps, nb, nx, ny, step, ex, ey, x2, y2, out);
 }
 
+/* Trilinear value of volume [nx x ny x nz] at (x,y,z), zero outside, voxel (i,j,k)
+ * centred at (i+0.5, j+0.5, k+0.5). */
+static double xo_trilinear(const double *vol, int nx, int ny, int nz, double x, double y,
+                           double z) {
+    const double zt = z - 0.5;
+    const double fk = floor(zt);
+    const double az = zt - fk;
+    const long k = (long)fk;
+    const long plane = (long)nx * ny;
+    const double v0 = (k >= 0 && k < nz) ? xo_bilinear(vol + k * plane, nx, ny, x, y) : 0.0;
+    const double v1 = (k + 1 >= 0 && k + 1 < nz) ? xo_bilinear(vol + (k + 1) * plane, nx, ny, x, y)
+                                                 : 0.0;
+    return (1.0 - az) * v0 + az * v1;
+}
+
+void xo_line_integrals_3d(const double *vols, int nb, int nx, int ny, int nz, double step,
+                          double x1, double y1, double z1, double x2, double y2, double z2,
+                          double *out) {
+    for (int b = 0; b < nb; b++) out[b] = 0.0;
+    double ex, ey;
+    if (xo_get_intersection(x1, y1, x2, y2, nx, ny, &ex, &ey) < 0) return;
+    double ux, uy, L;
+    xo_ray_dir(ex, ey, x2, y2, &ux, &uy, &L);
+    const int32_t n = xo_sample_count(L, step);
+    const double hx = x2 - x1, hy = y2 - y1;
+    const double H = sqrt(hx * hx + hy * hy);
+    const long vsize = (long)nx * ny * nz;
+    for (int32_t q = 0; q < n; q++) {
+        const double qr = step * (double)q;
+        const double px = ex + ux * qr;
+        const double py = ey + uy * qr;
+        const double pz = z2 + (z1 - z2) * ((L - qr) / H);
+        for (int b = 0; b < nb; b++) out[b] += xo_trilinear(vols + b * vsize, nx, ny, nz, px, py, pz);
+    }
+    for (int b = 0; b < nb; b++) out[b] *= step;
+}
+
 /* find_src_nrg_range.cu:8-25 (the window is shifted, never truncated). */
 void xo_find_src_nrg_range(int nrg, int n_energy, int halfwidth, int *lo, int *hi) {
     int mn = nrg - halfwidth;
@@ -258,6 +295,8 @@ void xo_scatter_geometry(const xrd_geometry *g, int view, int col, int row, int 
     o->res2 = sS2 + vV2 + dD2;
     o->geom = g_sv * g_vd;
     o->rayleigh = (1.0 + cth * cth) / 2.0;
+    o->vox_z = vz;
+    o->det_z = dz;
 }
 
 /* floor(t) clamped to [0, n-1]; NaN -> 0 (the clamps of integrate_diffraction.cu:187-189,
@@ -297,11 +336,61 @@ typedef struct {
     double a1c, a2c; /* (a1,a2) used by the Compton term */
 } xo_voxel;
 
+/* Volume attenuation for xo_forward_impl: vol = [a1 volume, a2 volume] of nz
+ * slices; the scattering slice sits at z_slice (slice centre). */
+typedef struct {
+    const double *vol;
+    int nz;
+    double z_slice;
+} xo_volume;
+
+static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
+                           const double *spectrum, const double *ifine, const int32_t *seg,
+                           const double *a1, const double *a2, const double *s2_table,
+                           double lut_q_min, double lut_q_max, double lut_a_min,
+                           double lut_a_max, const int32_t *views, int nvl, double *out_N,
+                           double *out_C, double *out_A, int nthreads, const xo_volume *volm);
+
 int xo_forward(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
                const double *ifine, const int32_t *seg, const double *a1, const double *a2,
                const double *s2_table, double lut_q_min, double lut_q_max, double lut_a_min,
                double lut_a_max, const int32_t *views, int nvl, double *out_N, double *out_C,
                double *out_A, int nthreads) {
+    return xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min, lut_q_max,
+                           lut_a_min, lut_a_max, views, nvl, out_N, out_C, out_A, nthreads, NULL);
+}
+
+int xo_forward_slice(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                     const double *ifine, const int32_t *seg_vol, const double *a1_vol,
+                     const double *a2_vol, int n_slices, int slice, const double *s2_table,
+                     double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
+                     const int32_t *views, int nvl, double *out_N, double *out_C,
+                     double *out_A, int nthreads) {
+    if (n_slices <= 0 || slice < 0 || slice >= n_slices) return XRD_ERR_DIMS;
+    const long plane = (long)g->nx * g->ny;
+    const int32_t *seg = seg_vol + plane * slice;
+    const double *a1 = a1_vol + plane * slice, *a2 = a2_vol + plane * slice;
+    if (n_slices == 1)
+        return xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min,
+                               lut_q_max, lut_a_min, lut_a_max, views, nvl, out_N, out_C, out_A,
+                               nthreads, NULL);
+    double *vol = (double *)malloc(sizeof(double) * 2 * plane * n_slices);
+    memcpy(vol, a1_vol, sizeof(double) * plane * n_slices);
+    memcpy(vol + plane * n_slices, a2_vol, sizeof(double) * plane * n_slices);
+    xo_volume vm = {vol, n_slices, (double)slice + 0.5};
+    const int rc = xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min,
+                                   lut_q_max, lut_a_min, lut_a_max, views, nvl, out_N, out_C,
+                                   out_A, nthreads, &vm);
+    free(vol);
+    return rc;
+}
+
+static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
+                           const double *spectrum, const double *ifine, const int32_t *seg,
+                           const double *a1, const double *a2, const double *s2_table,
+                           double lut_q_min, double lut_q_max, double lut_a_min,
+                           double lut_a_max, const int32_t *views, int nvl, double *out_N,
+                           double *out_C, double *out_A, int nthreads, const xo_volume *volm) {
     const int NX = g->nx, NY = g->ny, V = g->n_views, COL = g->n_cols, ROW = g->n_rows;
     const int DET = COL * ROW;
     const int E = d->n_energy, DN = d->energy_halfwidth, Q = d->n_q, QF = d->n_qf;
@@ -377,8 +466,13 @@ int xo_forward(const xrd_geometry *g, const xrd_discretization *d, const double 
         const int vi = (int)(t / (nvox > 0 ? nvox : 1)), k = (int)(t % (nvox > 0 ? nvox : 1));
         xo_scatter_geom sg;
         xo_scatter_geometry(g, vlist[vi], 0, 0, vox[k].ix, vox[k].iy, &sg);
-        xo_line_integrals(maps, 2, NX, NY, d->ray_step, sg.src_x, sg.src_y, vox[k].ix + 0.5,
-                          vox[k].iy + 0.5, &Ls[2 * t]);
+        if (volm)  /* the focal spot at the slice centre, the voxel at its wedge height */
+            xo_line_integrals_3d(volm->vol, 2, NX, NY, volm->nz, d->ray_step, sg.src_x, sg.src_y,
+                                 volm->z_slice, vox[k].ix + 0.5, vox[k].iy + 0.5,
+                                 volm->z_slice + sg.vox_z, &Ls[2 * t]);
+        else
+            xo_line_integrals(maps, 2, NX, NY, d->ray_step, sg.src_x, sg.src_y, vox[k].ix + 0.5,
+                              vox[k].iy + 0.5, &Ls[2 * t]);
     }
 
 #pragma omp parallel num_threads(nt)
@@ -403,8 +497,13 @@ int xo_forward(const xrd_geometry *g, const xrd_discretization *d, const double 
                 xo_scatter_geom sg;
                 xo_scatter_geometry(g, view, col, row, vx->ix, vx->iy, &sg);
                 double Ld[2];
-                xo_line_integrals(maps, 2, NX, NY, d->ray_step, sg.det_x, sg.det_y,
-                                  vx->ix + 0.5, vx->iy + 0.5, Ld);
+                if (volm)
+                    xo_line_integrals_3d(volm->vol, 2, NX, NY, volm->nz, d->ray_step, sg.det_x,
+                                         sg.det_y, volm->z_slice + sg.det_z, vx->ix + 0.5,
+                                         vx->iy + 0.5, volm->z_slice + sg.vox_z, Ld);
+                else
+                    xo_line_integrals(maps, 2, NX, NY, d->ray_step, sg.det_x, sg.det_y,
+                                      vx->ix + 0.5, vx->iy + 0.5, Ld);
                 const double *Lsrc = &Ls[2 * ((long)vi * nvox + k)];
                 /* Leg feeding Pin and the leg feeding the Compton Pout
                  * (integrate_diffraction.cu:259-260, :295-296, :347). */

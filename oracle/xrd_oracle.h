@@ -43,6 +43,15 @@ int xo_trace_ray(double x1, double y1, double x2, double y2, int nx, int ny, dou
 void xo_line_integrals(const double *maps, int nb, int nx, int ny, double step,
                        double x1, double y1, double x2, double y2, double *out);
 
+/* Line integrals of nb volumes [nb x nx x ny x nz] along the 3-D segment
+ * (x1,y1,z1)->(x2,y2,z2): the attenuator's samples of the 2-D projection
+ * (x1,y1)->(x2,y2) (ROI entry, then q*step while q*step <= |P2 - entry|), each at
+ * the height of the segment there (z linear in the 2-D distance to P2), trilinear
+ * with zero outside the volume. */
+void xo_line_integrals_3d(const double *vols, int nb, int nx, int ny, int nz, double step,
+                          double x1, double y1, double z1, double x2, double y2, double z2,
+                          double *out);
+
 /* Bilinear value of map [nx x ny] at texel coordinates (x,y), zero border,
  * texel (i,j) centred at (i+0.5, j+0.5) (tex2D semantics, attenuator.cu:58). */
 double xo_bilinear(const double *map, int nx, int ny, double x, double y);
@@ -66,10 +75,25 @@ typedef struct xo_scatter_geom {
     double res2;                        /* sS2 + vV2 + dD2                        */
     double geom;                        /* geom_src2vox * geom_vox2det            */
     double rayleigh;                    /* (1 + cos^2 theta)/2                    */
+    double vox_z, det_z;                /* heights of the voxel (wedge) and pixel */
 } xo_scatter_geom;
 
 void xo_scatter_geometry(const xrd_geometry *g, int view, int col, int row, int ix, int iy,
                          xo_scatter_geom *out);
+
+/* The forward model of slice `slice` of an n_slices-slice volume (seg_vol, a1_vol,
+ * a2_vol [nx x ny x n_slices]): the slice's voxels scatter with the arithmetic of
+ * xo_forward; the source leg runs from the focal spot at the slice centre
+ * (slice + 0.5) to the voxel at its wedge height, the detector leg from the pixel
+ * (slice centre + det_z0 + row*row_pitch) to the voxel, both through the volume
+ * (xo_line_integrals_3d).  n_slices = 1 is xo_forward (attenuation constant in z,
+ * as in the reference). */
+int xo_forward_slice(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
+                     const double *ifine, const int32_t *seg_vol, const double *a1_vol,
+                     const double *a2_vol, int n_slices, int slice, const double *s2_table,
+                     double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
+                     const int32_t *views, int nvl, double *out_N, double *out_C,
+                     double *out_A, int nthreads);
 
 /* Full forward model (integrate_diffraction.cu:9-409 with the launch of
  * Forward_XRD.cu:244-259) for the views listed in `views` (n_views_list of them;

@@ -96,6 +96,35 @@ def test_forward_multi_row_detector(xrd, oracle, name, rows):
     _compare(f"{name} rows={rows}", Ng, Cg, Ag, No, Co, Ao)
 
 
+def _host(prob, N, C, A):
+    import torch
+    torch.cuda.synchronize()
+    g, d = prob.geometry, prob.disc
+    return (N.cpu().numpy().reshape(-1), C.cpu().numpy().reshape(-1),
+            A.cpu().numpy().reshape(-1, d.n_mat * d.n_q))
+
+
+@pytest.mark.parametrize("slice_", [0, 2])
+def test_forward_volume_slice(xrd, oracle, slice_):
+    """A slice of a 4-slice volume: both attenuation legs are 3-D segments through the
+    volume, sampled trilinearly (xrd_forward_slice against xo_forward_slice)."""
+    from xrd_tomography_amd.synthetic import make_volume_problem
+    vp = make_volume_problem("small", n_slices=4)
+    Ng, Cg, Ag = _host(vp, *xrd.forward_volume_slice(vp, 4, slice_))
+    No, Co, Ao = oracle.forward_slice(vp, 4, slice_, nthreads=4)
+    _compare(f"volume slice {slice_}/4", Ng, Cg, Ag, No, Co, Ao)
+
+
+def test_forward_slice_of_one_is_forward(xrd):
+    import torch
+    from xrd_tomography_amd.synthetic import make_problem
+    p = make_problem("small")
+    N0, C0, A0 = xrd.forward_problem(p)
+    N1, C1, A1 = xrd.forward_volume_slice(p, 1, 0)
+    torch.cuda.synchronize()
+    assert torch.equal(N0, N1) and torch.equal(C0, C1) and torch.equal(A0, A1)
+
+
 def test_forward_fan64_view_subset(xrd, oracle):
     """64x64 voxels, 60 views, 64 pixels, 61 energies, 128 q-bins (3 views checked)."""
     from xrd_tomography_amd.synthetic import make_problem

@@ -512,6 +512,70 @@ def test_vjp_central_differences(oracle):
 
 
 # ---------------------------------------------------------------------------
+# volume slices: 3-D attenuation legs (trilinear), pinned by the z-invariant limit,
+# a numpy recomputation of the 3-D leg and the one-slice identity
+
+def _z_invariant_volume(p, nz):
+    import dataclasses
+    rep = lambda a: np.repeat(a[:, :, None], nz, axis=2)  # noqa: E731
+    return dataclasses.replace(p, segmentation=rep(p.segmentation), a1=rep(p.a1), a2=rep(p.a2))
+
+
+def test_volume_constant_in_z_is_the_2d_model(oracle):
+    """Identical slices: every leg stays inside the volume, the trilinear blend of equal
+    slices is the bilinear value, and the slice's counts are the 2-D model's."""
+    from xrd_tomography_amd.synthetic import make_problem
+    p = make_problem("small")
+    v = _z_invariant_volume(p, 12)
+    N0, C0, A0 = oracle.forward(p, nthreads=2)
+    N1, C1, A1 = oracle.forward_slice(v, 12, 5, nthreads=2)
+    for a, b in ((N1, N0), (C1, C0), (A1, A0)):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.abs(b).max()
+    N2, C2, A2 = oracle.forward_slice(_z_invariant_volume(p, 1), 1, 0, nthreads=2)
+    assert np.array_equal(N2, N0) and np.array_equal(C2, C0) and np.array_equal(A2, A0)
+
+
+def test_line_integrals_3d_numpy(oracle):
+    """3-D legs recomputed in numpy: the attenuator's 2-D samples from the ROI entry,
+    z linear in the 2-D distance to the end point, trilinear with a zero border."""
+    rng = np.random.default_rng(31)
+    nx, ny, nz, step = 17, 13, 6, 0.25
+    vol = rng.uniform(0.0, 1.0, (2, nz, ny, nx))        # [b, k, j, i]: x fastest in memory
+    pad = np.zeros((2, nz + 2, ny + 2, nx + 2))
+    pad[:, 1:-1, 1:-1, 1:-1] = vol
+
+    def tri(px, py, pz):
+        xt, yt, zt = px - 0.5, py - 0.5, pz - 0.5
+        i, j, k = np.floor(xt).astype(int), np.floor(yt).astype(int), np.floor(zt).astype(int)
+        ax, ay, az = xt - i, yt - j, zt - k
+        out = 0.0
+        for di, wx in ((0, 1 - ax), (1, ax)):
+            for dj, wy in ((0, 1 - ay), (1, ay)):
+                for dk, wz in ((0, 1 - az), (1, az)):
+                    ii = np.clip(i + di, -1, nx) + 1
+                    jj = np.clip(j + dj, -1, ny) + 1
+                    kk = np.clip(k + dk, -1, nz) + 1
+                    out = out + wx * wy * wz * pad[:, kk, jj, ii]
+        return out
+    for _ in range(40):
+        x2, y2 = rng.uniform(0.5, nx - 0.5), rng.uniform(0.5, ny - 0.5)
+        ang = rng.uniform(0, 2 * np.pi)
+        x1, y1 = x2 + 60 * np.cos(ang), y2 + 60 * np.sin(ang)
+        z1, z2 = rng.uniform(-1, nz + 1), rng.uniform(0.5, nz - 0.5)
+        got = oracle.line_integrals_3d(vol, nx, ny, nz, step, x1, y1, z1, x2, y2, z2)
+        side, ex, ey = oracle.get_intersection(x1, y1, x2, y2, nx, ny)
+        L = np.hypot(x2 - ex, y2 - ey)
+        H = np.hypot(x2 - x1, y2 - y1)
+        q = np.arange(int(L / step) + 2)
+        q = q[q * step <= L]
+        t = q * step
+        px, py = ex + (x2 - ex) / L * t, ey + (y2 - ey) / L * t
+        pz = z2 + (z1 - z2) * ((L - t) / H)
+        ref = step * tri(px, py, pz).sum(axis=1)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-14), (got, ref)
+
+
+# ---------------------------------------------------------------------------
 # CT transmission channel: chord by bisection, Beer-Lambert recomputed in numpy,
 # the zero-attenuation closed form and FP64 central differences for the VJP
 

@@ -34,6 +34,7 @@ EXPORTS = (
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
     "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
     "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups", "xrd_forward_ct", "xrd_vjp_ct",
+    "xrd_forward_slice",
 )
 
 
@@ -62,6 +63,8 @@ def _declare(L):
         "xrd_version": (ctypes.c_int, []),
         "xrd_forward": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp]),
+        "xrd_forward_slice": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
+                                             _vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp]),
         "xrd_trace_rays": (ctypes.c_int, [_vp, _i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp,
                                           _i32, _vp, _i32, _vp, _vp, _vp]),
         "xrd_project": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp, _vp, _vp]),

@@ -155,6 +155,38 @@ def Forward_XRD(spectrum: torch.Tensor, Ifine: torch.Tensor, segmentation: torch
     return N, C, A
 
 
+def forward_slice(spectrum, Ifine, segmentation, a1, a2, S2_grid, q_min, q_max, a_min, a_max,
+                  n_slices: int, slice_: int, *, geometry: Geometry, disc: Discretization,
+                  with_A: bool = True, ctx: Optional[Context] = None, stream=None):
+    """xrd_forward_slice: slice slice_ of a volume (segmentation, a1, a2 flat
+    [NX*NY*n_slices] device tensors, x fastest); returns (N, C, A) of that slice."""
+    g, d = geometry, disc
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid)
+    shape_n = (d.n_energy, g.n_rows, g.n_cols, g.n_views)
+    dev = spectrum.device
+    N = torch.empty(shape_n, dtype=torch.float32, device=dev)
+    C = torch.empty(shape_n, dtype=torch.float32, device=dev)
+    A = torch.empty(shape_n + (d.n_mat, d.n_q), dtype=torch.float32, device=dev) if with_A else None
+    rc = lib().xrd_forward_slice(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                                 _ptr(Ifine), _ptr(segmentation), _ptr(a1), _ptr(a2), n_slices,
+                                 slice_, _ptr(S2_grid), float(q_min), float(q_max), float(a_min),
+                                 float(a_max), _ptr(N), _ptr(C), _ptr(A), _stream(stream))
+    check(rc, "xrd_forward_slice")
+    return N, C, A
+
+
+def forward_volume_slice(vp, n_slices: int, slice_: int, device="cuda", with_A=True, ctx=None):
+    """forward_slice on a volume Problem (segmentation, a1, a2 [nx, ny, n_slices])."""
+    def dev(name, dt):
+        return torch.from_numpy(vp.flat(name)).to(device=device, dtype=dt)
+    return forward_slice(dev("spectrum", torch.float32), dev("ifine", torch.float32),
+                         dev("segmentation", torch.int32), dev("a1", torch.float32),
+                         dev("a2", torch.float32), dev("s2_table", torch.float32), *vp.lut,
+                         n_slices, slice_, geometry=vp.geometry, disc=vp.disc, with_A=with_A,
+                         ctx=ctx)
+
+
 def forward_problem(prob, device="cuda", with_A=True, ctx=None, disc=None):
     """Run Forward_XRD on a synthetic.Problem; returns (N, C, A) device tensors."""
     def dev(name, dt):

@@ -140,6 +140,30 @@ __global__ __launch_bounds__(1024) void prepare_list_kernel(PrepArgs a) {
     if (threadIdx.x == 0) a.mat_offsets[a.n_mat] = off;
 }
 
+__global__ void prepare_volume_kernel(const float *a1, const float *a2, int nx, int ny, int nz,
+                                      float2 *mu_vol) {
+    const int pw = nx + 2, ph = ny + 2;
+    const size_t n = (size_t)pw * ph * (nz + 2);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(i % pw), py = (int)((i / pw) % ph), pz = (int)(i / ((size_t)pw * ph));
+        float2 v = make_float2(0.f, 0.f);
+        if (px >= 1 && px <= nx && py >= 1 && py <= ny && pz >= 1 && pz <= nz) {
+            const size_t vox = (size_t)(px - 1) + (size_t)nx * ((py - 1) + (size_t)ny * (pz - 1));
+            v.x = a1[vox];
+            v.y = a2[vox];
+        }
+        mu_vol[i] = v;
+    }
+}
+
+hipError_t launch_prepare_volume(const float *a1, const float *a2, int nx, int ny, int nz,
+                                 float2 *mu_vol, hipStream_t s) {
+    hipLaunchKernelGGL(prepare_volume_kernel, dim3(4096), dim3(256), 0, s, a1, a2, nx, ny, nz,
+                       mu_vol);
+    return hipGetLastError();
+}
+
 hipError_t launch_prepare(const PrepArgs &a, hipStream_t s) {
     const int n = (a.nx + 2) * (a.ny + 2);
     int blocks = (n + 255) / 256;
@@ -161,15 +185,35 @@ __global__ __launch_bounds__(256) void source_legs_kernel(ForwardArgs a, float2 
     const VoxelEntry ve = a.voxels[k];
     const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
     const ViewConst vc = a.views[v];
-    const Ray r = make_ray(vc.src_x, vc.src_y, (double)ix + 0.5, (double)iy + 0.5, a.nx, a.ny,
-                           a.ray_step);
+    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
+    const Ray r = make_ray(vc.src_x, vc.src_y, vx, vy, a.nx, a.ny, a.ray_step);
     Kahan s1, s2;
-    for (int q = 0; q < r.n; q++) {
-        double px, py;
-        ray_sample(r, a.ray_step, q, px, py);
-        const float2 t = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
-        s1.add(t.x);
-        s2.add(t.y);
+    if (a.nz > 1) {
+        // 3-D leg: the focal spot at the slice centre, the voxel at its wedge height
+        // (integrate_diffraction.cu:199-205)
+#pragma clang fp contract(off)
+        const double ax = vx - vc.src_x, ay = vy - vc.src_y;
+        const double nsx = -vc.sp * a.sta, nsy = vc.cp * a.sta;
+        const double proj = (ax * nsx + ay * nsy) / sqrt(nsx * nsx + nsy * nsy);
+        const double vz = -proj * a.tan_w2;
+        const double lx = vx - r.ex, ly = vy - r.ey;
+        const double L = sqrt(lx * lx + ly * ly), H = sqrt(ax * ax + ay * ay);
+        for (int q = 0; q < r.n; q++) {
+            double px, py;
+            ray_sample(r, a.ray_step, q, px, py);
+            const double pz = leg_height(a.z_slice, a.z_slice + vz, L, H, a.ray_step, q);
+            const float2 t = trilinear2(a.mu_vol, a.nx, a.ny, a.nz, px, py, pz);
+            s1.add(t.x);
+            s2.add(t.y);
+        }
+    } else {
+        for (int q = 0; q < r.n; q++) {
+            double px, py;
+            ray_sample(r, a.ray_step, q, px, py);
+            const float2 t = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
+            s1.add(t.x);
+            s2.add(t.y);
+        }
     }
     src_int[(size_t)v * npix + k] =
         make_float2((float)((double)s1.value() * a.ray_step), (float)((double)s2.value() * a.ray_step));
@@ -316,6 +360,7 @@ __device__ __forceinline__ PixelGeom pixel_geometry(const ForwardArgs &a, int v,
 // FP64 scattering geometry (integrate_diffraction.cu:192-255), written as one
 // record for forward_kernel.  Consecutive threads take consecutive voxels of the
 // material-major list, so their rays run side by side through the map.
+template <bool VOLUME>
 __global__ __launch_bounds__(256) void pixel_voxel_kernel(ForwardArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nvox) return;
@@ -323,15 +368,35 @@ __global__ __launch_bounds__(256) void pixel_voxel_kernel(ForwardArgs a) {
     const PixelGeom pg = pixel_geometry(a, a.v0 + vb, p);
     const VoxelEntry ve = a.voxels[k];
     const int ix = ve.pix % a.nx, iy = ve.pix / a.nx;
-    const Ray ray = make_ray(pg.dx, pg.dy, (double)ix + 0.5, (double)iy + 0.5, a.nx, a.ny,
-                             a.ray_step);
+    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
+    const Ray ray = make_ray(pg.dx, pg.dy, vx, vy, a.nx, a.ny, a.ray_step);
     Kahan s1, s2;
-    for (int q = 0; q < ray.n; q++) {
-        double px, py;
-        ray_sample(ray, a.ray_step, q, px, py);
-        const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
-        s1.add(tv.x);
-        s2.add(tv.y);
+    if (VOLUME) {
+        // 3-D leg: from the pixel (slice centre + row height) down or up to the voxel
+        // at its wedge height
+#pragma clang fp contract(off)
+        const double ax = vx - pg.sx, ay = vy - pg.sy;
+        const double proj = (ax * pg.nsx + ay * pg.nsy) / sqrt(pg.nsx * pg.nsx + pg.nsy * pg.nsy);
+        const double vz = -proj * a.tan_w2;
+        const double bx = pg.dx - vx, by = pg.dy - vy;
+        const double lx = vx - ray.ex, ly = vy - ray.ey;
+        const double L = sqrt(lx * lx + ly * ly), H = sqrt(bx * bx + by * by);
+        for (int q = 0; q < ray.n; q++) {
+            double px, py;
+            ray_sample(ray, a.ray_step, q, px, py);
+            const double pz = leg_height(a.z_slice + pg.dz, a.z_slice + vz, L, H, a.ray_step, q);
+            const float2 tv = trilinear2(a.mu_vol, a.nx, a.ny, a.nz, px, py, pz);
+            s1.add(tv.x);
+            s2.add(tv.y);
+        }
+    } else {
+        for (int q = 0; q < ray.n; q++) {
+            double px, py;
+            ray_sample(ray, a.ray_step, q, px, py);
+            const float2 tv = bilinear2(a.mu_padded, a.nx, a.ny, px, py);
+            s1.add(tv.x);
+            s2.add(tv.y);
+        }
     }
     VoxShared vs;
     voxel_geometry(a, pg, ix, iy, vs);
@@ -346,7 +411,10 @@ __global__ __launch_bounds__(256) void pixel_voxel_kernel(ForwardArgs a) {
 hipError_t launch_pixel_voxel(const ForwardArgs &a, hipStream_t s) {
     if (a.nvox <= 0) return hipSuccess;
     dim3 grid((a.nvox + 255) / 256, a.n_det, a.nvb);
-    hipLaunchKernelGGL(pixel_voxel_kernel, grid, dim3(256), 0, s, a);
+    if (a.nz > 1)
+        hipLaunchKernelGGL(pixel_voxel_kernel<true>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(pixel_voxel_kernel<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ struct DevBuf {
 struct xrd_context {
     int device = 0;
     // forward workspace
-    DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables, pv;
+    DevBuf mu_padded, s2u8, cpx, ref_list, voxels, offsets, src_int, tables, pv, mu_vol;
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
@@ -190,7 +190,7 @@ int xrd_context_destroy(xrd_context *c) {
     (void)hipSetDevice(c->device);
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
-                      &c->src_int, &c->tables, &c->pv, &c->partial, &c->small, &c->cm_maps,
+                      &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial, &c->small, &c->cm_maps,
                       &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
@@ -204,7 +204,7 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     if (!c) return 0;
     int64_t s = 0;
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
-                            &c->offsets, &c->src_int, &c->tables, &c->pv, &c->partial,
+                            &c->offsets, &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial,
                             &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h,
                             &c->cm_tm, &c->cm_gtm})
         s += (int64_t)b->bytes;
@@ -365,14 +365,19 @@ int setup_forward_common(xrd_context *ctx, const xrd_geometry *g, const xrd_disc
 
 }  // namespace
 
-extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
-                           const float *spectrum, const float *ifine,
-                           const int32_t *segmentation, const float *a1, const float *a2,
-                           const float *s2_table, double lut_q_min, double lut_q_max,
-                           double lut_a_min, double lut_a_max, float *out_counts,
-                           float *out_compton, float *out_A, void *stream) {
-    const char *fn = "xrd_forward";
+namespace {
+
+// xrd_forward for the slice `slice` of an n_slices-slice volume (segmentation, a1,
+// a2 point at the volumes); n_slices = 1 is the 2-D model.
+int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                 const float *spectrum, const float *ifine, const int32_t *seg_vol,
+                 const float *a1_vol, const float *a2_vol, int n_slices, int slice,
+                 const float *s2_table, double lut_q_min, double lut_q_max, double lut_a_min,
+                 double lut_a_max, float *out_counts, float *out_compton, float *out_A,
+                 void *stream, const char *fn) {
     if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    const int32_t *segmentation = seg_vol;
+    const float *a1 = a1_vol, *a2 = a2_vol;
     int rc = check_geometry(g, d, fn);
     if (rc) return rc;
     if (!spectrum || !ifine || !segmentation || !a1 || !a2 || !s2_table || !out_counts ||
@@ -393,13 +398,27 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
     if (d->n_energy > 64 || d->n_q > 256 || d->n_qf > 768)
         return fail(XRD_ERR_UNSUPPORTED,
                     "%s: this build supports n_energy <= 64, n_q <= 256, n_qf <= 768", fn);
+    if (n_slices <= 0 || slice < 0 || slice >= n_slices)
+        return fail(XRD_ERR_DIMS, "%s: need 0 <= slice < n_slices", fn);
+    if ((size_t)(g->nx + 2) * (g->ny + 2) * (size_t)(n_slices + 2) > ((size_t)1 << 31))
+        return fail(XRD_ERR_DIMS, "%s: volume too large", fn);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
 
     const int nx = g->nx, ny = g->ny, V = g->n_views;
     const size_t npix = (size_t)nx * ny;
     const size_t npad = (size_t)(nx + 2) * (ny + 2);
+    segmentation += npix * slice;
+    a1 += npix * slice;
+    a2 += npix * slice;
     HIP_TRY(ctx->mu_padded.ensure(npad * sizeof(float2)), "workspace alloc");
+    if (n_slices > 1) {
+        HIP_TRY(ctx->mu_vol.ensure(npad * (size_t)(n_slices + 2) * sizeof(float2)),
+                "workspace alloc");
+        HIP_TRY(launch_prepare_volume(a1_vol, a2_vol, nx, ny, n_slices, (float2 *)ctx->mu_vol.ptr,
+                                      s),
+                "volume preparation");
+    }
     HIP_TRY(ctx->s2u8.ensure((size_t)d->lut_nq * d->lut_na), "workspace alloc");
     HIP_TRY(ctx->cpx.ensure(2 * npix * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->ref_list.ensure(npix * sizeof(int)), "workspace alloc");
@@ -442,6 +461,9 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
     fa.out_n = out_counts;
     fa.out_c = out_compton;
     fa.out_a = out_A;
+    fa.nz = n_slices;
+    fa.z_slice = (double)slice + 0.5;
+    fa.mu_vol = n_slices > 1 ? (const float2 *)ctx->mu_vol.ptr : nullptr;
     HIP_TRY(launch_source_integrals(fa, (float2 *)ctx->src_int.ptr, s), "source-leg kernel");
     // the non-air voxel count sizes the per-(view, pixel, voxel) records: views are
     // processed in batches whose records fit in kPixelVoxelBudget bytes
@@ -472,6 +494,31 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
         HIP_TRY(e, "forward kernel");
     }
     return XRD_OK;
+}
+
+}  // namespace
+
+extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                           const float *spectrum, const float *ifine,
+                           const int32_t *segmentation, const float *a1, const float *a2,
+                           const float *s2_table, double lut_q_min, double lut_q_max,
+                           double lut_a_min, double lut_a_max, float *out_counts,
+                           float *out_compton, float *out_A, void *stream) {
+    return forward_impl(ctx, g, d, spectrum, ifine, segmentation, a1, a2, 1, 0, s2_table,
+                        lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts, out_compton,
+                        out_A, stream, "xrd_forward");
+}
+
+extern "C" int xrd_forward_slice(xrd_context *ctx, const xrd_geometry *g,
+                                 const xrd_discretization *d, const float *spectrum,
+                                 const float *ifine, const int32_t *segmentation, const float *a1,
+                                 const float *a2, int32_t n_slices, int32_t slice,
+                                 const float *s2_table, double lut_q_min, double lut_q_max,
+                                 double lut_a_min, double lut_a_max, float *out_counts,
+                                 float *out_compton, float *out_A, void *stream) {
+    return forward_impl(ctx, g, d, spectrum, ifine, segmentation, a1, a2, n_slices, slice,
+                        s2_table, lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts,
+                        out_compton, out_A, stream, "xrd_forward_slice");
 }
 
 extern "C" int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step,

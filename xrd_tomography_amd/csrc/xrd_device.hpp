@@ -128,6 +128,33 @@ __device__ __forceinline__ float2 bilinear2(const float2 *__restrict__ padded, i
     return v;
 }
 
+// Trilinear sample of the padded (a1,a2) volume [(nx+2) x (ny+2) x (nz+2)] at
+// (px, py, pz): bilinear in the two bracketing slices (each with its zero
+// border), blended in z; slices outside [0, nz) are the zero border slices.
+__device__ __forceinline__ float2 trilinear2(const float2 *__restrict__ vol, int nx, int ny, int nz,
+                                             double px, double py, double pz) {
+    const double zt = pz - 0.5;
+    const double fz = floor(zt);
+    const float az = (float)(zt - fz);
+    const int k0 = (int)fz;
+    const int k0c = min(max(k0, -1), nz) + 1, k1c = min(max(k0 + 1, -1), nz) + 1;
+    const size_t plane = (size_t)(nx + 2) * (ny + 2);
+    const float2 v0 = bilinear2(vol + plane * k0c, nx, ny, px, py);
+    const float2 v1 = bilinear2(vol + plane * k1c, nx, ny, px, py);
+    const float bz = 1.f - az;
+    return make_float2(bz * v0.x + az * v1.x, bz * v0.y + az * v1.y);
+}
+
+// Height of sample q of a leg traced from (x1,y1,z1) to the voxel (x2,y2,z2): the
+// segment's z, linear in the 2-D distance L - q*step left to the voxel (L the
+// distance from the ROI entry, H the whole 2-D length).
+__device__ __forceinline__ double leg_height(double z1, double z2, double L, double H, double step,
+                                             int q) {
+#pragma clang fp contract(off)
+    const double qr = step * (double)q;
+    return z2 + (z1 - z2) * ((L - qr) / H);
+}
+
 // ---- wave reductions (wave64) ---------------------------------------------------
 
 __device__ __forceinline__ float wave_sum(float v) {

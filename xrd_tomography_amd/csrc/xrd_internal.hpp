@@ -67,6 +67,12 @@ struct ForwardArgs {
     double *pv_s, *pv_cth;  // sin(theta/2), cos(theta)
     float4 *pv_geo;         // resolution terms As, Bs and geometry weights gR, gp
     float2 *pv_leg;         // detector-leg integrals (a1, a2)
+    // volume attenuation (nz > 1): both legs are 3-D segments through the padded
+    // [(nx+2) x (ny+2) x (nz+2)] (a1,a2) volume; the scattering slice's centre is at
+    // height z_slice.  nz <= 1: the 2-D map, constant in z.
+    const float2 *mu_vol;
+    int nz;
+    double z_slice;
 };
 
 struct PrepArgs {
@@ -85,6 +91,9 @@ struct PrepArgs {
 };
 
 hipError_t launch_prepare(const PrepArgs &a, hipStream_t s);
+// (a1, a2) volume [nx x ny x nz] into the padded volume with a zero border.
+hipError_t launch_prepare_volume(const float *a1, const float *a2, int nx, int ny, int nz,
+                                 float2 *mu_vol, hipStream_t s);
 hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStream_t s);
 hipError_t launch_pixel_voxel(const ForwardArgs &a, hipStream_t s);
 // Bytes of one (view, pixel, voxel) record, and the workspace the records of one

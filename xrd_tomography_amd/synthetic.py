@@ -304,6 +304,34 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
                    mlem_iterations=int(p["mlem_iterations"]), voxel_mm=vmm)
 
 
+def make_volume_problem(name: str = "small", n_slices: int = 4, seed: Optional[int] = None,
+                        **overrides) -> Problem:
+    """A volume of n_slices slices (one voxel apart) for xrd_forward_slice: the
+    geometry, discretisation, spectrum, form factors and Compton table of workload
+    `name`; every slice is painted with its own seeded phantom, and the materials'
+    attenuation is one table for the whole volume.  segmentation, a1 and a2 are
+    [nx, ny, n_slices]."""
+    base = make_problem(name, seed=seed, **overrides)
+    M = base.disc.n_mat
+    s0 = 0 if seed is None else int(seed)
+    rng = np.random.Generator(np.random.PCG64(1_000_003 + s0))
+    rho, a, b = _materials(M, rng)
+    vox_cm = base.voxel_mm / 10.0
+    a1m, a2m = rho * a * vox_cm, rho * b * vox_cm
+    nx, ny = base.geometry.nx, base.geometry.ny
+    seg = np.zeros((nx, ny, n_slices), np.int32)
+    a1 = np.zeros((nx, ny, n_slices))
+    a2 = np.zeros((nx, ny, n_slices))
+    for k in range(n_slices):
+        sk = make_problem(name, seed=s0 * 1000 + 101 + k, **overrides).segmentation
+        seg[:, :, k] = sk
+        for m in range(M):
+            a1[:, :, k][sk == m + 1] = a1m[m]
+            a2[:, :, k][sk == m + 1] = a2m[m]
+    return dataclasses.replace(base, name=f"{name}-volume{n_slices}", segmentation=seg,
+                               a1=a1.astype(np.float32), a2=a2.astype(np.float32))
+
+
 def poisson_counts(expected: np.ndarray, seed: int) -> np.ndarray:
     """Poisson noise (the poissrnd of main.m:141), seeded."""
     rng = np.random.Generator(np.random.PCG64(seed))
