@@ -119,12 +119,17 @@ def main():
     A = A.view(nrow, ncol)
     N = N.reshape(-1)
     C = C.reshape(-1)
-    # photon scale: expected N + c totals photons_per_view per view; the Compton
-    # calibration factor (the XS_grid_edges(6)/255 of main.m:68) is set so that the
-    # background is 30% of the diffraction signal.
-    n_sum = float(N.double().sum())
-    kappa = 0.3 * n_sum / max(float(C.double().sum()), 1e-300)
-    scale = prob.photons_per_view * g.n_views / (n_sum * 1.3)
+    if prob.photons_per_view == 0:
+        # absolute photons (the paper's problem): y ~ Poisson(N + C * XS_grid_edges(6)/255)
+        # and background C * XS_grid_edges(6)/255, as main.m:68,138,145
+        scale, kappa = 1.0, prob.compton_scale
+    else:
+        # photon scale: expected N + c totals photons_per_view per view; the Compton
+        # calibration factor (the XS_grid_edges(6)/255 of main.m:68) is set so that the
+        # background is 30% of the diffraction signal.
+        n_sum = float(N.double().sum())
+        kappa = 0.3 * n_sum / max(float(C.double().sum()), 1e-300)
+        scale = prob.photons_per_view * g.n_views / (n_sum * 1.3)
     A.mul_(scale)
     c = (C * (kappa * scale)).contiguous()
     expected = (N * scale + c).double().cpu().numpy()
@@ -255,7 +260,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.dtype,
-        "data": "synthetic",
+        "data": ("the reference's own inputs (phantom_generator, measured patterns, "
+                 "SpekCalc spectrum) through the builders, Poisson counts"
+                 if args.workload == "paper100" else "synthetic"),
         "config": {
             "workload": f"{args.workload}: {g.nx}x{g.ny} voxels, {g.n_views} views, "
                         f"{g.n_cols} pixels x {d.n_energy} energies, {d.n_q} q-bins x "

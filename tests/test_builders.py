@@ -161,3 +161,236 @@ def test_spectrum_response_from_reference_spectrum():
     scale = 0.100e-3 * 10.0 * 100.0 ** 2
     emitted = np.sum(N[:-1] * scale * np.diff(E))
     assert 0.9 * emitted < nd.sum() <= emitted * 1.0001
+
+
+# ---------------------------------------------------------------------------
+# MAT v7.3 (HDF5) reader, pinned by files the reference holds in two formats or
+# whose contents MATLAB computed from other files
+
+def _ref(*parts):
+    return os.path.join(REF, *parts)
+
+
+@needs_ref
+def test_hdf5_spectrum_equals_its_text_export():
+    """Spectrum_80kVp/spectral_data.mat (v7.3, a struct of two 73 x 1 arrays) holds the
+    same SpekCalc spectrum as the text export beside it, bit for bit."""
+    from xrd_tomography_amd.matfile import load_mat, mat_version
+    fn = _ref("Spectrum_80kVp", "spectral_data.mat")
+    assert mat_version(fn) == "7.3"
+    sp = load_mat(fn)["spectrum"]
+    E, N = B.read_spekcalc(_ref("Spectrum_80kVp", "SpeckCacl_80kVp.txt"))
+    assert sp["energy"].shape == (73, 1) and sp["counts"].shape == (73, 1)
+    assert np.array_equal(sp["energy"].ravel(), E) and np.array_equal(sp["counts"].ravel(), N)
+
+
+@needs_ref
+def test_hdf5_basis_coefficients_equal_the_fit():
+    """PE_and_Compton_coefs.mat (v7.3, 2 x 27) holds MATLAB's fitlm coefficients of
+    XSection_builder.m:35-60 for Z = 1..27; the least-squares restatement on the v5
+    attenuation table reproduces them."""
+    from xrd_tomography_amd.matfile import load_mat
+    pc = load_mat(_ref("Physics_inputs", "PE_and_Compton_coefs.mat"))["PE_and_Compton_coefs"]
+    assert pc.shape == (2, 27)
+    phys = B.load_physics(_ref("Physics_inputs"))
+    a1, a2 = B.fit_basis(phys["attenuation"], z_range=range(1, 28))
+    assert np.max(np.abs(a1 - pc[0]) / np.abs(pc[0])) < 1e-9
+    assert np.max(np.abs(a2 - pc[1]) / np.abs(pc[1])) < 1e-12
+
+
+@needs_ref
+def test_hdf5_hd_spectrum_against_the_1kev_export():
+    """spectral_data_HD.mat (v7.3, chunked + deflate, 1441 points at 0.05 keV) is the
+    same SpekCalc spectrum on a finer grid: at whole keV it equals the 1 keV export
+    to 1e-3 away from tungsten's K lines (57-60 and 66-70 keV, which the finer grid
+    resolves), and the two integrate to the same total within 0.2%."""
+    E, N = B.read_spekcalc(_ref("Spectrum_80kVp", "SpeckCacl_80kVp.txt"))
+    e, n = B.load_spectrum_hd(_ref("Spectrum_80kVp", "spectral_data_HD.mat"))
+    assert e.shape == (1441,) and e[0] == 8.0 and e[-1] == 80.0
+    assert np.allclose(np.diff(e), 0.05, atol=1e-9)
+    idx = np.round((E - 8.0) / 0.05).astype(int)
+    assert np.array_equal(e[idx], E)
+    off_lines = ~(((E >= 57) & (E <= 60)) | ((E >= 66) & (E <= 70)))
+    zero = N == 0                                  # the 80 keV end point
+    assert np.array_equal(n[idx][zero], N[zero])
+    rel = np.abs(n[idx] - N)[~zero] / N[~zero]
+    assert np.max(rel[off_lines[~zero]]) < 1e-3
+    assert abs(np.trapezoid(n, e) / np.trapezoid(N, E) - 1.0) < 2e-3
+
+
+@needs_ref
+def test_measured_patterns_load():
+    from xrd_tomography_amd.matfile import load_mat
+    for fname, rows in (("cellulose.mat", 117), ("aluminum_alloy.mat", 173),
+                        ("ammoniumnitrate.mat", 194), ("TNT.mat", 289)):
+        D = load_mat(_ref("open_source_data", fname), ["D"])["D"]
+        assert D.shape == (rows, 2) and D.dtype == np.float64, fname
+        assert np.all(D[:, 1] > 0) and len(np.unique(D[:, 0])) == rows, fname
+        assert 5.0 < D[:, 0].min() and D[:, 0].max() < 140.0, fname
+
+
+def test_mat_reader_rejects_what_it_does_not_read(tmp_path):
+    from xrd_tomography_amd.matfile import load_mat
+    fn = tmp_path / "x.mat"
+    fn.write_bytes(b"MATLAB 7.3 MAT-file" + b" " * 493 + b"\0" * 600)
+    with pytest.raises(ValueError, match="superblock"):
+        load_mat(str(fn))
+    fn.write_bytes(b"not a mat file" + b"\0" * 200)
+    with pytest.raises(ValueError, match="MAT-file"):
+        load_mat(str(fn))
+
+
+# ---------------------------------------------------------------------------
+# XRD_builder.m
+
+def test_pattern_q_is_bragg():
+    """q = 2 E sin(theta) / hbarc = 2 pi / d: aluminium (111), d = 2.3380 A, sits at
+    2theta = 2 asin(lambda / 2d) with lambda = 2 pi hbarc / E."""
+    lam = 2 * np.pi * B.HBARC_XRD / B.E_CU_KALPHA
+    d111 = 4.0495 / np.sqrt(3.0)
+    tt = 2 * np.degrees(np.arcsin(lam / (2 * d111)))
+    D = np.array([[tt + 5.0, 3.0], [tt, 10.0], [tt - 3.0, 0.0]])
+    q, inten = B.pattern_q(D)
+    assert len(q) == 2 and abs(q[0] - 2 * np.pi / d111) < 1e-12   # sorted, zero dropped
+    assert np.array_equal(inten, [10.0, 3.0])
+    _, ip = B.pattern_q(D, polarization=True)
+    th = np.radians(tt)
+    assert abs(ip[0] - 10.0 / ((1 + np.cos(th) ** 2) / 2)) < 1e-12
+
+
+def _makima_published(x, v, xq):
+    """Modified Akima as published (Akima 1970 with the weights w1 = |d[i+1] - d[i]| +
+    |d[i+1] + d[i]|/2, w2 = |d[i-1] - d[i-2]| + |d[i-1] + d[i-2]|/2; two extra slopes
+    extrapolated linearly at each end; slope (d[i-1] + d[i])/2 when both weights
+    vanish), Hermite cubic per interval, NaN outside [x0, xn]."""
+    n = len(x)
+    d = [(v[i + 1] - v[i]) / (x[i + 1] - x[i]) for i in range(n - 1)]
+    d = [3 * d[0] - 2 * d[1], 2 * d[0] - d[1]] + d + [2 * d[-1] - d[-2], 3 * d[-1] - 2 * d[-2]]
+    s = []
+    for i in range(n):
+        dm2, dm1, d0, dp1 = d[i], d[i + 1], d[i + 2], d[i + 3]
+        w1 = abs(dp1 - d0) + abs(dp1 + d0) / 2
+        w2 = abs(dm1 - dm2) + abs(dm1 + dm2) / 2
+        s.append((dm1 + d0) / 2 if w1 + w2 == 0 else (w1 * dm1 + w2 * d0) / (w1 + w2))
+    out = []
+    for t in xq:
+        if t < x[0] or t > x[-1]:
+            out.append(np.nan)
+            continue
+        i = min(max(int(np.searchsorted(x, t, side="right")) - 1, 0), n - 2)
+        h = x[i + 1] - x[i]
+        u = (t - x[i]) / h
+        h00, h10 = 2 * u ** 3 - 3 * u ** 2 + 1, u ** 3 - 2 * u ** 2 + u
+        h01, h11 = -2 * u ** 3 + 3 * u ** 2, u ** 3 - u ** 2
+        out.append(h00 * v[i] + h10 * h * s[i] + h01 * v[i + 1] + h11 * h * s[i + 1])
+    return np.array(out)
+
+
+def test_makima1_is_the_published_algorithm():
+    rng = np.random.default_rng(5)
+    x = np.cumsum(rng.uniform(0.05, 1.0, 14))
+    v = rng.normal(size=14)
+    v[4:7] = 1.0                                  # a flat run: both weights vanish
+    xq = np.linspace(x[0] - 0.3, x[-1] + 0.3, 101)
+    a = B.makima1(x, v, xq)
+    b = _makima_published(x, v, xq)
+    assert np.array_equal(np.isnan(a), np.isnan(b)) and np.isnan(a[0]) and np.isnan(a[-1])
+    ok = ~np.isnan(a)
+    assert np.allclose(a[ok], b[ok], rtol=0, atol=1e-12)
+
+
+def test_theory_cross_sections_at_grid_nodes():
+    """At a (q, a1/a2) grid node the makima tables return the node value:
+    S = mean_a2 * S_max / 255 * grid[i, j]."""
+    rng = np.random.default_rng(6)
+    S1 = rng.uniform(0, 255, (16, 12))
+    S2 = rng.uniform(0, 255, (16, 12))
+    edges = (-11.0, -5.0, -16.0, -7.0, 2.5, 0.3)
+    qg = edges[0] + np.arange(16) * 6.0 / 15
+    ag = edges[2] + np.arange(12) * 9.0 / 11
+    j = 7
+    mean_a2 = 0.25
+    mean_a1 = mean_a2 * 2.0 ** ag[j]
+    q = 2.0 ** qg[[1, 5, 9]] * 2.0 * B.MC2_XRD / B.HBARC_XRD
+    s1, s2 = B.theory_cross_sections(np.r_[q, 1e-6], mean_a1, mean_a2, edges, S1, S2)
+    assert np.allclose(s1[:3], mean_a2 * 2.5 / 255 * S1[[1, 5, 9], j], rtol=1e-10)
+    assert np.allclose(s2[:3], mean_a2 * 0.3 / 255 * S2[[1, 5, 9], j], rtol=1e-10)
+    assert np.isnan(s1[3]) and np.isnan(s2[3])   # off the grid: NaN, as interpn
+
+
+def test_xrd_builder_measured_inside_theory_outside():
+    rng = np.random.default_rng(8)
+    S1 = rng.uniform(1, 255, (8, 8))
+    S2 = rng.uniform(1, 255, (8, 8))
+    edges = (-11.0, -5.0, -16.0, -7.0, 2.5, 0.3)
+    seg = np.zeros((6, 6), np.int32)
+    seg[1:3, 1:4] = 1
+    a2 = np.where(seg == 1, 0.2, 0.0)
+    a1 = a2 * 2.0 ** -11.5
+    tt = np.linspace(20.0, 60.0, 41)
+    D = np.stack([tt, 100.0 + 20.0 * np.sin(tt)], 1)
+    f, qf = B.xrd_builder(a1, a2, seg, edges, S1, S2, [(D, 0.01, False)], n_qf=96)
+    qx, ix = B.pattern_q(D)
+    inside = (qf >= qx[0]) & (qf <= qx[-1])
+    assert inside.any() and (~inside).any()
+    assert np.allclose(f[inside, 0], 0.01 * B.makima1(qx, ix, qf[inside]), rtol=1e-14)
+    s1, s2 = B.theory_cross_sections(qf, 0.2 * 2.0 ** -11.5, 0.2, edges, S1, S2)
+    assert np.allclose(f[~inside, 0], (s1 + s2)[~inside], rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# phantom_generator.m and the paper's problem
+
+def test_material_coefficients_are_additive():
+    """Mass-fraction mixing: a mixture's (a1, a2) at density rho is the weighted sum of
+    its pure elements' coefficients at that density; a pure element is
+    K1/mc2^m rho Z^n / M and K2 rho Z / M (phantom_generator.m:132-138)."""
+    Z, M = [13, 26], [26.9815, 55.845]
+    w = np.array([0.7, 0.3])
+    mix = B.material_coefficients(2.0, Z, w, M)
+    pure = [B.material_coefficients(2.0, [z], [1.0], [m]) for z, m in zip(Z, M)]
+    assert np.allclose(mix, w @ np.array(pure), rtol=1e-14)
+    ph = B.PHANTOM_PHYS
+    assert np.isclose(pure[0][0], ph["K1"] / ph["mc2"] ** ph["m"] * 2.0 * 13 ** ph["n"] / 26.9815)
+    assert np.isclose(pure[0][1], ph["K2"] * 2.0 * 13 / 26.9815)
+
+
+def test_phantom_generator_regions():
+    """Voxel counts per material equal the painted ellipse areas (pi a b over 4 mm^2
+    voxels) within 3%, accounting for the objects painted over cellulose."""
+    M = np.linspace(1.0, 200.0, 92)
+    a1, a2, seg = B.phantom_generator(M)
+    assert seg.shape == (100, 100) and set(np.unique(seg)) == {0, 1, 2, 3}
+    area = lambda a, b: np.pi * a * b / 4.0      # noqa: E731
+    al = area(12, 16)
+    an = area(12, 12) + area(20, 12)
+    cel = area(24, 80) + area(40, 40) + area(40, 30) - al - an
+    for mid, expect in ((1, cel), (2, al), (3, an)):
+        assert abs((seg == mid).sum() / expect - 1.0) < 0.03, mid
+    for mid in (1, 2, 3):
+        assert len(np.unique(a1[seg == mid])) == 1 and len(np.unique(a2[seg == mid])) == 1
+    assert np.all(a1[seg == 0] == 0) and np.all(a2[seg == 0] == 0)
+
+
+def test_paper_fixture_form_factors():
+    """The committed paper problem: form factors positive, within the ranges main.m
+    plots them in (main.m:193: cellulose 1e-3..0.05, aluminium 0.1..40, ammonium
+    nitrate 0.03..5), and its coarse truth is the fine curve resampled."""
+    p = make_problem("paper100")
+    g, d = p.geometry, p.disc
+    assert (g.nx, g.ny, g.n_views, g.n_cols, d.n_energy, d.n_q, d.n_qf, d.n_mat) == \
+        (100, 100, 32, 1024, 64, 256, 768, 3)
+    for m, (lo, hi) in enumerate(((1e-3, 0.05), (0.1, 40.0), (0.03, 5.0))):
+        assert lo <= p.ifine[:, m].min() and p.ifine[:, m].max() <= hi, m
+    assert p.compton_scale > 0 and np.all(p.spectrum >= 0)
+
+
+@needs_ref
+def test_paper_fixture_rebuilds_from_the_reference(tmp_path):
+    """data/paper100 is what paper_problem builds from the reference's input files."""
+    q = B.paper_problem(REF)
+    p = make_problem("paper100")
+    for n in ("spectrum", "ifine", "segmentation", "a1", "a2", "s2_table", "x_true"):
+        assert np.array_equal(getattr(p, n), getattr(q, n)), n
+    assert p.lut == q.lut and p.compton_scale == q.compton_scale
+    assert p.geometry.as_dict() == q.geometry.as_dict() and p.disc.as_dict() == q.disc.as_dict()

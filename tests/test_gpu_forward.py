@@ -96,6 +96,25 @@ def test_forward_multi_row_detector(xrd, oracle, name, rows):
     _compare(f"{name} rows={rows}", Ng, Cg, Ag, No, Co, Ao)
 
 
+@pytest.mark.parametrize("flags", [0, 3])
+def test_forward_paper100(xrd, oracle, flags):
+    """The paper's own problem (main.m:12-63, built from the reference's input files:
+    100x100 phantom, 3 measured materials, 768 fine and 256 coarse q-bins, 64
+    energies), one view of a 16-column detector on the reference scanner."""
+    import dataclasses
+    from xrd_tomography_amd.descriptors import Discretization
+    from xrd_tomography_amd.synthetic import make_problem
+    prob = make_problem("paper100", n_cols=16)
+    if flags:
+        d = prob.disc
+        prob = dataclasses.replace(prob, disc=Discretization.reference(
+            n_energy=d.n_energy, n_q=d.n_q, n_qf=d.n_qf, n_mat=d.n_mat, flags=flags))
+    views = [5]
+    Ng, Cg, Ag = _run(xrd, prob, views=views)
+    No, Co, Ao = oracle.forward(prob, views=views, nthreads=16)
+    _compare(f"paper100 flags={flags}", Ng, Cg, Ag, No, Co, Ao)
+
+
 def _host(prob, N, C, A):
     import torch
     torch.cuda.synchronize()

@@ -11,7 +11,15 @@ them as plain arrays) and a SpekCalc text export of the source spectrum.
 * ``xsection_tables``: coherent and incoherent cross-sections over the
   (log2 q, log2 a1/a2) grid, scaled to 0..255 (XSection_builder.m:62-128).
 * ``spectrum_response``: the detector-response-smeared spectrum matrix
-  Ndet[e_src, e_det] (spectrum_builder.m:8-95).
+  Ndet[e_src, e_det] (spectrum_builder.m:8-95); ``load_spectrum_hd`` reads its
+  0.05 keV input, Spectrum_80kVp/spectral_data_HD.mat (MAT v7.3).
+* ``pattern_q`` and ``xrd_builder``: measured 2-theta diffraction patterns
+  (open_source_data/*.mat, MAT v7.3) converted to calibrated form factors on the
+  fine q axis, with the Rayleigh + Compton theory outside the measured range
+  (XRD_builder.m).
+* ``phantom_generator``: the paper's 100 x 100 phantom of cellulose, aluminium
+  alloy and ammonium nitrate (phantom_generator.m).
+* ``paper_problem``: the problem main.m:12-63 assembles from all of the above.
 """
 from __future__ import annotations
 
@@ -169,3 +177,232 @@ def spectrum_response(n_energy: int, energy: np.ndarray, counts: np.ndarray,
                     np.maximum(lo[:, None], e_left[None, :]))             # [bin, table bin]
     ndet = wt @ window.T @ wt.T                                           # [s, d]
     return ndet, edges
+
+
+def load_spectrum_hd(path: str):
+    """(energy keV, N per keV cm^2 mAs at 1 m) of the SpekCalc spectrum struct in a
+    MAT-file (spectral_data_HD.mat: 1441 points 8..80 keV at 0.05 keV,
+    spectrum_builder.m:9)."""
+    from .matfile import load_mat
+    sp = load_mat(path, ["spectrum"])["spectrum"]
+    return (np.asarray(sp["energy"], np.float64).ravel(),
+            np.asarray(sp["counts"], np.float64).ravel())
+
+
+# ---------------------------------------------------------------------------
+# XRD_builder.m
+
+E_CU_KALPHA = 8.04            # keV, the diffractometer's source (XRD_builder.m:10)
+HBARC_XRD = 1.973269804       # keV Angstrom (XRD_builder.m:11)
+MC2_XRD = 510.99895           # keV (XRD_builder.m:12)
+
+# (file, calibration factor, polarization-corrected) per material, in the order of
+# XRD_builder.m:36-52 (cellulose), :86-101 (aluminium alloy), :136-152 (ammonium nitrate)
+PAPER_PATTERNS = (("cellulose.mat", 0.00003, False),
+                  ("aluminum_alloy.mat", 0.005, True),
+                  ("ammoniumnitrate.mat", 0.0007, False))
+
+
+def pattern_q(D: np.ndarray, e_src_kev: float = E_CU_KALPHA, hbarc: float = HBARC_XRD,
+              polarization: bool = False):
+    """(q 1/Angstrom, intensity) of a measured pattern D = [2theta deg, counts]
+    (XRD_builder.m:40-50): zero-count points are dropped, q = 2 E sin(theta/2) / hbarc
+    and, with polarization, the counts are divided by (1 + cos^2 theta) / 2.  The
+    points are returned sorted by q (stable), as interp1 sorts unsorted sample
+    points; aluminum_alloy.mat has two 2theta values out of order."""
+    D = np.asarray(D, np.float64)
+    D = D[D[:, 1] > 0]
+    theta = D[:, 0] / 180.0 * np.pi
+    q = np.sin(theta / 2.0) * (2.0 * e_src_kev) / hbarc
+    inten = D[:, 1].copy()
+    if polarization:
+        inten = inten / ((1.0 + np.cos(theta) ** 2) / 2.0)
+    order = np.argsort(q, kind="stable")
+    if np.any(np.diff(q[order]) == 0):
+        raise ValueError("pattern has repeated 2-theta values (interp1 needs distinct points)")
+    return q[order], inten[order]
+
+
+def makima1(x: np.ndarray, v: np.ndarray, xq: np.ndarray) -> np.ndarray:
+    """interp1(x, v, xq, 'makima', NaN): modified Akima, NaN outside [x0, xn]."""
+    from scipy.interpolate import Akima1DInterpolator
+    return Akima1DInterpolator(x, v, method="makima", extrapolate=False)(xq)
+
+
+def fine_q_axis(n_qf: int = 768, q_min: float = 0.5, q_max: float = 6.0):
+    """Centres of n_qf equal bins on [q_min, q_max] (XRD_builder.m:25-27)."""
+    edges = q_min + np.arange(n_qf + 1) / n_qf * (q_max - q_min)
+    return 0.5 * (edges[:-1] + edges[1:])
+
+
+def theory_cross_sections(q: np.ndarray, mean_a1: float, mean_a2: float, grid_edges,
+                          S1_grid: np.ndarray, S2_grid: np.ndarray,
+                          hbarc: float = HBARC_XRD, mc2: float = MC2_XRD):
+    """(S1, S2) per unit volume of a material with mean attenuation coefficients
+    (mean_a1, mean_a2) at momentum transfers q (XRD_builder.m:54-61): the 0..255
+    tables at (log2(q hbarc / mc2 / 2), log2(a1/a2)) by tensor-product makima (q
+    first), times mean_a2 * S_max / 255.  Points off the table grid are NaN, as
+    interpn returns there."""
+    q_min, q_max, a_min, a_max, s1_max, s2_max = grid_edges
+    nq, na = S1_grid.shape
+    qg = q_min + np.arange(nq) * (q_max - q_min) / (nq - 1)
+    ag = a_min + np.arange(na) * (a_max - a_min) / (na - 1)
+    lq = np.log2(np.asarray(q, np.float64) * hbarc / mc2 / 2.0)
+    la = float(np.log2(mean_a1 / mean_a2))
+    tol = 1e-12 * max(1.0, abs(q_max - q_min), abs(a_max - a_min))
+    off = (lq < q_min - tol) | (lq > q_max + tol)
+    out = []
+    for S, smax in ((S1_grid, s1_max), (S2_grid, s2_max)):
+        if la < a_min - tol or la > a_max + tol:
+            v = np.full(lq.shape, np.nan)
+        else:
+            v = makima2(qg, ag, np.asarray(S, np.float64), lq, np.array([la]))[:, 0]
+            v[off] = np.nan
+        out.append(mean_a2 * smax / 255.0 * v)
+    return out[0], out[1]
+
+
+def xrd_builder(a1: np.ndarray, a2: np.ndarray, segmentation: np.ndarray, grid_edges,
+                S1_grid: np.ndarray, S2_grid: np.ndarray, patterns, n_qf: int = 768,
+                q_min: float = 0.5, q_max: float = 6.0):
+    """(ifine [n_qf, M], q_fine) of XRD_builder.m for M = len(patterns) materials.
+
+    patterns[m] = (D, calibration, polarization) for material id m + 1, D the measured
+    [2theta deg, counts] table.  Material m's form factor on the fine q axis is
+    calibration * makima(q_exp, I_exp) inside the measured q range, and the theory
+    S1 + S2 at the material's mean (a1, a2) (a1, a2 in 1/cm) outside it
+    (XRD_builder.m:36-66)."""
+    qf = fine_q_axis(n_qf, q_min, q_max)
+    seg = np.asarray(segmentation)
+    out = np.zeros((n_qf, len(patterns)))
+    for m, (D, calib, pol) in enumerate(patterns):
+        mask = seg == m + 1
+        if not mask.any():
+            raise ValueError(f"material {m + 1} has no voxels in the segmentation")
+        qx, ix = pattern_q(D, polarization=pol)
+        f = calib * makima1(qx, ix, qf)
+        s1, s2 = theory_cross_sections(qf, float(np.mean(a1[mask])), float(np.mean(a2[mask])),
+                                       grid_edges, S1_grid, S2_grid)
+        nan = np.isnan(f)
+        f[nan] = s1[nan] + s2[nan]
+        out[:, m] = f
+    return out, qf
+
+
+# ---------------------------------------------------------------------------
+# phantom_generator.m
+
+# photoelectric and Compton amplitudes of phantom_generator.m:31-36 (K1 cm^2 keV^3/mol,
+# K2 cm^2/mol, energy slope m, Z slope n, mc2 keV)
+PHANTOM_PHYS = dict(K1=13.96, K2=0.30, m=3.00, n=4.20, mc2=510.975)
+
+
+def material_coefficients(density: float, Z: Sequence[int], weight: Sequence[float],
+                          M: Sequence[float], phys: dict = PHANTOM_PHYS):
+    """(a1, a2) in 1/cm of a material from its mass fractions `weight` of elements Z
+    with molar masses M: a1 = K1/mc2^m rho sum_i w_i Z_i^n / M_i and
+    a2 = K2 rho sum_i w_i Z_i / M_i (phantom_generator.m:54-60, :140-145).  A
+    molecule N_i x Z_i has w_i = N_i M_i / sum_j N_j M_j (phantom_generator.m:54-60
+    writes that case directly)."""
+    Z = np.asarray(Z, np.float64)
+    w = np.asarray(weight, np.float64)
+    M = np.asarray(M, np.float64)
+    a1 = phys["K1"] / phys["mc2"] ** phys["m"] * density * np.sum(w * Z ** phys["n"] / M)
+    a2 = phys["K2"] * density * np.sum(w * Z / M)
+    return float(a1), float(a2)
+
+
+def _molecule(density, Z, N, M):
+    N = np.asarray(N, np.float64)
+    M = np.asarray(M, np.float64)
+    return material_coefficients(density, Z, N * M / np.sum(N * M), M)
+
+
+def phantom_generator(molar_mass: np.ndarray, nx: int = 100, ny: int = 100,
+                      size_mm: float = 200.0):
+    """(a1, a2 in 1/cm, segmentation) of the paper's phantom (phantom_generator.m):
+    three cellulose ellipses (id 1), one aluminium-alloy ellipse (id 2) and two
+    ammonium-nitrate ellipses (id 3) on an nx x ny grid over size_mm^2, painted in
+    that order (later objects overwrite).  molar_mass[Z-1] in g/mol
+    (physical_constants.mat phys.M) serves the alloy."""
+    x, y = np.meshgrid((np.arange(1, nx + 1) - 0.5) * size_mm / nx,
+                       (np.arange(1, ny + 1) - 0.5) * size_mm / ny, indexing="ij")
+    a1 = np.zeros((nx, ny))
+    a2 = np.zeros((nx, ny))
+    seg = np.zeros((nx, ny), np.int32)
+    cellulose = _molecule(0.1, [1, 6, 8], [10, 6, 5], [1.00784, 12.0107, 15.999])
+    al_Z = [26, 14, 29, 30, 22, 25, 12, 13]
+    al_w = np.array([0.12, 1.9, 0.005, 0.01, 0.009, 0.58, 5.17, 92.206]) / 100.0
+    alloy = material_coefficients(2.6, al_Z, al_w, np.asarray(molar_mass)[np.array(al_Z) - 1])
+    nitrate = _molecule(1.725, [1, 7, 8], [4, 2, 3], [1.00784, 14.0067, 15.999])
+    # (material id, coefficients, x0, y0 mm, tilt deg, semi-axes a, b mm)
+    objects = [(1, cellulose, 40.0, 110.0, 0.0, 24.0, 80.0),
+               (1, cellulose, 130.0, 150.0, -5.0, 40.0, 40.0),
+               (1, cellulose, 130.0, 60.0, 5.0, 40.0, 30.0),
+               (2, alloy, 40.0, 130.0, 10.0, 12.0, 16.0),
+               (3, nitrate, 40.0, 80.0, 10.0, 12.0, 12.0),
+               (3, nitrate, 130.0, 60.0, 10.0, 20.0, 12.0)]
+    for mid, (c1, c2), x0, y0, tilt, a, b in objects:
+        t = tilt / 180.0 * np.pi
+        inside = ((((x - x0) * np.cos(t) - (y - y0) * np.sin(t)) / a) ** 2
+                  + (((x - x0) * np.sin(t) + (y - y0) * np.cos(t)) / b) ** 2) < 1
+        a1[inside] = c1
+        a2[inside] = c2
+        seg[inside] = mid
+    return a1, a2, seg
+
+
+# ---------------------------------------------------------------------------
+# main.m:12-63
+
+def paper_problem(ref_dir: str, n_views: int = 32, n_cols: int = 1024, n_energy: int = 64):
+    """The problem main.m assembles before Forward_XRD, from the reference's own input
+    files under ref_dir: the SpekCalc HD spectrum through spectrum_builder (NRG = 64),
+    the cross-section tables (NQ = NA = 64), the 100 x 100 phantom, and the three
+    measured patterns through XRD_builder (NQF = 768, NWT = 256), on the scanner
+    literals of integrate_diffraction.cu:81-140 with SRC = 32 views and COL = 1024
+    columns (Forward_XRD.cu:16-23).  a1, a2 are given to the projector in 1/voxel
+    with main.m's conversion vox_over_cm = 0.1 / (NX / 100) (main.m:48-61).
+    x_true is the fine form factor linearly resampled at the coarse q centres, and
+    compton_scale = XS_grid_edges(6) / 255 converts the projector's C output to
+    photons (main.m:68)."""
+    from .descriptors import Discretization, Geometry, q_axis
+    from .matfile import load_mat
+    from .synthetic import Problem
+
+    phys = load_physics(os.path.join(ref_dir, "Physics_inputs"))
+    e_hd, n_hd = load_spectrum_hd(os.path.join(ref_dir, "Spectrum_80kVp", "spectral_data_HD.mat"))
+    ndet, _ = spectrum_response(n_energy, e_hd, n_hd)
+    S1, S2, edges = xsection_tables(phys, 64, 64)
+    a1, a2, seg = phantom_generator(phys["M"])
+    pats = []
+    for fname, calib, pol in PAPER_PATTERNS:
+        D = load_mat(os.path.join(ref_dir, "open_source_data", fname), ["D"])["D"]
+        pats.append((D, calib, pol))
+    ifine, _ = xrd_builder(a1, a2, seg, edges, S1, S2, pats, n_qf=768)
+    nx, ny = seg.shape
+    g = Geometry.reference(nx, ny, n_views=n_views, n_cols=n_cols, n_rows=1)
+    d = Discretization.reference(n_energy=n_energy, n_q=256, n_qf=768, n_mat=3)
+    q, _, qf, _ = q_axis(d)
+    x_true = np.stack([np.interp(q, qf, ifine[:, m]) for m in range(3)], axis=1)
+    vox_over_cm = 0.1 / (nx / 100.0)
+    prob = Problem(name="paper100", geometry=g, disc=d,
+                   spectrum=ndet.astype(np.float32), ifine=ifine.astype(np.float32),
+                   segmentation=seg, a1=(a1 * vox_over_cm).astype(np.float32),
+                   a2=(a2 * vox_over_cm).astype(np.float32),
+                   s2_table=np.asarray(S2, np.float32), lut=tuple(float(v) for v in edges[:4]),
+                   x_true=x_true, photons_per_view=0.0, mlem_iterations=1000,
+                   voxel_mm=nx / 100.0, compton_scale=float(edges[5]) / 255.0)
+    return prob
+
+
+if __name__ == "__main__":
+    # python -m xrd_tomography_amd.builders REF_DIR OUT_DIR: write the paper's problem
+    # (paper_problem) in the on-disk format of problem_io
+    import sys
+    from .problem_io import save_problem
+    if len(sys.argv) != 3:
+        raise SystemExit("usage: python -m xrd_tomography_amd.builders REF_DIR OUT_DIR")
+    prob = paper_problem(sys.argv[1])
+    save_problem(prob, sys.argv[2])
+    print(f"wrote {sys.argv[2]}: {prob.name}, compton_scale {prob.compton_scale:.9g}")

@@ -7,7 +7,8 @@ problem.json::
      "geometry": {<every xrd_geometry field>},
      "discretization": {<every xrd_discretization field>},
      "lut": [q_min, q_max, a_min, a_max],
-     "meta": {"photons_per_view": ..., "mlem_iterations": ..., "voxel_mm": ...},
+     "meta": {"photons_per_view": ..., "mlem_iterations": ..., "voxel_mm": ...,
+              "compton_scale": ... (optional)},
      "arrays": {"<name>": {"file": "<name>.bin", "dtype": "<f4" | "<i4" | "<f8",
                            "shape": [...], "order": "first index fastest"}}}
 
@@ -22,6 +23,10 @@ fastest, the MATLAB column-major layout of Forward_XRD.cu:80-119):
   x_true        <f8 [NWT, MAT]     form factor at the coarse q centres (optional)
   counts        <f4 [rows]         measured counts y, row v + V*(det + DET*e) (optional)
   background    <f4 [rows]         Compton/background estimate c (optional)
+
+compton_scale is the factor that turns the projector's Compton output C into
+photons (XS_grid_edges(6)/255 in main.m:68); photons_per_view = 0 means the
+spectrum is already in absolute photons, so the counts are N + compton_scale * C.
 
 Loading checks every array's size against the descriptors.
 """
@@ -79,6 +84,8 @@ def save_problem(prob: Problem, path: str, counts: Optional[np.ndarray] = None,
            "meta": {"photons_per_view": prob.photons_per_view,
                     "mlem_iterations": prob.mlem_iterations, "voxel_mm": prob.voxel_mm},
            "arrays": arrays}
+    if prob.compton_scale:
+        doc["meta"]["compton_scale"] = float(prob.compton_scale)
     with open(os.path.join(path, "problem.json"), "w") as f:
         json.dump(doc, f, indent=1)
 
@@ -117,6 +124,7 @@ def load_problem(path: str):
                    x_true=out.get("x_true", np.zeros(shapes["x_true"])),
                    photons_per_view=float(meta.get("photons_per_view", 0.0)),
                    mlem_iterations=int(meta.get("mlem_iterations", 0)),
-                   voxel_mm=float(meta.get("voxel_mm", 1.0)))
+                   voxel_mm=float(meta.get("voxel_mm", 1.0)),
+                   compton_scale=float(meta.get("compton_scale", 0.0)))
     extras = {k: out[k] for k in ("counts", "background") if k in out}
     return prob, extras

@@ -55,9 +55,10 @@ class Problem:
     s2_table: np.ndarray      # [NQ x NA] float32, 0..255
     lut: tuple                # (q_min, q_max, a_min, a_max)
     x_true: np.ndarray        # [Q x M] float64 form factor sampled at coarse centres
-    photons_per_view: float
+    photons_per_view: float   # 0: the spectrum is in absolute photons (no rescaling)
     mlem_iterations: int
     voxel_mm: float
+    compton_scale: float = 0.0  # C output -> photons (main.m:68); 0 when not calibrated
 
     @property
     def n_rows(self) -> int:
@@ -234,6 +235,11 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
                  (tests).
     ``reference`` : the reference's own scanner literals (integrate_diffraction.cu:81-140)
                  on a 40x40 grid with 3 materials, 4 views, 16 pixels, 16 energies.
+    ``paper100``: the problem main.m builds from the reference's own inputs (100x100
+                 phantom, 3 measured materials, 32 views, 1024 pixels, 64 energies,
+                 256 q-bins, 768 fine q-bins), stored in data/paper100 by
+                 ``python -m xrd_tomography_amd.builders``; n_views / n_cols overrides
+                 keep the inputs and change the scanner's view and column counts.
     Keyword overrides replace sizes (nx, n_views, n_cols, n_rows, n_energy, n_q, n_qf,
     n_mat, flags, voxel_mm, photons_per_view, mlem_iterations).
     """
@@ -251,6 +257,8 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
                           n_q=32, n_qf=96, photons_per_view=1e4, mlem_iterations=10,
                           allow_rect=False, axis=(8.0, 20.0), disc=35.0, seed=3),
     }
+    if name == "paper100":
+        return _paper100(**overrides)
     if name not in presets:
         raise ValueError(f"unknown workload {name!r}; choose from {sorted(presets)}")
     p = dict(presets[name])
@@ -302,6 +310,20 @@ def make_problem(name: str = "small", seed: Optional[int] = None, **overrides) -
                    s2_table=s2.astype(np.float32), lut=lut, x_true=x_true,
                    photons_per_view=float(p["photons_per_view"]),
                    mlem_iterations=int(p["mlem_iterations"]), voxel_mm=vmm)
+
+
+def _paper100(**overrides) -> Problem:
+    from .problem_io import load_problem
+    prob, _ = load_problem(os.path.join(_DATA, "paper100"))
+    extra = set(overrides) - {"n_views", "n_cols"}
+    if extra:
+        raise ValueError(f"paper100 takes only n_views / n_cols overrides, not {sorted(extra)}")
+    if overrides:
+        g = prob.geometry
+        prob.geometry = Geometry.reference(g.nx, g.ny,
+                                           n_views=int(overrides.get("n_views", g.n_views)),
+                                           n_cols=int(overrides.get("n_cols", g.n_cols)))
+    return prob
 
 
 def make_volume_problem(name: str = "small", n_slices: int = 4, seed: Optional[int] = None,
