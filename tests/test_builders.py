@@ -161,6 +161,13 @@ def test_spectrum_response_from_reference_spectrum():
     scale = 0.100e-3 * 10.0 * 100.0 ** 2
     emitted = np.sum(N[:-1] * scale * np.diff(E))
     assert 0.9 * emitted < nd.sum() <= emitted * 1.0001
+    # the same on the 0.05 keV table spectrum_builder.m:9 loads, and the two
+    # quadratures agree on the total to 1%
+    e, n = B.load_spectrum_hd(os.path.join(REF, "Spectrum_80kVp", "spectral_data_HD.mat"))
+    nd_hd, _ = B.spectrum_response(61, e, n)
+    emitted_hd = np.sum(n[:-1] * scale * np.diff(e))
+    assert 0.9 * emitted_hd < nd_hd.sum() <= emitted_hd * 1.0001
+    assert abs(nd_hd.sum() / nd.sum() - 1.0) < 0.01
 
 
 # ---------------------------------------------------------------------------

@@ -2,9 +2,10 @@
 
 Host-side, run once per scanner/material set; their outputs are inputs of
 xrd_forward (the spectrum response matrix, the Compton S2 table and its grid
-edges).  They read the reference's physics tables, which ship as MATLAB v5 files
-(attenuation_data.mat, Hubbell_data.mat, physical_constants.mat; scipy.io reads
-them as plain arrays) and a SpekCalc text export of the source spectrum.
+edges, the form factors, the phantom).  They read the reference's input files:
+the physics tables (MATLAB v5: attenuation_data.mat, Hubbell_data.mat,
+physical_constants.mat), the SpekCalc source spectrum and the measured
+diffraction patterns (MATLAB v7.3, read by xrd_tomography_amd.matfile).
 
 * ``fit_basis``: photoelectric + Compton basis fit of mu/rho per element
   (XSection_builder.m:15-60, least squares without intercept).
