@@ -228,64 +228,193 @@ __device__ __forceinline__ void lane_sample(const Ray &ray, double step, int q, 
     j0 = (int)fy;
 }
 
-// Kahan sum, for lane b's map, of the bilinear samples of one ray (the caller
-// multiplies by the step).  Lane l prepares sample q0 + l (FP64 position, stencil
-// offset, FP32 fractions); the wave then walks the 64 samples for every map.
-__device__ __forceinline__ float wave_trace_tm(const float *__restrict__ m, int nx, int ny,
-                                               double step, const Ray &ray, int lane) {
-    const int pw = nx + 2;
-    const int n = __builtin_amdgcn_readfirstlane(ray.n);
-    Kahan k;
-    for (int q0 = 0; q0 < n; q0 += kWave) {
-        int i0, j0;
-        double dax, day;
-        lane_sample(ray, step, min(q0 + lane, n - 1), i0, j0, dax, day);
-        const float fax = (float)dax, fay = (float)day;
-        // samples lie on a segment inside the grid, so i0 is in [-1, nx-1] and j0 in
-        // [-1, ny-1] (the clamp only guards the address) and the stencil's other
-        // corners sit at +1 texel and +1 row
-        const int o00 = ((min(max(i0, -1), nx - 1) + 1) + pw * (min(max(j0, -1), ny - 1) + 1)) *
-                        kTexelMaps;
-        const int cnt = min(kWave, n - q0);
-        // the loads of four samples are in flight at a time; each four are added in
-        // FP32 and their sum enters the compensated total
-        for (int u0 = 0; u0 < cnt; u0 += 4) {
-            float pv[4][4], wx[4], wy[4];
+// One run of samples in one cell (ci, cj): element offsets texel*64 of its four
+// stencil corners in the unpadded grid (-1 outside the map, or when a later run
+// of the window adds this corner's weight) and their summed bilinear weights.
+struct CellRun {
+    int off[4];
+    int ci, cj, pad0, pad1;
+    double w[4];
+};
+
+// The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
+// share a bilinear cell, in ray order, in the wave's list runs[0 .. nr); returns nr.
+// A segmented wave scan adds the FP64 bilinear weights of each run, and the
+// run-end lanes write their cells.  A stencil corner is shared by at most three
+// consecutive cells of a straight ray (the cell indices are monotone along it), so
+// each run then folds in the weights of the two runs before it for the corners it
+// shares with them and drops the corners that one of the next two runs shares:
+// every texel of the window appears once, with its total weight.
+__device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n, int q0, int nx,
+                                              int ny, int lane, CellRun *runs) {
+    const int cnt = min(kWave, n - q0);
+    int i0, j0;
+    double ax, ay;
+    lane_sample(ray, step, min(q0 + lane, n - 1), i0, j0, ax, ay);
+    const bool act = lane < cnt;
+    const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
+    const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
+    const bool first = lane == 0 || pi != i0 || pj != j0;
+    const bool last = act && (lane == cnt - 1 || ni != i0 || nj != j0);
+    const unsigned long long starts = __ballot(first);
+    // first lane of this lane's run: highest start at or below it
+    const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int rs = 63 - __clzll(below);
+    double w[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                const int u = min(u0 + kk, cnt - 1);
-                wx[kk] = read_lane(fax, u);
-                wy[kk] = read_lane(fay, u);
-                const float *p0 = m + (unsigned)(read_lane(o00, u) + lane);
-                const float *p1 = p0 + pw * kTexelMaps;
-                pv[kk][0] = p0[0];
-                pv[kk][1] = p0[kTexelMaps];
-                pv[kk][2] = p1[0];
-                pv[kk][3] = p1[kTexelMaps];
-            }
-            __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
-            float s4 = 0.f;
+    for (int d = 1; d < kWave; d <<= 1) {
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                if (u0 + kk >= cnt) break;
-                const float ax = wx[kk], ay = wy[kk], bx = 1.f - ax, by = 1.f - ay;
-                s4 += by * (bx * pv[kk][0] + ax * pv[kk][1]) + ay * (bx * pv[kk][2] + ax * pv[kk][3]);
-            }
-            k.add(s4);
+        for (int c = 0; c < 4; c++) {
+            const double up = __shfl_up(w[c], d, kWave);
+            if (lane - d >= rs) w[c] += up;
         }
     }
-    return k.value();
+    const unsigned long long ends = __ballot(last);
+    const int nr = __popcll(ends);
+    if (last) {
+        CellRun cr;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
+            const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
+            cr.off[c] = in ? (ti + nx * tj) * kTexelMaps : -1;
+            cr.w[c] = w[c];
+        }
+        cr.ci = i0;
+        cr.cj = j0;
+        cr.pad0 = cr.pad1 = 0;
+        runs[__popcll(ends & ((1ull << lane) - 1ull))] = cr;
+    }
+    // a wave's LDS accesses complete in issue order, so only the compiler must
+    // keep the list reads below the writes (a memory fence would also wait for
+    // every global access in flight)
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    // lane r < nr settles run r's corners against runs r-2 .. r+2
+    int newoff[4];
+    double neww[4];
+    if (lane < nr) {
+        const CellRun me = runs[lane];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int ti = me.ci + (c & 1), tj = me.cj + (c >> 1);
+            bool later = false;
+            double wsum = me.w[c];
+#pragma unroll
+            for (int k = -2; k <= 2; k++) {
+                if (k == 0 || lane + k < 0 || lane + k >= nr) continue;
+                const int di = ti - runs[lane + k].ci, dj = tj - runs[lane + k].cj;
+                if (di < 0 || di > 1 || dj < 0 || dj > 1) continue;
+                if (k > 0)
+                    later = true;
+                else
+                    wsum += runs[lane + k].w[di + 2 * dj];
+            }
+            newoff[c] = later ? -1 : me.off[c];
+            neww[c] = wsum;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    if (lane < nr) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            runs[lane].off[c] = newoff[c];
+            runs[lane].w[c] = neww[c];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    return nr;
+}
+
+// A kept corner of a window: element offset texel*64 in the padded texel-major maps
+// and its total weight.  Four per run at most, so the list fits in the run list's
+// space.
+struct TexW {
+    int off, pad;
+    double w;
+};
+static_assert(sizeof(TexW) * 4 == sizeof(CellRun), "texel list aliases the run list");
+
+// Line integral (before the factor step), for lane b's map, of one ray through the
+// texel-major maps: sum over texels of m[texel][b] times the texel's summed bilinear
+// weight.  Each window of 64 samples lists its texels once (wave_cell_runs), the kept
+// corners are compacted into a (texel, weight) list in the same LDS, and the wave
+// walks it with eight coalesced 256-byte loads in flight, accumulating in FP64.
+// A texel costs one load per window instead of one per sample and corner.
+__device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int nx, int ny,
+                                                double step, const Ray &ray, int lane,
+                                                CellRun *runs) {
+    const int pw = nx + 2;
+    const int n = __builtin_amdgcn_readfirstlane(ray.n);
+    TexW *ents = reinterpret_cast<TexW *>(runs);
+    double acc = 0.0;
+    for (int q0 = 0; q0 < n; q0 += kWave) {
+        const int nr = wave_cell_runs(ray, step, n, q0, nx, ny, lane, runs);
+        bool kept[4] = {false, false, false, false};
+        int off[4] = {0, 0, 0, 0};
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        if (lane < nr) {
+            const CellRun me = runs[lane];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                // kept corners lie inside the grid, so the padded index is in range
+                kept[c] = me.off[c] >= 0;
+                off[c] = ((me.ci + (c & 1) + 1) + pw * (me.cj + (c >> 1) + 1)) * kTexelMaps;
+                w[c] = me.w[c];
+            }
+        }
+        // entries in lane order, corners in order within a lane
+        const unsigned long long lower = (1ull << lane) - 1ull;
+        int pos = 0, nent = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const unsigned long long bal = __ballot(kept[c]);
+            pos += __popcll(bal & lower);
+            nent += __popcll(bal);
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (kept[c]) {
+                ents[pos].off = off[c];
+                ents[pos].w = w[c];
+                pos++;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        for (int e0 = 0; e0 < nent; e0 += 8) {
+            float v[8];
+            double ww[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const TexW te = ents[min(e0 + k, nent - 1)];
+                ww[k] = e0 + k < nent ? te.w : 0.0;
+                v[k] = m[(unsigned)(te.off + lane)];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc = fma(ww[k], (double)v[k], acc);
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+    }
+    return acc;
 }
 
 // Detector legs of a voxel chunk, one wave per ray, one lane per map.
 __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
-                                               int nk, float *Ld) {
+                                               int nk, float *Ld, CellRun (*cruns)[kWave]) {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     for (int j = wid; j < nk; j += NT / kWave) {
         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
                                  a.f.nx, a.f.ny, a.f.ray_step);
-        const float sum = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane);
-        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)((double)sum * a.f.ray_step);
+        const double sum = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane,
+                                         cruns[wid]);
+        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)(sum * a.f.ray_step);
     }
 }
 
@@ -427,6 +556,7 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ double red[(NT / kWave) * EPT + 64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
+    __shared__ CellRun cruns[NT / kWave][kWave];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
@@ -472,7 +602,7 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
         }
         __syncthreads();
         if (a.maps_tm)
-            trace_chunk_tm(a, pg, vox, nk, Ld);
+            trace_chunk_tm(a, pg, vox, nk, Ld, cruns);
         else
             trace_chunk(a, pg, vox, nk, part, Ld);
         // F_vox(q_n) for the chunk
@@ -586,109 +716,17 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
     flush();
 }
 
-// One run of samples in one cell (ci, cj): element offsets texel*64 of its four
-// stencil corners (-1 outside the map, or when a later run adds this corner's
-// weight) and their summed bilinear weights.
-struct CellRun {
-    int off[4];
-    int ci, cj, pad0, pad1;
-    double w[4];
-};
-
-// Texel-major form of scatter_ray for one wave.  Lanes take one sample each (64 at a
-// time) and add the bilinear weights of each run of consecutive samples in one cell
-// with a segmented wave scan; the run-end lanes write their cells to the wave's
-// CellRun list in LDS.  A stencil corner is shared by at most three consecutive
-// cells of a straight ray, so each run then folds in the weights of the two runs
-// before it for the corners it shares with them and leaves out the corners that
-// one of the next two runs shares: each texel is emitted once per 64 samples.
-// Finally the wave walks the list: for each cell and kept corner, lane b adds
+// Texel-major form of scatter_ray for one wave: for each window of 64 samples,
+// wave_cell_runs lists the window's texels once each with their summed weights,
+// and the wave walks the list: for each cell and kept corner, lane b adds
 // gamma[b] times the weight to grad_tm[texel][b] (one coalesced atomic
 // instruction).  The list keeps this walk in vector instructions.
 __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
                                                int lane, CellRun *runs) {
-    const int nx = a.f.nx, ny = a.f.ny;
     const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     for (int q0 = 0; q0 < n; q0 += kWave) {
-        const int cnt = min(kWave, n - q0);
-        int i0, j0;
-        double ax, ay;
-        lane_sample(ray, a.f.ray_step, min(q0 + lane, n - 1), i0, j0, ax, ay);
-        const bool act = lane < cnt;
-        const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
-        const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
-        const bool first = lane == 0 || pi != i0 || pj != j0;
-        const bool last = act && (lane == cnt - 1 || ni != i0 || nj != j0);
-        const unsigned long long starts = __ballot(first);
-        // first lane of this lane's run: highest start at or below it
-        const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-        const int rs = 63 - __clzll(below);
-        double w[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const double up = __shfl_up(w[c], d, kWave);
-                if (lane - d >= rs) w[c] += up;
-            }
-        }
-        const unsigned long long ends = __ballot(last);
-        const int nr = __popcll(ends);
-        if (last) {
-            CellRun cr;
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
-                const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
-                cr.off[c] = in ? (ti + nx * tj) * kTexelMaps : -1;
-                cr.w[c] = w[c];
-            }
-            cr.ci = i0;
-            cr.cj = j0;
-            cr.pad0 = cr.pad1 = 0;
-            runs[__popcll(ends & ((1ull << lane) - 1ull))] = cr;
-        }
-        // a wave's LDS accesses complete in issue order, so only the compiler must
-        // keep the list reads below the writes (a memory fence would also wait for
-        // every atomic in flight)
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
-        // lane r < nr settles run r's corners against runs r-2 .. r+2
-        int newoff[4];
-        double neww[4];
-        if (lane < nr) {
-            const CellRun me = runs[lane];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int ti = me.ci + (c & 1), tj = me.cj + (c >> 1);
-                bool later = false;
-                double wsum = me.w[c];
-#pragma unroll
-                for (int k = -2; k <= 2; k++) {
-                    if (k == 0 || lane + k < 0 || lane + k >= nr) continue;
-                    const int di = ti - runs[lane + k].ci, dj = tj - runs[lane + k].cj;
-                    if (di < 0 || di > 1 || dj < 0 || dj > 1) continue;
-                    if (k > 0)
-                        later = true;
-                    else
-                        wsum += runs[lane + k].w[di + 2 * dj];
-                }
-                newoff[c] = later ? -1 : me.off[c];
-                neww[c] = wsum;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
-        if (lane < nr) {
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                runs[lane].off[c] = newoff[c];
-                runs[lane].w[c] = neww[c];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
+        const int nr = wave_cell_runs(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, runs);
         if (gs != 0.0) {
             for (int r = 0; r < nr; r++) {
                 const CellRun cr = runs[r];
@@ -803,7 +841,7 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
             }
             __syncthreads();
             if (a.maps_tm)
-                trace_chunk_tm(a, pg, vox, nk, Ld);
+                trace_chunk_tm(a, pg, vox, nk, Ld, cruns);
             else
                 trace_chunk(a, pg, vox, nk, part, Ld);
             __syncthreads();
@@ -1002,12 +1040,12 @@ __device__ __forceinline__ bool ct_chord(double sx, double sy, double dx, double
 
 // Per ray (one wave): lane b's line integral L_b, then lane e's exp(-mu_e).
 __device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int lane, Ray &ray,
-                                           bool &hit) {
+                                           bool &hit, CellRun *runs) {
     const int E = a.f.n_energy, B = a.n_basis;
     const PixG pg = pixel_setup(a, v, c);
     hit = ct_chord(pg.sx, pg.sy, pg.dx, pg.dy, a.f.nx, a.f.ny, a.f.ray_step, ray);
     double Lb = 0.0;
-    if (hit) Lb = (double)wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane) *
+    if (hit) Lb = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane, runs) *
                   a.f.ray_step;
     const int e = min(lane, E - 1);
     double mu = 0.0;
@@ -1017,14 +1055,15 @@ __device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int
 
 // One wave per (view, column) ray; 4 rays per workgroup.
 __global__ __launch_bounds__(256) void ct_forward_kernel(CoeffArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
+    __shared__ CellRun cruns[4][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int V = a.f.n_views, COL = a.f.n_cols, E = a.f.n_energy;
-    const int ray_id = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ray_id = blockIdx.x * 4 + wid;
     if (ray_id >= V * COL) return;  // uniform per wave
     const int v = ray_id % V, c = ray_id / V;
     Ray ray;
     bool hit;
-    const double P = ct_ray_p(a, v, c, lane, ray, hit);
+    const double P = ct_ray_p(a, v, c, lane, ray, hit, cruns[wid]);
     // lane ed: sum over its source-energy window (all windows have 2 dn + 1 bins)
     const int ed = min(lane, E - 1);
     int lo, hi;
@@ -1050,7 +1089,7 @@ __global__ __launch_bounds__(256) void ct_vjp_kernel(CoeffArgs a) {
     const int v = ray_id % V, c = ray_id / V;
     Ray ray;
     bool hit;
-    const double P = ct_ray_p(a, v, c, lane, ray, hit);
+    const double P = ct_ray_p(a, v, c, lane, ray, hit, cruns[wid]);
     if (!hit) return;
     double wt = 0.0;
     if (lane < E)
