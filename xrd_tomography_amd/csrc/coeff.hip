@@ -228,25 +228,35 @@ __device__ __forceinline__ void lane_sample(const Ray &ray, double step, int q, 
     j0 = (int)fy;
 }
 
-// One run of samples in one cell (ci, cj): element offsets texel*64 of its four
-// stencil corners in the unpadded grid (-1 outside the map, or when a later run
-// of the window adds this corner's weight) and their summed bilinear weights.
-struct CellRun {
-    int off[4];
-    int ci, cj, pad0, pad1;
-    double w[4];
+// One wave's list in LDS, as arrays indexed by entry so that lane-indexed accesses
+// are free of bank conflicts and entry-indexed ones are broadcasts.
+//   runs: run r of samples in one cell (ci, cj), with per stencil corner c the element
+//         offset texel*64 in the unpadded grid (-1 outside the map, or when a later
+//         run of the window adds this corner's weight) and the summed bilinear weight;
+//   tex:  the kept corners of a window compacted, with the element offset in the
+//         padded texel-major maps; at most four per run.
+union WaveList {
+    struct {
+        double w[4][kWave];
+        int off[4][kWave];
+        int ci[kWave], cj[kWave];
+    } runs;
+    struct {
+        double w[4 * kWave];
+        int off[4 * kWave];
+    } tex;
 };
 
 // The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
-// share a bilinear cell, in ray order, in the wave's list runs[0 .. nr); returns nr.
-// A segmented wave scan adds the FP64 bilinear weights of each run, and the
-// run-end lanes write their cells.  A stencil corner is shared by at most three
+// share a bilinear cell, in ray order, as runs[0 .. nr) of the wave's list; returns
+// nr.  Each run-end lane forms its run's FP64 bilinear weight sums in closed form
+// and writes its cell.  A stencil corner is shared by at most three
 // consecutive cells of a straight ray (the cell indices are monotone along it), so
 // each run then folds in the weights of the two runs before it for the corners it
 // shares with them and drops the corners that one of the next two runs shares:
 // every texel of the window appears once, with its total weight.
 __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n, int q0, int nx,
-                                              int ny, int lane, CellRun *runs) {
+                                              int ny, int lane, WaveList &L) {
     const int cnt = min(kWave, n - q0);
     int i0, j0;
     double ax, ay;
@@ -260,30 +270,31 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     // first lane of this lane's run: highest start at or below it
     const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
     const int rs = 63 - __clzll(below);
-    double w[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const double up = __shfl_up(w[c], d, kWave);
-            if (lane - d >= rs) w[c] += up;
-        }
-    }
+    // the run's weight sums in closed form: along the run the fractions fall back
+    // from this lane's (ax, ay) by (sx, sy) per sample (positions are linear in the
+    // sample index), so with k = 0 .. nn-1 samples back,
+    //   sum ax = nn ax - sx S1,  sum ay = nn ay - sy S1,
+    //   sum ax ay = nn ax ay - (ax sy + ay sx) S1 + sx sy S2,
+    // S1 = sum k, S2 = sum k^2; only the run-end lanes' values are used
+    const double sx = ray.ux * step, sy = ray.uy * step;
+    const double nn = (double)(lane - rs + 1);
+    const double s1 = nn * (nn - 1.0) * 0.5, s2 = (nn - 1.0) * nn * (2.0 * nn - 1.0) / 6.0;
+    const double sax = nn * ax - sx * s1, say = nn * ay - sy * s1;
+    const double saxy = nn * ax * ay - (ax * sy + ay * sx) * s1 + sx * sy * s2;
+    const double w[4] = {nn - sax - say + saxy, sax - saxy, say - saxy, saxy};
     const unsigned long long ends = __ballot(last);
     const int nr = __popcll(ends);
     if (last) {
-        CellRun cr;
+        const int r = __popcll(ends & ((1ull << lane) - 1ull));
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
             const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
-            cr.off[c] = in ? (ti + nx * tj) * kTexelMaps : -1;
-            cr.w[c] = w[c];
+            L.runs.off[c][r] = in ? (ti + nx * tj) * kTexelMaps : -1;
+            L.runs.w[c][r] = w[c];
         }
-        cr.ci = i0;
-        cr.cj = j0;
-        cr.pad0 = cr.pad1 = 0;
-        runs[__popcll(ends & ((1ull << lane) - 1ull))] = cr;
+        L.runs.ci[r] = i0;
+        L.runs.cj[r] = j0;
     }
     // a wave's LDS accesses complete in issue order, so only the compiler must
     // keep the list reads below the writes (a memory fence would also wait for
@@ -294,23 +305,30 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     int newoff[4];
     double neww[4];
     if (lane < nr) {
-        const CellRun me = runs[lane];
+        const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+        int nci[5], ncj[5];
+#pragma unroll
+        for (int k = -2; k <= 2; k++) {
+            const bool ok = k != 0 && lane + k >= 0 && lane + k < nr;
+            nci[k + 2] = ok ? L.runs.ci[lane + k] : INT_MIN / 2;
+            ncj[k + 2] = ok ? L.runs.cj[lane + k] : INT_MIN / 2;
+        }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const int ti = me.ci + (c & 1), tj = me.cj + (c >> 1);
+            const int ti = ci + (c & 1), tj = cj + (c >> 1);
             bool later = false;
-            double wsum = me.w[c];
+            double wsum = L.runs.w[c][lane];
 #pragma unroll
             for (int k = -2; k <= 2; k++) {
-                if (k == 0 || lane + k < 0 || lane + k >= nr) continue;
-                const int di = ti - runs[lane + k].ci, dj = tj - runs[lane + k].cj;
+                if (k == 0) continue;
+                const int di = ti - nci[k + 2], dj = tj - ncj[k + 2];
                 if (di < 0 || di > 1 || dj < 0 || dj > 1) continue;
                 if (k > 0)
                     later = true;
                 else
-                    wsum += runs[lane + k].w[di + 2 * dj];
+                    wsum += L.runs.w[di + 2 * dj][lane + k];
             }
-            newoff[c] = later ? -1 : me.off[c];
+            newoff[c] = later ? -1 : L.runs.off[c][lane];
             neww[c] = wsum;
         }
     }
@@ -319,8 +337,8 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     if (lane < nr) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            runs[lane].off[c] = newoff[c];
-            runs[lane].w[c] = neww[c];
+            L.runs.off[c][lane] = newoff[c];
+            L.runs.w[c][lane] = neww[c];
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -328,41 +346,33 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     return nr;
 }
 
-// A kept corner of a window: element offset texel*64 in the padded texel-major maps
-// and its total weight.  Four per run at most, so the list fits in the run list's
-// space.
-struct TexW {
-    int off, pad;
-    double w;
-};
-static_assert(sizeof(TexW) * 4 == sizeof(CellRun), "texel list aliases the run list");
-
 // Line integral (before the factor step), for lane b's map, of one ray through the
 // texel-major maps: sum over texels of m[texel][b] times the texel's summed bilinear
 // weight.  Each window of 64 samples lists its texels once (wave_cell_runs), the kept
 // corners are compacted into a (texel, weight) list in the same LDS, and the wave
-// walks it with eight coalesced 256-byte loads in flight, accumulating in FP64.
+// walks it with sixteen coalesced 256-byte loads in flight, accumulating in FP64.
 // A texel costs one load per window instead of one per sample and corner.
 __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int nx, int ny,
                                                 double step, const Ray &ray, int lane,
-                                                CellRun *runs) {
+                                                WaveList &L) {
+    constexpr int U = 16;
     const int pw = nx + 2;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
-    TexW *ents = reinterpret_cast<TexW *>(runs);
-    double acc = 0.0;
+    // four partial sums break the FMA dependency chain; added in a fixed order
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q0 = 0; q0 < n; q0 += kWave) {
-        const int nr = wave_cell_runs(ray, step, n, q0, nx, ny, lane, runs);
+        const int nr = wave_cell_runs(ray, step, n, q0, nx, ny, lane, L);
         bool kept[4] = {false, false, false, false};
         int off[4] = {0, 0, 0, 0};
         double w[4] = {0.0, 0.0, 0.0, 0.0};
         if (lane < nr) {
-            const CellRun me = runs[lane];
+            const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 // kept corners lie inside the grid, so the padded index is in range
-                kept[c] = me.off[c] >= 0;
-                off[c] = ((me.ci + (c & 1) + 1) + pw * (me.cj + (c >> 1) + 1)) * kTexelMaps;
-                w[c] = me.w[c];
+                kept[c] = L.runs.off[c][lane] >= 0;
+                off[c] = ((ci + (c & 1) + 1) + pw * (cj + (c >> 1) + 1)) * kTexelMaps;
+                w[c] = L.runs.w[c][lane];
             }
         }
         // entries in lane order, corners in order within a lane
@@ -379,41 +389,37 @@ __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             if (kept[c]) {
-                ents[pos].off = off[c];
-                ents[pos].w = w[c];
+                L.tex.off[pos] = off[c];
+                L.tex.w[pos] = w[c];
                 pos++;
             }
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
-        for (int e0 = 0; e0 < nent; e0 += 8) {
-            float v[8];
-            double ww[8];
+        for (int e0 = 0; e0 < nent; e0 += U) {
+            float v[U];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const TexW te = ents[min(e0 + k, nent - 1)];
-                ww[k] = e0 + k < nent ? te.w : 0.0;
-                v[k] = m[(unsigned)(te.off + lane)];
-            }
+            for (int k = 0; k < U; k++) v[k] = m[(unsigned)(L.tex.off[min(e0 + k, nent - 1)] + lane)];
             __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
 #pragma unroll
-            for (int k = 0; k < 8; k++) acc = fma(ww[k], (double)v[k], acc);
+            for (int k = 0; k < U; k++)
+                if (e0 + k < nent) acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
     }
-    return acc;
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // Detector legs of a voxel chunk, one wave per ray, one lane per map.
 __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
-                                               int nk, float *Ld, CellRun (*cruns)[kWave]) {
+                                               int nk, float *Ld, WaveList *lists) {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     for (int j = wid; j < nk; j += NT / kWave) {
         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
                                  a.f.nx, a.f.ny, a.f.ray_step);
         const double sum = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane,
-                                         cruns[wid]);
+                                         lists[wid]);
         if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)(sum * a.f.ray_step);
     }
 }
@@ -556,7 +562,7 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ double red[(NT / kWave) * EPT + 64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
-    __shared__ CellRun cruns[NT / kWave][kWave];
+    __shared__ WaveList cruns[NT / kWave];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
@@ -722,17 +728,18 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
 // gamma[b] times the weight to grad_tm[texel][b] (one coalesced atomic
 // instruction).  The list keeps this walk in vector instructions.
 __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
-                                               int lane, CellRun *runs) {
+                                               int lane, WaveList &L) {
     const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     for (int q0 = 0; q0 < n; q0 += kWave) {
-        const int nr = wave_cell_runs(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, runs);
+        const int nr = wave_cell_runs(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, L);
         if (gs != 0.0) {
             for (int r = 0; r < nr; r++) {
-                const CellRun cr = runs[r];
 #pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (cr.off[c] >= 0) unsafeAtomicAdd(a.grad_tm + cr.off[c] + lane, gs * cr.w[c]);
+                for (int c = 0; c < 4; c++) {
+                    const int off = L.runs.off[c][r];
+                    if (off >= 0) unsafeAtomicAdd(a.grad_tm + off + lane, gs * L.runs.w[c][r]);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -770,7 +777,7 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     __shared__ double wt[64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
-    __shared__ CellRun cruns[NT / kWave][kWave];
+    __shared__ WaveList cruns[NT / kWave];
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int lane = t & (kWave - 1), wid = t >> 6;
@@ -1040,7 +1047,7 @@ __device__ __forceinline__ bool ct_chord(double sx, double sy, double dx, double
 
 // Per ray (one wave): lane b's line integral L_b, then lane e's exp(-mu_e).
 __device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int lane, Ray &ray,
-                                           bool &hit, CellRun *runs) {
+                                           bool &hit, WaveList &runs) {
     const int E = a.f.n_energy, B = a.n_basis;
     const PixG pg = pixel_setup(a, v, c);
     hit = ct_chord(pg.sx, pg.sy, pg.dx, pg.dy, a.f.nx, a.f.ny, a.f.ray_step, ray);
@@ -1055,7 +1062,7 @@ __device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int
 
 // One wave per (view, column) ray; 4 rays per workgroup.
 __global__ __launch_bounds__(256) void ct_forward_kernel(CoeffArgs a) {
-    __shared__ CellRun cruns[4][kWave];
+    __shared__ WaveList cruns[4];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int V = a.f.n_views, COL = a.f.n_cols, E = a.f.n_energy;
     const int ray_id = blockIdx.x * 4 + wid;
@@ -1081,7 +1088,7 @@ __global__ __launch_bounds__(256) void ct_forward_kernel(CoeffArgs a) {
 // sum_{ed: e in win(ed)} RF[e,ed] w[v,c,ed], scattered along the chord.
 __global__ __launch_bounds__(256) void ct_vjp_kernel(CoeffArgs a) {
     __shared__ double gam[4][kWave];
-    __shared__ CellRun cruns[4][kWave];
+    __shared__ WaveList cruns[4];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int V = a.f.n_views, COL = a.f.n_cols, E = a.f.n_energy, B = a.n_basis;
     const int ray_id = blockIdx.x * 4 + wid;
