@@ -246,6 +246,7 @@ union WaveList {
         int off[4 * kWave];
     } tex;
 };
+static_assert((4 * kWave) % 16 == 0, "the texel list holds whole blocks of 16 entries");
 
 // The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
 // share a bilinear cell, in ray order, as runs[0 .. nr) of the wave's list; returns
@@ -394,16 +395,22 @@ __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int
                 pos++;
             }
         }
+        // pad the list to a multiple of U with the zero border texel 0 at weight 0, so
+        // the walk reads whole, aligned blocks of U entries (4 * kWave is a multiple of U)
+        const int nblk = (nent + U - 1) / U * U;
+        if (nent + lane < nblk) {
+            L.tex.off[nent + lane] = 0;
+            L.tex.w[nent + lane] = 0.0;
+        }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
-        for (int e0 = 0; e0 < nent; e0 += U) {
+        for (int e0 = 0; e0 < nblk; e0 += U) {
             float v[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) v[k] = m[(unsigned)(L.tex.off[min(e0 + k, nent - 1)] + lane)];
+            for (int k = 0; k < U; k++) v[k] = m[(unsigned)(L.tex.off[e0 + k] + lane)];
             __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
 #pragma unroll
-            for (int k = 0; k < U; k++)
-                if (e0 + k < nent) acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
+            for (int k = 0; k < U; k++) acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
