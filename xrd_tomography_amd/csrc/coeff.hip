@@ -246,7 +246,6 @@ union WaveList {
         int off[4 * kWave];
     } tex;
 };
-static_assert((4 * kWave) % 16 == 0, "the texel list holds whole blocks of 16 entries");
 
 // The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
 // share a bilinear cell, in ray order, as runs[0 .. nr) of the wave's list; returns
@@ -347,70 +346,85 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     return nr;
 }
 
+// Compacts the kept corners of runs[0 .. nr) of the wave's list into its texel list,
+// in lane order and corner order within a lane: the element offset of the texel in
+// the padded texel-major maps (padded) or in the unpadded grid, and its weight.  The
+// list is padded with (0, 0.0) entries to a whole number of blocks of kTexBlock, so
+// a walk reads aligned 128-bit broadcasts; returns the entry count, nblk the padded
+// length.  Offset 0 is the zero border texel of the padded maps and a real texel of
+// the unpadded grid, so walks that scatter skip the padding.
+constexpr int kTexBlock = 16;
+static_assert((4 * kWave) % kTexBlock == 0, "the texel list holds whole blocks");
+
+__device__ __forceinline__ int compact_texels(WaveList &L, int lane, int nr, int nx, bool padded,
+                                              int &nblk) {
+    const int pw = nx + 2;
+    bool kept[4] = {false, false, false, false};
+    int off[4] = {0, 0, 0, 0};
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (lane < nr) {
+        const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int o = L.runs.off[c][lane];
+            // kept corners lie inside the grid, so the padded index is in range
+            kept[c] = o >= 0;
+            off[c] = padded ? ((ci + (c & 1) + 1) + pw * (cj + (c >> 1) + 1)) * kTexelMaps : o;
+            w[c] = L.runs.w[c][lane];
+        }
+    }
+    const unsigned long long lower = (1ull << lane) - 1ull;
+    int pos = 0, nent = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const unsigned long long bal = __ballot(kept[c]);
+        pos += __popcll(bal & lower);
+        nent += __popcll(bal);
+    }
+    // the texel list overlays the run list: every run is in registers now
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (kept[c]) {
+            L.tex.off[pos] = off[c];
+            L.tex.w[pos] = w[c];
+            pos++;
+        }
+    }
+    nblk = (nent + kTexBlock - 1) / kTexBlock * kTexBlock;
+    if (nent + lane < nblk) {
+        L.tex.off[nent + lane] = 0;
+        L.tex.w[nent + lane] = 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    return nent;
+}
+
 // Line integral (before the factor step), for lane b's map, of one ray through the
 // texel-major maps: sum over texels of m[texel][b] times the texel's summed bilinear
-// weight.  Each window of 64 samples lists its texels once (wave_cell_runs), the kept
-// corners are compacted into a (texel, weight) list in the same LDS, and the wave
-// walks it with sixteen coalesced 256-byte loads in flight, accumulating in FP64.
-// A texel costs one load per window instead of one per sample and corner.
+// weight.  Each window of 64 samples lists its texels once (wave_cell_runs,
+// compact_texels) and the wave walks the list with sixteen coalesced 256-byte loads
+// in flight, accumulating in FP64 (four partial sums, added in a fixed order).  A
+// texel costs one load per window instead of one per sample and corner.
 __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int nx, int ny,
                                                 double step, const Ray &ray, int lane,
                                                 WaveList &L) {
-    constexpr int U = 16;
-    const int pw = nx + 2;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
-    // four partial sums break the FMA dependency chain; added in a fixed order
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q0 = 0; q0 < n; q0 += kWave) {
         const int nr = wave_cell_runs(ray, step, n, q0, nx, ny, lane, L);
-        bool kept[4] = {false, false, false, false};
-        int off[4] = {0, 0, 0, 0};
-        double w[4] = {0.0, 0.0, 0.0, 0.0};
-        if (lane < nr) {
-            const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+        int nblk;
+        compact_texels(L, lane, nr, nx, true, nblk);
+        for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
+            float v[kTexBlock];
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                // kept corners lie inside the grid, so the padded index is in range
-                kept[c] = L.runs.off[c][lane] >= 0;
-                off[c] = ((ci + (c & 1) + 1) + pw * (cj + (c >> 1) + 1)) * kTexelMaps;
-                w[c] = L.runs.w[c][lane];
-            }
-        }
-        // entries in lane order, corners in order within a lane
-        const unsigned long long lower = (1ull << lane) - 1ull;
-        int pos = 0, nent = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const unsigned long long bal = __ballot(kept[c]);
-            pos += __popcll(bal & lower);
-            nent += __popcll(bal);
-        }
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (kept[c]) {
-                L.tex.off[pos] = off[c];
-                L.tex.w[pos] = w[c];
-                pos++;
-            }
-        }
-        // pad the list to a multiple of U with the zero border texel 0 at weight 0, so
-        // the walk reads whole, aligned blocks of U entries (4 * kWave is a multiple of U)
-        const int nblk = (nent + U - 1) / U * U;
-        if (nent + lane < nblk) {
-            L.tex.off[nent + lane] = 0;
-            L.tex.w[nent + lane] = 0.0;
-        }
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
-        for (int e0 = 0; e0 < nblk; e0 += U) {
-            float v[U];
-#pragma unroll
-            for (int k = 0; k < U; k++) v[k] = m[(unsigned)(L.tex.off[e0 + k] + lane)];
+            for (int k = 0; k < kTexBlock; k++) v[k] = m[(unsigned)(L.tex.off[e0 + k] + lane)];
             __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
 #pragma unroll
-            for (int k = 0; k < U; k++) acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
+            for (int k = 0; k < kTexBlock; k++)
+                acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
@@ -730,23 +744,29 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
 }
 
 // Texel-major form of scatter_ray for one wave: for each window of 64 samples,
-// wave_cell_runs lists the window's texels once each with their summed weights,
-// and the wave walks the list: for each cell and kept corner, lane b adds
-// gamma[b] times the weight to grad_tm[texel][b] (one coalesced atomic
-// instruction).  The list keeps this walk in vector instructions.
+// wave_cell_runs and compact_texels list the window's texels once each with their
+// summed weights, and the wave walks the list: lane b adds gamma[b] times the weight
+// to grad_tm[texel][b], one coalesced atomic instruction per texel.
 __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
                                                int lane, WaveList &L) {
     const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     for (int q0 = 0; q0 < n; q0 += kWave) {
         const int nr = wave_cell_runs(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, L);
+        int nblk;
+        const int nent = compact_texels(L, lane, nr, a.f.nx, false, nblk);
         if (gs != 0.0) {
-            for (int r = 0; r < nr; r++) {
+            for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
+                int off[kTexBlock];
+                double w[kTexBlock];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const int off = L.runs.off[c][r];
-                    if (off >= 0) unsafeAtomicAdd(a.grad_tm + off + lane, gs * L.runs.w[c][r]);
+                for (int k = 0; k < kTexBlock; k++) {
+                    off[k] = L.tex.off[e0 + k];
+                    w[k] = L.tex.w[e0 + k];
                 }
+#pragma unroll
+                for (int k = 0; k < kTexBlock; k++)
+                    if (e0 + k < nent) unsafeAtomicAdd(a.grad_tm + off[k] + lane, gs * w[k]);
             }
         }
         __builtin_amdgcn_wave_barrier();
