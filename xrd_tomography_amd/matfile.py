@@ -21,7 +21,10 @@ layout:
     with the deflate filter;
   * v1 attribute messages (MATLAB_class: double, single, integer and logical arrays).
 
-That subset covers every v7.3 file the reference ships. Anything outside it
+That subset covers every v7.3 file the reference ships. Their chunked datasets
+are single chunks: assembling several chunks (and walking chunk B-trees deeper than
+one level) follows the specification but no shipped file exercises it. Anything
+outside the subset
 (complex or sparse data, cell and char arrays, other filters, v2 object headers)
 raises ValueError naming what was found. Every B-tree, symbol-table and heap
 signature is checked, and every read is bounds-checked against the file.
