@@ -24,9 +24,8 @@ layout:
 That subset covers every v7.3 file the reference ships. Their chunked datasets
 are single chunks: assembling several chunks (and walking chunk B-trees deeper than
 one level) follows the specification but no shipped file exercises it. Anything
-outside the subset
-(complex or sparse data, cell and char arrays, other filters, v2 object headers)
-raises ValueError naming what was found. Every B-tree, symbol-table and heap
+outside the subset (complex or sparse data, cell and char arrays, other filters,
+v2 object headers) raises ValueError naming what was found. Every B-tree, symbol-table and heap
 signature is checked, and every read is bounds-checked against the file.
 
 MATLAB stores an array of size [d1, d2, ...] as an HDF5 dataset of dims [..., d2, d1]
