@@ -228,6 +228,17 @@ __device__ __forceinline__ void lane_sample(const Ray &ray, double step, int q, 
     j0 = (int)fy;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by id, and each XCD has its
+// own L2.  xcd_contiguous renumbers block id b of nb so that XCD x takes a contiguous
+// range of logical indices: the workgroups running together on one XCD then work on
+// neighbouring items (a bijection on [0, nb)).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, slot = b / kXcds;
+    return x * q + min(x, r) + slot;
+}
+
 // One wave's list in LDS, as arrays indexed by entry so that lane-indexed accesses
 // are free of bank conflicts and entry-indexed ones are broadcasts.
 //   runs: run r of samples in one cell (ci, cj), with per stencil corner c the element
@@ -587,7 +598,10 @@ __global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
-    const int v = blockIdx.x % V, p = blockIdx.x / V;
+    // neighbouring pixels of one view run together on an XCD: their ray fans from a
+    // voxel chunk cross nearly the same texels, which stay in that XCD's L2
+    const int l = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int v = l / f.n_det, p = l % f.n_det;
     const size_t npix = (size_t)f.nx * f.ny;
     const PixG pg = pixel_setup(a, v, p);
     float qch = 1e30f, qcl = 0.f, dqc = 0.f;
