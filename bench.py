@@ -424,11 +424,13 @@ def run_volume(args, dev):
     energies, 10 materials): the slice's explicit model matrix (the attenuation legs are
     3-D segments through the whole volume), its counts against the FP64 oracle on one
     view, and the 30-iteration MLEM image with A stored in FP32, BF24, BF16 and F16
-    against the FP64 oracle MLEM on the same FP32 matrix (36 of the 720 views)."""
+    against the FP64 oracle MLEM on the same FP32 matrix (36 of the 720 views).  The
+    counts are also reported with the form-factor and attenuation tables in BF16 with
+    a per-material FP32 scale."""
     import numpy as np
     import torch
     from xrd_tomography_amd import api
-    from xrd_tomography_amd.synthetic import make_volume_problem
+    from xrd_tomography_amd.synthetic import make_volume_problem, with_bf16_tables
     from oracle import oracle as O
 
     nz, sl = 32, args.volume_slice
@@ -456,7 +458,7 @@ def run_volume(args, dev):
     y = torch.from_numpy(np.random.default_rng(11).poisson(expected).astype(np.float32)).to(dev)
 
     # counts against the FP64 oracle on one view (all pixels, energies)
-    cnt_err = None
+    cnt_err = cnt_err_bf16 = None
     if not args.no_cpu:
         t0 = time.perf_counter()
         No, Co, _ = O.forward_slice(vp, nz, sl, views=[0], with_A=False)
@@ -467,6 +469,15 @@ def run_volume(args, dev):
         cnt_err = float(np.max(np.abs(Ng[live] - No[live]) / No[live]))
         log(f"[vol] counts of view 0 against FP64: max rel err {cnt_err:.2e} "
             f"(oracle {t_oracle:.1f} s)")
+        # the same with the form-factor and attenuation tables in BF16 with a
+        # per-material FP32 scale, against the FP64 counts of the FP32 tables
+        Nq, _, _ = api.forward_volume_slice(with_bf16_tables(vp), nz, sl, with_A=False)
+        Nq = Nq.view(d.n_energy, g.n_rows * g.n_cols, g.n_views)[:, :, 0].reshape(-1)
+        Nq = Nq.double().cpu().numpy()
+        cnt_err_bf16 = float(np.max(np.abs(Nq[live] - No[live]) / No[live]))
+        del Nq
+        log(f"[vol] counts of view 0 with BF16 tables against FP64: max rel err "
+            f"{cnt_err_bf16:.2e}")
 
     # MLEM image per storage format on 36 of the 720 views, against FP64 MLEM
     views = np.linspace(0, g.n_views - 1, 36).round().astype(np.int64)
@@ -524,6 +535,7 @@ def run_volume(args, dev):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_it / ms_it / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
         "counts_rel_err_vs_fp64_view0": cnt_err,
+        "counts_rel_err_bf16_tables_vs_fp64_view0": cnt_err_bf16,
         "mlem30_rel_err_vs_fp64": study,
         "mlem_sample": "36 of 720 views of the slice; FP64 oracle MLEM on the FP32 matrix",
     }

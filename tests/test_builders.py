@@ -401,3 +401,25 @@ def test_paper_fixture_rebuilds_from_the_reference(tmp_path):
         assert np.array_equal(getattr(p, n), getattr(q, n)), n
     assert p.lut == q.lut and p.compton_scale == q.compton_scale
     assert p.geometry.as_dict() == q.geometry.as_dict() and p.disc.as_dict() == q.disc.as_dict()
+
+
+# ---------------------------------------------------------------------------
+# BF16 tables (with_bf16_tables)
+
+def test_bf16_round_matches_torch_and_bounds_tables():
+    """bf16_round is round-to-nearest-even to bfloat16 (torch's conversion, including
+    ties and negative values); with_bf16_tables moves each form-factor entry by at most
+    2^-8 of itself and leaves per-material-constant maps unchanged."""
+    import torch
+    from xrd_tomography_amd.synthetic import bf16_round, with_bf16_tables
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(size=20000) * 10.0 ** rng.integers(-20, 20, 20000),
+                        [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 0.0]])
+    x = x.astype(np.float32)
+    assert np.array_equal(bf16_round(x), torch.from_numpy(x).to(torch.bfloat16).float().numpy())
+    p = make_problem("small")
+    q = with_bf16_tables(p)
+    rel = np.abs(q.ifine.astype(np.float64) - p.ifine) / np.abs(p.ifine)
+    assert rel.max() <= 2.0 ** -8 * (1 + 1e-6) and rel.max() > 2.0 ** -12
+    assert np.array_equal(q.a1, p.a1) and np.array_equal(q.a2, p.a2)
+    assert np.array_equal(q.spectrum, p.spectrum) and np.array_equal(q.s2_table, p.s2_table)

@@ -115,6 +115,33 @@ def test_forward_paper100(xrd, oracle, flags):
     _compare(f"paper100 flags={flags}", Ng, Cg, Ag, No, Co, Ao)
 
 
+def test_forward_bf16_tables(xrd, oracle):
+    """Form-factor and attenuation tables stored as BF16 with a per-material FP32 scale
+    (with_bf16_tables).  The GPU matches the FP64 oracle on the same rounded tables to
+    the bars above.  A does not depend on the form factors and the per-material maps
+    are unchanged, so A is bit-identical to the FP32-table A.  Each form-factor entry
+    moves by at most 2^-8 of itself and N is a positive combination of the entries, so
+    N moves by at most 2^-8 from the FP32-table counts."""
+    import torch
+    from xrd_tomography_amd.synthetic import make_problem, with_bf16_tables
+    p = make_problem("small")
+    q = with_bf16_tables(p)
+    assert np.array_equal(q.a1, p.a1) and np.array_equal(q.a2, p.a2)
+    assert not np.array_equal(q.ifine, p.ifine)
+    Ng, Cg, Ag = _run(xrd, q)
+    No, Co, Ao = oracle.forward(q, nthreads=4)
+    _compare("small, BF16 tables", Ng, Cg, Ag, No, Co, Ao)
+    _, _, Ap = xrd.forward_problem(p)
+    _, _, Aq = xrd.forward_problem(q)
+    torch.cuda.synchronize()
+    assert torch.equal(Ap, Aq)
+    Np, _, _ = oracle.forward(p, with_A=False, nthreads=4)
+    live = Np > 1e-9 * Np.max()
+    rel = float(np.max(np.abs(np.asarray(Ng, np.float64)[live] - Np[live]) / Np[live]))
+    print(f"[BF16 tables] counts vs FP64 on the FP32 tables: {rel:.2e} (bound 2^-8 = 3.9e-3)")
+    assert rel <= 2.0 ** -8 + RTOL_COUNTS
+
+
 def _host(prob, N, C, A):
     import torch
     torch.cuda.synchronize()

@@ -326,6 +326,40 @@ def _paper100(**overrides) -> Problem:
     return prob
 
 
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 values rounded to the nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(np.shape(x))
+
+
+def with_bf16_tables(prob: Problem) -> Problem:
+    """A copy of prob whose form-factor table and attenuation maps are stored as
+    bfloat16 with a per-material FP32 scale s_m: ifine[:, m] = s_m bf16(ifine[:, m] /
+    s_m) with s_m the column's largest magnitude, and a1, a2 likewise over material m's
+    voxels.  BF16 keeps 8 significant bits, so round-to-nearest moves every table
+    entry by at most 2^-8 of itself; the projector then computes in FP32 with exactly
+    these values.  Maps that are constant per material (the phantoms here) are
+    unchanged."""
+    ifine = np.array(prob.ifine, dtype=np.float32)
+    for m in range(ifine.shape[1]):
+        sm = np.float32(np.abs(ifine[:, m]).max())
+        if sm > 0:
+            ifine[:, m] = bf16_round(ifine[:, m] / sm) * sm
+    maps = []
+    for name in ("a1", "a2"):
+        a = np.array(getattr(prob, name), dtype=np.float32)
+        for m in np.unique(prob.segmentation):
+            if m <= 0:
+                continue
+            sel = prob.segmentation == m
+            sm = np.float32(np.abs(a[sel]).max())
+            if sm > 0:
+                a[sel] = bf16_round(a[sel] / sm) * sm
+        maps.append(a)
+    return dataclasses.replace(prob, ifine=ifine, a1=maps[0], a2=maps[1])
+
+
 def make_volume_problem(name: str = "small", n_slices: int = 4, seed: Optional[int] = None,
                         **overrides) -> Problem:
     """A volume of n_slices slices (one voxel apart) for xrd_forward_slice: the
