@@ -201,14 +201,10 @@ __device__ __forceinline__ void trace_chunk(const CoeffArgs &a, const PixG &pg, 
 
 // Texel-major layout for many maps: maps_tm[(i+1) + (nx+2)(j+1)][b] holds map b at
 // texel (i, j), 64 floats per texel (zero for b >= B and on the border).  One wave
-// per ray, one lane per map: each sample is four coalesced 256-byte loads, where
-// the planar layout would touch one cache line per map.
+// per ray, one lane per map: each texel a ray touches is one coalesced 256-byte load,
+// where the planar layout would touch one cache line per map.
 constexpr int kTexelMaps = 64;
 
-__device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ float read_lane(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
 __device__ __forceinline__ double read_lane(double v, int l) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                             __builtin_amdgcn_readlane(__double2loint(v), l));
