@@ -64,7 +64,8 @@ def main():
                     help="slice of the vol192 workload's 32-slice volume")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device-to-device copy 10 times (calibrates HBM "
-                         "byte counters: 2 GiB read + 2 GiB written per launch)")
+                         "byte counters: 2 GiB read + 2 GiB written per launch) and a "
+                         "read-only sum over 20 GiB (a plain streaming-read rate)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the MLEM path would "
@@ -90,9 +91,22 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / 10
+        # a read-only reduction of a 20 GiB array (the size of the fan256 FP32 matrix)
+        big = torch.ones(5 << 30, dtype=torch.float32, device=dev)
+        big.sum()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(5):
+            big.sum()
+        e1.record()
+        torch.cuda.synchronize()
+        ms_read = e0.elapsed_time(e1) / 5
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
                           "ms_per_copy": round(ms, 4),
-                          "copy_GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}))
+                          "copy_GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1),
+                          "sum_probe_bytes_read": big.numel() * 4,
+                          "ms_per_sum": round(ms_read, 4),
+                          "sum_GBps_read": round(big.numel() * 4 / ms_read / 1e6, 1)}))
         return
     if args.workload == "batch48":
         run_batch(args, dev)
