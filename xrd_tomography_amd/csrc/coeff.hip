@@ -15,6 +15,7 @@
 // respect to the coefficients are accumulated per voxel in registers
 // (deterministic); gradients with respect to the maps are scattered along the
 // rays with FP64 atomics (order-dependent only at the 1e-16 level).
+#include <algorithm>
 #include <climits>
 
 #include "xrd_device.hpp"
@@ -253,6 +254,14 @@ union WaveList {
         int off[4 * kWave];
     } tex;
 };
+
+// Orders a wave's LDS accesses around a list update.  A wave's LDS accesses
+// complete in issue order, so only the compiler must keep the reads below the
+// writes (a memory fence would also wait for every global access in flight).
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
 
 // The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
 // share a bilinear cell, in ray order, as runs[0 .. nr) of the wave's list; returns
@@ -784,6 +793,146 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
     }
 }
 
+// Detector legs of the map VJP in bundles of kBundle adjacent pixels: one wave per
+// (view of the slab, voxel, pixel bundle).  Rays from one voxel to adjacent pixels
+// stay within a small fraction of a texel of each other across the grid, so in each
+// 64-sample window their texel lists nearly coincide.  The wave lists each ray's
+// texels (wave_cell_runs) and its run lanes add their kept corners' weights into an
+// LDS box over the window's texels, one weight column per ray; then every occupied
+// texel is emitted
+// once, in box order: lane b adds sum_i w_i gamma_i[b] (FP64, rays in order) to
+// grad_tm[texel][b].  That is one atomic instruction per texel and bundle instead of
+// one per texel and ray.  A window whose box would exceed kBoxN texels scatters each
+// ray's list directly.
+constexpr int kBundle = 8, kBoxSide = 24, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
+
+struct BundleBox {
+    float w[kBoxN][kBundle];  // per texel of the box, one weight per ray of the bundle
+    int occ[kBoxN];           // texel touched by some ray of the bundle in this window
+    int list[kBoxN];          // the touched texels, in box order
+};
+
+__global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kernel(CoeffArgs a,
+                                                                                  int nb_pix) {
+    __shared__ WaveList lists[kBundleWaves];
+    __shared__ BundleBox boxes[kBundleWaves];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int nv = a.n_vox, P = a.f.n_det, nx = a.f.nx, ny = a.f.ny;
+    const double step = a.f.ray_step;
+    BundleBox &bx = boxes[wid];
+    WaveList &L = lists[wid];
+    for (int sl = lane; sl < kBoxN; sl += kWave) {
+        bx.occ[sl] = 0;
+#pragma unroll
+        for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
+    }
+    wave_lds_order();
+    const int64_t item = (int64_t)blockIdx.x * kBundleWaves + wid;
+    if (item >= (int64_t)a.v_count * nv * nb_pix) return;  // uniform per wave
+    const int bnd = (int)(item % nb_pix);
+    const int k = (int)((item / nb_pix) % nv);
+    const int vs = (int)(item / ((int64_t)nb_pix * nv));
+    const int v = a.v_base + vs;
+    const int pix = a.vlist[k];
+    const double vx = (double)(pix % nx) + 0.5, vy = (double)(pix / nx) + 0.5;
+    Ray ray[kBundle];
+    double g[kBundle];
+    int n[kBundle];
+    int nmax = 0;
+#pragma unroll
+    for (int i = 0; i < kBundle; i++) {
+        const int p = bnd * kBundle + i;  // pixel 0 of a bundle always exists
+        const PixG pg = pixel_setup(a, v, p < P ? p : bnd * kBundle);
+        ray[i] = make_ray(pg.dx, pg.dy, vx, vy, nx, ny, step);
+        n[i] = p < P ? __builtin_amdgcn_readfirstlane(ray[i].n) : 0;
+        g[i] = p < P ? (double)a.gam_buf[(((size_t)vs * nv + (size_t)k) * P + p) * kTexelMaps + lane] *
+                           step
+                     : 0.0;
+        nmax = max(nmax, n[i]);
+    }
+    const unsigned long long lower = (1ull << lane) - 1ull;
+    for (int q0 = 0; q0 < nmax; q0 += kWave) {
+        // the window's cells are bounded by those of each ray's first and last sample
+        // in it (cell indices are monotone along a ray); corners add one, and only
+        // texels inside the grid are kept
+        int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
+#pragma unroll
+        for (int i = 0; i < kBundle; i++) {
+            if (q0 >= n[i]) continue;
+            const int qe[2] = {q0, min(q0 + kWave - 1, n[i] - 1)};
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                int ci, cj;
+                double ax, ay;
+                lane_sample(ray[i], step, qe[e], ci, cj, ax, ay);
+                x0 = min(x0, ci);
+                x1 = max(x1, ci + 1);
+                y0 = min(y0, cj);
+                y1 = max(y1, cj + 1);
+            }
+        }
+        x0 = max(x0, 0);
+        y0 = max(y0, 0);
+        x1 = min(x1, nx - 1);
+        y1 = min(y1, ny - 1);
+        const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+        const bool boxed = bw > 0 && bh > 0 && bw * bh <= kBoxN;
+#pragma unroll
+        for (int i = 0; i < kBundle; i++) {
+            if (q0 >= n[i]) continue;
+            const int nr = wave_cell_runs(ray[i], step, n[i], q0, nx, ny, lane, L);
+            if (boxed) {
+                // each kept corner is a texel of its own within this ray's window, so the
+                // run lanes write their corners straight into the ray's box column
+                if (lane < nr) {
+                    const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (L.runs.off[c][lane] < 0) continue;
+                        const int sl = (ci + (c & 1) - x0) + bw * (cj + (c >> 1) - y0);
+                        bx.w[sl][i] = (float)L.runs.w[c][lane];
+                        bx.occ[sl] = 1;
+                    }
+                }
+            } else {
+                int nblk;
+                const int nent = compact_texels(L, lane, nr, nx, false, nblk);
+                if (g[i] != 0.0)
+                    for (int e = 0; e < nent; e++)
+                        unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g[i] * L.tex.w[e]);
+            }
+            wave_lds_order();
+        }
+        if (!boxed) continue;
+        // the touched texels in box order, then one atomic instruction each
+        int cnt = 0;
+        for (int s0 = 0; s0 < bw * bh; s0 += kWave) {
+            const int sl = s0 + lane;
+            const bool o = sl < bw * bh && bx.occ[sl] != 0;
+            const unsigned long long bal = __ballot(o);
+            if (o) bx.list[cnt + __popcll(bal & lower)] = sl;
+            cnt += __popcll(bal);
+        }
+        wave_lds_order();
+        for (int f = 0; f < cnt; f++) {
+            const int sl = bx.list[f];
+            double val = 0.0;
+#pragma unroll
+            for (int i = 0; i < kBundle; i++) val = fma((double)bx.w[sl][i], g[i], val);
+            const int ti = x0 + sl % bw, tj = y0 + sl / bw;
+            if (val != 0.0) unsafeAtomicAdd(a.grad_tm + (size_t)(ti + nx * tj) * kTexelMaps + lane, val);
+        }
+        wave_lds_order();
+        for (int f = lane; f < cnt; f += kWave) {
+            const int sl = bx.list[f];
+            bx.occ[sl] = 0;
+#pragma unroll
+            for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
+        }
+        wave_lds_order();
+    }
+}
+
 // Lane r of sp lanes that share a ray takes a contiguous run of its samples.
 __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1) {
     const int per = (n + sp - 1) / sp;
@@ -824,7 +973,9 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     const int k0 = blockIdx.x * KC;
     if (k0 >= nv) return;  // uniform per block
     const int nk = min(KC, nv - k0);
-    const int va = blockIdx.y * views_per_group, vb = min(V, va + views_per_group);
+    // views [v_base, v_base + v_count) of a slab (v_count 0: every view)
+    const int vend = a.v_count > 0 ? a.v_base + a.v_count : V;
+    const int va = a.v_base + blockIdx.y * views_per_group, vb = min(vend, va + views_per_group);
     const bool maps = a.grad_maps != nullptr;
     float qch = 1e30f, qcl = 0.f, dqc = 0.f;
     if (tq < Q) {
@@ -946,8 +1097,15 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
                     gsrc[j * MAXB + b] += g;
                 }
                 __syncthreads();
-                // scatter gamma along the detector legs
-                if (a.grad_tm) {
+                // scatter gamma along the detector legs, or hand it to the bundled
+                // scatter: gam_buf[view in slab][voxel][pixel][b]
+                if (a.gam_buf) {
+                    for (int idx = t; idx < nk * kTexelMaps; idx += NT) {
+                        const int j = idx / kTexelMaps, b = idx % kTexelMaps;
+                        a.gam_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * P + p) *
+                                      kTexelMaps + b] = b < B ? (float)gam[j * MAXB + b] : 0.f;
+                    }
+                } else if (a.grad_tm) {
                     for (int j = wid; j < nk; j += NT / kWave) {
                         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5,
                                                  (double)vox[j].iy + 0.5, f.nx, f.ny, f.ray_step);
@@ -1009,7 +1167,9 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     }
     if (a.grad_coef)
         for (int idx = t; idx < nk * TQ; idx += NT)
-            a.hpart[((size_t)blockIdx.y * a.nvox_cap + k0) * TQ + idx] = hred[idx];
+            a.hpart[((size_t)blockIdx.y * a.nvox_cap + k0) * TQ + idx] =
+                (a.hpart_add ? a.hpart[((size_t)blockIdx.y * a.nvox_cap + k0) * TQ + idx] : 0.0) +
+                hred[idx];
 }
 
 // One workgroup of TQ lanes per voxel: h(q_n) = sum of the view groups' partials in
@@ -1239,6 +1399,37 @@ hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
         if (a.grad_maps && a.grad_tm)
             hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+size_t coeff_gam_floats_per_view(int n_vox, int n_det) {
+    return (size_t)n_vox * (size_t)n_det * kTexelMaps;
+}
+
+hipError_t launch_coeff_vjp_bundled(CoeffArgs a, size_t gam_floats, hipStream_t s) {
+    const int V = a.f.n_views, P = a.f.n_det;
+    const size_t per_view = coeff_gam_floats_per_view(a.n_vox, P);
+    size_t fit = per_view > 0 ? gam_floats / per_view : (size_t)V;
+    const int slab = (int)std::max<size_t>(1, std::min<size_t>((size_t)V, fit));
+    const int ng = coeff_vjp_groups(slab, a.nvox_cap);
+    const int nb = (a.nvox_cap + KC - 1) / KC;
+    const int nbp = (P + kBundle - 1) / kBundle;
+    if (nb <= 0) return hipSuccess;
+    for (int v0 = 0; v0 < V; v0 += slab) {
+        a.v_base = v0;
+        a.v_count = std::min(slab, V - v0);
+        a.hpart_add = v0 > 0;
+        const int ngs = std::min(ng, a.v_count);
+        const int vpg = (a.v_count + ngs - 1) / ngs;
+        hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
+        const int64_t items = (int64_t)a.v_count * a.n_vox * nbp;
+        const int64_t blocks = (items + kBundleWaves - 1) / kBundleWaves;
+        if (blocks > 0)
+            hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3((unsigned)blocks),
+                               dim3(kBundleWaves * kWave), 0, s, a, nbp);
+    }
+    if (a.grad_coef) hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
+    hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

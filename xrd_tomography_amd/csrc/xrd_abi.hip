@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -69,7 +70,7 @@ struct xrd_context {
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
-    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm;
+    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm, cm_gam;
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -191,7 +192,7 @@ int xrd_context_destroy(xrd_context *c) {
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
                       &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial, &c->small, &c->cm_maps,
-                      &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm})
+                      &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm, &c->cm_gam})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
@@ -206,7 +207,7 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
                             &c->offsets, &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial,
                             &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h,
-                            &c->cm_tm, &c->cm_gtm})
+                            &c->cm_tm, &c->cm_gtm, &c->cm_gam})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -1002,6 +1003,24 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
             HIP_TRY(launch_zero_f64(ca.grad_tm, npix * 64, s), "zero kernel");
         } else {
             HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
+        }
+    }
+    if (grad_maps && ca.maps_tm) {
+        // detector legs scattered in bundles of adjacent pixels, views in slabs whose
+        // gamma buffer (FP32, voxels x pixels x 64 per view) fits in kGamBudget bytes
+        constexpr size_t kGamBudget = (size_t)8 << 30;
+        int nv = 0;
+        HIP_TRY(hipMemcpyAsync(&nv, ca.nvox, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+        HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+        ca.n_vox = nv;
+        if (nv > 0) {
+            const size_t per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
+            const size_t views = std::max<size_t>(
+                1, std::min<size_t>((size_t)g->n_views, kGamBudget / (per_view * sizeof(float))));
+            HIP_TRY(ctx->cm_gam.ensure(views * per_view * sizeof(float)), "workspace alloc");
+            ca.gam_buf = (float *)ctx->cm_gam.ptr;
+            HIP_TRY(launch_coeff_vjp_bundled(ca, views * per_view, s), "bundled map VJP");
+            return XRD_OK;
         }
     }
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");

@@ -129,11 +129,21 @@ struct CoeffArgs {
     double *grad_tm;         // [nx*ny x 64] map gradient while accumulating, or NULL
     int nvox_cap;            // nx*ny: bound of the voxel list
     double *hpart;           // coeff_hpart_doubles(V, nx*ny) when grad_coef is set
+    // map VJP in view slabs: the VJP launch covers views [v_base, v_base + v_count),
+    // adds into hpart when hpart_add, and writes the detector legs' gamma to gam_buf
+    // ([v_count x nv x n_det x 64] FP32, voxel-list index nv = n_vox) for the
+    // bundled scatter instead of scattering them itself (NULL: scatter in place)
+    int v_base, v_count, hpart_add, n_vox;
+    float *gam_buf;
 };
 
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
+// Map VJP with the detector legs scattered in bundles of adjacent pixels: views in
+// slabs whose gamma buffer fits in gam_floats floats (a.gam_buf); a.n_vox set.
+hipError_t launch_coeff_vjp_bundled(CoeffArgs a, size_t gam_floats, hipStream_t s);
+size_t coeff_gam_floats_per_view(int n_vox, int n_det);
 size_t coeff_hpart_doubles(int n_views, int npix);
 bool coeff_texel_major(int n_basis);
 hipError_t launch_ct_prepare(const CoeffArgs &a, hipStream_t s);
