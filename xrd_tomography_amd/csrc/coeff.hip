@@ -1410,7 +1410,10 @@ hipError_t launch_coeff_vjp_bundled(CoeffArgs a, size_t gam_floats, hipStream_t 
     const int V = a.f.n_views, P = a.f.n_det;
     const size_t per_view = coeff_gam_floats_per_view(a.n_vox, P);
     size_t fit = per_view > 0 ? gam_floats / per_view : (size_t)V;
-    const int slab = (int)std::max<size_t>(1, std::min<size_t>((size_t)V, fit));
+    const int most = (int)std::max<size_t>(1, std::min<size_t>((size_t)V, fit));
+    // equal slabs (the last one no thinner than the rest), each within the buffer
+    const int n_slabs = (V + most - 1) / most;
+    const int slab = (V + n_slabs - 1) / n_slabs;
     const int ng = coeff_vjp_groups(slab, a.nvox_cap);
     const int nb = (a.nvox_cap + KC - 1) / KC;
     const int nbp = (P + kBundle - 1) / kBundle;
