@@ -263,16 +263,26 @@ __device__ __forceinline__ void wave_lds_order() {
     asm volatile("" ::: "memory");
 }
 
+// Run r of a window, settled, in lane r's registers: its cell and, per stencil
+// corner, the element offset texel*64 in the unpadded grid (-1 outside the map or
+// when a later run adds this corner's weight) and the total weight.
+struct SettledRun {
+    int ci, cj;
+    int off[4];
+    double w[4];
+};
+
 // The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
-// share a bilinear cell, in ray order, as runs[0 .. nr) of the wave's list; returns
-// nr.  Each run-end lane forms its run's FP64 bilinear weight sums in closed form
-// and writes its cell.  A stencil corner is shared by at most three
-// consecutive cells of a straight ray (the cell indices are monotone along it), so
-// each run then folds in the weights of the two runs before it for the corners it
-// shares with them and drops the corners that one of the next two runs shares:
-// every texel of the window appears once, with its total weight.
-__device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n, int q0, int nx,
-                                              int ny, int lane, WaveList &L) {
+// share a bilinear cell, in ray order; returns their number nr, run r settled in lane
+// r's registers.  Each run-end lane forms its run's FP64 bilinear weight sums in
+// closed form and writes its cell to the wave's run list.  A stencil corner is
+// shared by at most three consecutive cells of a straight ray (the cell indices are
+// monotone along it), so each run then folds in the weights of the two runs before it
+// for the corners it shares with them and drops the corners that one of the next two
+// runs shares: every texel of the window appears once, with its total weight.  The
+// run list is free again on return.
+__device__ __forceinline__ int cell_runs_settled(const Ray &ray, double step, int n, int q0, int nx,
+                                                 int ny, int lane, WaveList &L, SettledRun &sr) {
     const int cnt = min(kWave, n - q0);
     int i0, j0;
     double ax, ay;
@@ -318,10 +328,16 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
     __builtin_amdgcn_wave_barrier();
     asm volatile("" ::: "memory");
     // lane r < nr settles run r's corners against runs r-2 .. r+2
-    int newoff[4];
-    double neww[4];
+    sr.ci = sr.cj = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        sr.off[c] = -1;
+        sr.w[c] = 0.0;
+    }
     if (lane < nr) {
         const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+        sr.ci = ci;
+        sr.cj = cj;
         int nci[5], ncj[5];
 #pragma unroll
         for (int k = -2; k <= 2; k++) {
@@ -344,27 +360,17 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
                 else
                     wsum += L.runs.w[di + 2 * dj][lane + k];
             }
-            newoff[c] = later ? -1 : L.runs.off[c][lane];
-            neww[c] = wsum;
+            sr.off[c] = later ? -1 : L.runs.off[c][lane];
+            sr.w[c] = wsum;
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
-    if (lane < nr) {
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            L.runs.off[c][lane] = newoff[c];
-            L.runs.w[c][lane] = neww[c];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
+    wave_lds_order();
     return nr;
 }
 
-// Compacts the kept corners of runs[0 .. nr) of the wave's list into its texel list,
-// in lane order and corner order within a lane: the element offset of the texel in
-// the padded texel-major maps (padded) or in the unpadded grid, and its weight.  The
+// Compacts the kept corners of the settled runs into the wave's texel list, in lane
+// order and corner order within a lane: the element offset of the texel in the
+// padded texel-major maps (padded) or in the unpadded grid, and its weight.  The
 // list is padded with (0, 0.0) entries to a whole number of blocks of kTexBlock, so
 // a walk reads aligned 128-bit broadcasts; returns the entry count, nblk the padded
 // length.  Offset 0 is the zero border texel of the padded maps and a real texel of
@@ -372,22 +378,17 @@ __device__ __forceinline__ int wave_cell_runs(const Ray &ray, double step, int n
 constexpr int kTexBlock = 16;
 static_assert((4 * kWave) % kTexBlock == 0, "the texel list holds whole blocks");
 
-__device__ __forceinline__ int compact_texels(WaveList &L, int lane, int nr, int nx, bool padded,
-                                              int &nblk) {
+__device__ __forceinline__ int compact_texels(WaveList &L, int lane, const SettledRun &sr, int nx,
+                                              bool padded, int &nblk) {
     const int pw = nx + 2;
-    bool kept[4] = {false, false, false, false};
-    int off[4] = {0, 0, 0, 0};
-    double w[4] = {0.0, 0.0, 0.0, 0.0};
-    if (lane < nr) {
-        const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+    bool kept[4];
+    int off[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int o = L.runs.off[c][lane];
-            // kept corners lie inside the grid, so the padded index is in range
-            kept[c] = o >= 0;
-            off[c] = padded ? ((ci + (c & 1) + 1) + pw * (cj + (c >> 1) + 1)) * kTexelMaps : o;
-            w[c] = L.runs.w[c][lane];
-        }
+    for (int c = 0; c < 4; c++) {
+        // kept corners lie inside the grid, so the padded index is in range
+        kept[c] = sr.off[c] >= 0;
+        off[c] = padded ? ((sr.ci + (c & 1) + 1) + pw * (sr.cj + (c >> 1) + 1)) * kTexelMaps
+                        : sr.off[c];
     }
     const unsigned long long lower = (1ull << lane) - 1ull;
     int pos = 0, nent = 0;
@@ -397,14 +398,11 @@ __device__ __forceinline__ int compact_texels(WaveList &L, int lane, int nr, int
         pos += __popcll(bal & lower);
         nent += __popcll(bal);
     }
-    // the texel list overlays the run list: every run is in registers now
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         if (kept[c]) {
             L.tex.off[pos] = off[c];
-            L.tex.w[pos] = w[c];
+            L.tex.w[pos] = sr.w[c];
             pos++;
         }
     }
@@ -420,7 +418,7 @@ __device__ __forceinline__ int compact_texels(WaveList &L, int lane, int nr, int
 
 // Line integral (before the factor step), for lane b's map, of one ray through the
 // texel-major maps: sum over texels of m[texel][b] times the texel's summed bilinear
-// weight.  Each window of 64 samples lists its texels once (wave_cell_runs,
+// weight.  Each window of 64 samples lists its texels once (cell_runs_settled,
 // compact_texels) and the wave walks the list with sixteen coalesced 256-byte loads
 // in flight, accumulating in FP64 (four partial sums, added in a fixed order).  A
 // texel costs one load per window instead of one per sample and corner.
@@ -430,9 +428,10 @@ __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q0 = 0; q0 < n; q0 += kWave) {
-        const int nr = wave_cell_runs(ray, step, n, q0, nx, ny, lane, L);
+        SettledRun sr;
+        cell_runs_settled(ray, step, n, q0, nx, ny, lane, L, sr);
         int nblk;
-        compact_texels(L, lane, nr, nx, true, nblk);
+        compact_texels(L, lane, sr, nx, true, nblk);
         for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
             float v[kTexBlock];
 #pragma unroll
@@ -763,7 +762,7 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
 }
 
 // Texel-major form of scatter_ray for one wave: for each window of 64 samples,
-// wave_cell_runs and compact_texels list the window's texels once each with their
+// cell_runs_settled and compact_texels list the window's texels once each with their
 // summed weights, and the wave walks the list: lane b adds gamma[b] times the weight
 // to grad_tm[texel][b], one coalesced atomic instruction per texel.
 __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ray, const double *gam,
@@ -771,9 +770,10 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
     const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
     for (int q0 = 0; q0 < n; q0 += kWave) {
-        const int nr = wave_cell_runs(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, L);
+        SettledRun sr;
+        cell_runs_settled(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, L, sr);
         int nblk;
-        const int nent = compact_texels(L, lane, nr, a.f.nx, false, nblk);
+        const int nent = compact_texels(L, lane, sr, a.f.nx, false, nblk);
         if (gs != 0.0) {
             for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
                 int off[kTexBlock];
@@ -797,7 +797,7 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
 // (view of the slab, voxel, pixel bundle).  Rays from one voxel to adjacent pixels
 // stay within a small fraction of a texel of each other across the grid, so in each
 // 64-sample window their texel lists nearly coincide.  The wave lists each ray's
-// texels (wave_cell_runs) and its run lanes add their kept corners' weights into an
+// texels (cell_runs_settled) and its run lanes add their kept corners' weights into an
 // LDS box over the window's texels, one weight column per ray; then every occupied
 // texel is emitted
 // once, in box order: lane b adds sum_i w_i gamma_i[b] (FP64, rays in order) to
@@ -880,23 +880,21 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
 #pragma unroll
         for (int i = 0; i < kBundle; i++) {
             if (q0 >= n[i]) continue;
-            const int nr = wave_cell_runs(ray[i], step, n[i], q0, nx, ny, lane, L);
+            SettledRun sr;
+            cell_runs_settled(ray[i], step, n[i], q0, nx, ny, lane, L, sr);
             if (boxed) {
                 // each kept corner is a texel of its own within this ray's window, so the
                 // run lanes write their corners straight into the ray's box column
-                if (lane < nr) {
-                    const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
 #pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        if (L.runs.off[c][lane] < 0) continue;
-                        const int sl = (ci + (c & 1) - x0) + bw * (cj + (c >> 1) - y0);
-                        bx.w[sl][i] = (float)L.runs.w[c][lane];
-                        bx.occ[sl] = 1;
-                    }
+                for (int c = 0; c < 4; c++) {
+                    if (sr.off[c] < 0) continue;
+                    const int sl = (sr.ci + (c & 1) - x0) + bw * (sr.cj + (c >> 1) - y0);
+                    bx.w[sl][i] = (float)sr.w[c];
+                    bx.occ[sl] = 1;
                 }
             } else {
                 int nblk;
-                const int nent = compact_texels(L, lane, nr, nx, false, nblk);
+                const int nent = compact_texels(L, lane, sr, nx, false, nblk);
                 if (g[i] != 0.0)
                     for (int e = 0; e < nent; e++)
                         unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g[i] * L.tex.w[e]);
