@@ -588,17 +588,28 @@ __global__ __launch_bounds__(256) void coeff_src_kernel(CoeffArgs a) {
 }
 
 // Forward counts: one workgroup per (view, pixel).
-__global__ __launch_bounds__(NT) void coeff_forward_kernel(CoeffArgs a) {
+// Four waves per SIMD (two workgroups per CU) at 128 VGPRs: the spills this costs
+// are cheaper than the latency that two waves per SIMD leave exposed (fan256, 12 views:
+// 0.98 -> 0.69 s).  Two workgroups need the LDS below 80 KB, so the trace's run
+// lists share their space with the chunk's table, which is never live at the same time.
+__global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ VoxC vox[KC];
     __shared__ float2 part[NT];
     __shared__ float Ld[KC * MAXB];
     __shared__ float Fq[KC * TQ];
-    __shared__ float4 tw[KC * 64];
-    __shared__ float tb[KC * 64];
     __shared__ double red[(NT / kWave) * EPT + 64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
-    __shared__ WaveList cruns[NT / kWave];
+    __shared__ union {
+        WaveList cruns[NT / kWave];
+        struct {
+            float4 tw[KC * 64];
+            float tb[KC * 64];
+        } tab;
+    } ov;
+    WaveList *cruns = ov.cruns;
+    float4 *tw = ov.tab.tw;
+    float *tb = ov.tab.tb;
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
@@ -947,21 +958,37 @@ __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1)
 //     from the same Gaussians (16-value transposed wave sums), the ray weights
 //     gamma_b = -sum_e beta[b,e] W(e) P(e) S_e, and scatters them along the
 //     detector legs; the source legs carry the per-view sums of gamma.
-__global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
+// Four waves per SIMD at 128 VGPRs, like coeff_forward_kernel (fan256, 12 views:
+// coefficient VJP 1.11 -> 0.85 s): the trace's lists and the table/Gaussian arrays
+// share their LDS space (live in different phases of each pixel), and beta is read
+// from global memory, where its 16 KB stay in L1.
+__global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
     __shared__ VoxC vox[KC];
-    __shared__ float2 part[NT];
     __shared__ float Ld[KC * MAXB];
     __shared__ float Fq[KC * TQ];
-    __shared__ float4 tw[KC * 64];
-    __shared__ float tb[KC * 64];
-    __shared__ double pw[KC * 64];        // W(e) P(e)
-    __shared__ float sp_[NT / kWave][KC][16];  // per-wave partial S
     __shared__ double gam[KC * MAXB];
     __shared__ double gsrc[KC * MAXB];
     __shared__ double wt[64];
-    __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
-    __shared__ WaveList cruns[NT / kWave];
+    __shared__ union {
+        struct {
+            WaveList cruns[NT / kWave];
+            float2 part[NT];
+        } trace;
+        struct {
+            float4 tw[KC * 64];
+            float tb[KC * 64];
+            double pw[KC * 64];                // W(e) P(e)
+            float sp_[NT / kWave][KC][16];     // per-wave partial S
+        } gauss;
+    } ov;
+    WaveList *cruns = ov.trace.cruns;
+    float2 *part = ov.trace.part;
+    float4 *tw = ov.gauss.tw;
+    float *tb = ov.gauss.tb;
+    double *pw = ov.gauss.pw;
+    auto &sp_ = ov.gauss.sp_;
+    const float *bet = a.beta;
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int lane = t & (kWave - 1), wid = t >> 6;
@@ -1002,7 +1029,6 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
     double ht[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) ht[j] = 0.0;
-    for (int idx = t; idx < B * E; idx += NT) bet[idx] = a.beta[idx];
     for (int v = va; v < vb; v++) {
         for (int idx = t; idx < KC * MAXB; idx += NT) gsrc[idx] = 0.0;
         for (int idx = t; idx < nk * B; idx += NT) {
@@ -1146,11 +1172,11 @@ __global__ __launch_bounds__(NT) void coeff_vjp_kernel(CoeffArgs a, int views_pe
         }
         __syncthreads();
     }
-    // this view group's h: add the G energy groups of each bin in order (the table
-    // tw is free now and holds KC*TQ doubles), then store; coeff_grad_kernel adds
-    // the view groups and contracts with the basis
-    static_assert(sizeof(tw) >= sizeof(double) * KC * TQ, "LDS reuse");
-    double *hred = reinterpret_cast<double *>(tw);
+    // this view group's h: add the G energy groups of each bin in order (the shared
+    // space of the trace and the table is free now and holds KC*TQ doubles), then
+    // store; coeff_grad_kernel adds the view groups and contracts with the basis
+    static_assert(sizeof(ov) >= sizeof(double) * KC * TQ, "LDS reuse");
+    double *hred = reinterpret_cast<double *>(&ov);
     for (int hh = 0; hh < G; hh++) {
         if (h == hh) {
 #pragma unroll
