@@ -815,7 +815,7 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
 // grad_tm[texel][b].  That is one atomic instruction per texel and bundle instead of
 // one per texel and ray.  A window whose box would exceed kBoxN texels scatters each
 // ray's list directly.
-constexpr int kBundle = 8, kBoxSide = 24, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
+constexpr int kBundle = 8, kBoxSide = 20, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
 
 struct BundleBox {
     float w[kBoxN][kBundle];  // per texel of the box, one weight per ray of the bundle
