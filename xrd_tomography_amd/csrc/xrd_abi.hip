@@ -1008,7 +1008,8 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
     if (grad_maps && ca.maps_tm) {
         // detector legs scattered in bundles of adjacent pixels, views in slabs whose
         // gamma buffer (FP32, voxels x pixels x 64 per view) fits in kGamBudget bytes
-        constexpr size_t kGamBudget = (size_t)8 << 30;
+        // (16 GiB: 20 views of a 256x256 phantom with 256 pixels)
+        constexpr size_t kGamBudget = (size_t)16 << 30;
         int nv = 0;
         HIP_TRY(hipMemcpyAsync(&nv, ca.nvox, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
         HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
