@@ -218,7 +218,11 @@ int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discre
  * grad_counts [rows] (e.g. 1 - y/(counts + background) for the Poisson NLL).
  * Line integrals are recomputed, not stored.  grad_coef [nx*ny x K] (FP64,
  * deterministic) and grad_maps [B x nx*ny] (FP64, accumulated with atomics) are
- * overwritten; either may be NULL. */
+ * overwritten; either may be NULL.  With B > 4 the map gradient's detector legs are
+ * scattered in bundles of adjacent pixels from a per-view gamma workspace of at most
+ * 16 GiB (views processed in slabs), unless one view's gamma exceeds it or the
+ * environment sets XRD_VJP_SCATTER=inplace; then they are scattered inside the VJP
+ * launch (same results up to atomic order). */
 int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
                    const float *spectrum, const int32_t *segmentation, const float *coef,
                    int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,

@@ -37,9 +37,15 @@ def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu):
     assert rel <= 1e-5
 
 
-@pytest.mark.parametrize("full_mu", [True, False])
-def test_vjp_vs_oracle(xrd, oracle, full_mu):
+@pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "inplace"),
+                                             (False, "bundled")])
+def test_vjp_vs_oracle(xrd, oracle, full_mu, scatter, monkeypatch):
+    """scatter: the texel-major map gradient's detector legs in pixel bundles (the
+    default), or inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of problems
+    whose gamma buffer would not fit)."""
     import torch
+    if scatter == "inplace":
+        monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
     cp = _case(full_mu=full_mu)
     args = xrd.coeff_problem_tensors(cp)
     rng = np.random.default_rng(3)
@@ -55,7 +61,7 @@ def test_vjp_vs_oracle(xrd, oracle, full_mu):
     ec, em = _rel(gc, gco), _rel(gm, gmo)
     ec_el = float(np.max(np.abs(gc - gco) / np.maximum(np.abs(sco), 1e-300)))
     em_el = float(np.max(np.abs(gm - gmo) / np.maximum(np.abs(smo), 1e-300)))
-    print(f"\n[vjp B={cp.B}] coef rel err {ec:.2e} (elementwise vs J'|w| {ec_el:.2e}); "
+    print(f"\n[vjp B={cp.B}, {scatter}] coef rel err {ec:.2e} (elementwise vs J'|w| {ec_el:.2e}); "
           f"maps {em:.2e} (elementwise {em_el:.2e})")
     assert ec <= 1e-5 and em <= 1e-5
     assert ec_el <= 1e-5 and em_el <= 1e-5

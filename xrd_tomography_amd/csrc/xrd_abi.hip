@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -1005,7 +1006,11 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
             HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
         }
     }
-    if (grad_maps && ca.maps_tm) {
+    // XRD_VJP_SCATTER=inplace keeps the detector legs' scatter inside the VJP launch
+    // (the path of problems whose gamma buffer would not fit; also for testing)
+    const char *mode = std::getenv("XRD_VJP_SCATTER");
+    const bool inplace = mode && std::strcmp(mode, "inplace") == 0;
+    if (grad_maps && ca.maps_tm && !inplace) {
         // detector legs scattered in bundles of adjacent pixels, views in slabs whose
         // gamma buffer (FP32, voxels x pixels x 64 per view) fits in kGamBudget bytes
         // (16 GiB: 20 views of a 256x256 phantom with 256 pixels)
@@ -1014,8 +1019,9 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
         HIP_TRY(hipMemcpyAsync(&nv, ca.nvox, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
         HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
         ca.n_vox = nv;
-        if (nv > 0) {
-            const size_t per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
+        const size_t per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
+        // one view's gamma must fit the budget; larger problems scatter in place
+        if (nv > 0 && per_view * sizeof(float) <= kGamBudget) {
             const size_t views = std::max<size_t>(
                 1, std::min<size_t>((size_t)g->n_views, kGamBudget / (per_view * sizeof(float))));
             HIP_TRY(ctx->cm_gam.ensure(views * per_view * sizeof(float)), "workspace alloc");
