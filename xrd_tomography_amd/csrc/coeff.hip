@@ -587,6 +587,24 @@ __global__ __launch_bounds__(256) void coeff_src_kernel(CoeffArgs a) {
     }
 }
 
+// Source legs with texel-major maps (B > 4): one wave per (view, voxel) ray, every
+// map at once through wave_trace_tm (lane b: map b) instead of one thread walking
+// the ray once per four planar maps.
+__global__ __launch_bounds__(256) void coeff_src_tm_kernel(CoeffArgs a) {
+    __shared__ WaveList lists[4];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int k = blockIdx.x * 4 + wid, v = blockIdx.y;
+    if (k >= a.nvox[0]) return;  // uniform per wave
+    const int pix = a.vlist[k];
+    const int nx = a.f.nx, ny = a.f.ny;
+    const ViewConst vc = a.f.views[v];
+    const Ray r = make_ray(vc.src_x, vc.src_y, (double)(pix % nx) + 0.5, (double)(pix / nx) + 0.5,
+                           nx, ny, a.f.ray_step);
+    const double sum = wave_trace_tm(a.maps_tm, nx, ny, a.f.ray_step, r, lane, lists[wid]);
+    if (lane < a.n_basis)
+        a.src_ints[((size_t)v * nx * ny + k) * a.n_basis + lane] = (float)(sum * a.f.ray_step);
+}
+
 // Forward counts: one workgroup per (view, pixel).
 // Four waves per SIMD (two workgroups per CU) at 128 VGPRs: the spills this costs
 // are cheaper than the latency that two waves per SIMD leave exposed (fan256, 12 views:
@@ -1384,8 +1402,13 @@ hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s) {
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(coeff_pad_kernel, dim3(blocks), dim3(256), 0, s, a);
     if (a.maps_tm) hipLaunchKernelGGL(coeff_pad_tm_kernel, dim3(8192), dim3(256), 0, s, a);
-    dim3 grid((a.f.nx * a.f.ny + 255) / 256, a.f.n_views);
-    hipLaunchKernelGGL(coeff_src_kernel, grid, dim3(256), 0, s, a);
+    if (a.maps_tm) {
+        dim3 grid((a.f.nx * a.f.ny + 3) / 4, a.f.n_views);
+        hipLaunchKernelGGL(coeff_src_tm_kernel, grid, dim3(256), 0, s, a);
+    } else {
+        dim3 grid((a.f.nx * a.f.ny + 255) / 256, a.f.n_views);
+        hipLaunchKernelGGL(coeff_src_kernel, grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
