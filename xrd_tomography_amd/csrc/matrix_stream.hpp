@@ -225,7 +225,7 @@ __device__ __forceinline__ float row_dot(const typename Vec<T>::raw *rv,
 //   kColsOnes:     1
 // and the row is accumulated into the lane's FP64 column sums acc.  Every
 // product is added to acc in FP64 (the MLEM ratio formed in FP64), except in the
-// MLEM pass over BF16, F16 and BF24 matrices (storage error 2^-9, 2^-12, 2^-16):
+// MLEM pass over BF16, F16 and BF24 matrices (storage error 2^-8, 2^-11, 2^-16):
 // there the ratio is formed in FP32 and the step's R products of a column are
 // added in FP32 before one FP64 addition, which halves the vector instructions
 // per element.
