@@ -458,29 +458,31 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
         qch = 1e30f;  // never in band; dqc = 0 keeps the term at zero
     }
     // fine centres q_min + dq_fine*(0.5 + n) (integrate_diffraction.cu:302);
-    // q_coarse holds [Q centres][Q widths][q_min].
-    float qfh[FS], qfl[FS];
+    // q_coarse holds [Q centres][Q widths][q_min].  The lane's FS centres are evenly
+    // spaced, so the walk forms one hi/lo difference, at the middle centre (qmh, qml),
+    // and reaches centre s by subtracting dfs[s] = (s - FS/2) dq_fine: the extra
+    // rounding is 2^-24 of |d| + dq_fine.  Centres past the axis (n >= QF) keep a
+    // finite difference and weigh zero (wf).
+    constexpr int SM = FS / 2;
+    float qmh, qml, dfs[FS];
     float wlo_c = 1e30f, whi_c = -1e30f, wlo_f = 1e30f, whi_f = -1e30f;
     {
         const double qmin_axis = a.q_coarse[2 * Q];
+        const double qm = qmin_axis + a.dq_fine * (0.5 + (double)(tq * FS + SM));
+        qmh = (float)qm;
+        qml = (float)(qm - (double)qmh);
+#pragma unroll
+        for (int s = 0; s < FS; s++) dfs[s] = (float)((double)(s - SM) * a.dq_fine);
+        float lo = 1e30f, hi = -1e30f;
 #pragma unroll
         for (int s = 0; s < FS; s++) {
             const int n = tq * FS + s;
-            qfh[s] = 1e30f;
-            qfl[s] = 0.f;
             if (n < QF) {
-                const double q = qmin_axis + a.dq_fine * (0.5 + (double)n);
-                qfh[s] = (float)q;
-                qfl[s] = (float)(q - (double)qfh[s]);
+                const float qf = (float)(qmin_axis + a.dq_fine * (0.5 + (double)n));
+                lo = fminf(lo, qf);
+                hi = fmaxf(hi, qf);
             }
         }
-        float lo = 1e30f, hi = -1e30f;
-#pragma unroll
-        for (int s = 0; s < FS; s++)
-            if (tq * FS + s < QF) {
-                lo = fminf(lo, qfh[s]);
-                hi = fmaxf(hi, qfh[s]);
-            }
         float clo = (WITH_A && tq < Q) ? qch : 1e30f, chi = (WITH_A && tq < Q) ? qch : -1e30f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -633,25 +635,49 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                     uint64_t mc = __ballot(in_c);
                     uint64_t mf = __ballot(in_f);
-                    while (mc) {
-                        const int j = __ffsll((unsigned long long)mc) - 1;
-                        mc &= mc - 1;
-                        const float4 tv = tw[j * E + e];
+                    // two voxels per step while two are left, both table reads issued
+                    // before either is waited on; the sums keep ascending voxel order
+                    auto coarse = [&](const float4 &tv) {
                         const float d = (tv.x - qch) + (tv.y - qcl);
-                        const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
-                        pc[i] = fmaf(tv.w * dqc, g, pc[i]);
-                    }
-                    while (mf) {
-                        const int j = __ffsll((unsigned long long)mf) - 1;
-                        mf &= mf - 1;
-                        const float4 tv = tw[j * E + e];
+                        return __builtin_amdgcn_exp2f(d * d * tv.z);
+                    };
+                    auto fine = [&](const float4 &tv) {
+                        const float dm = (tv.x - qmh) + (tv.y - qml);
                         float acc = 0.f;
 #pragma unroll
                         for (int s = 0; s < FS; s++) {
-                            const float d = (tv.x - qfh[s]) + (tv.y - qfl[s]);
+                            const float d = s == SM ? dm : dm - dfs[s];
                             acc = fmaf(wf[s], __builtin_amdgcn_exp2f(d * d * tv.z), acc);
                         }
-                        pf[i] = fmaf(tv.w, acc, pf[i]);
+                        return acc;
+                    };
+                    while (mc & (mc - 1)) {
+                        const int j0 = __ffsll((unsigned long long)mc) - 1;
+                        mc &= mc - 1;
+                        const int j1 = __ffsll((unsigned long long)mc) - 1;
+                        mc &= mc - 1;
+                        const float4 t0 = tw[j0 * E + e], t1 = tw[j1 * E + e];
+                        const float g0 = coarse(t0), g1 = coarse(t1);
+                        pc[i] = fmaf(t0.w, g0, pc[i]);  // times dqc at the flush
+                        pc[i] = fmaf(t1.w, g1, pc[i]);
+                    }
+                    if (mc) {
+                        const float4 tv = tw[(__ffsll((unsigned long long)mc) - 1) * E + e];
+                        pc[i] = fmaf(tv.w, coarse(tv), pc[i]);
+                    }
+                    while (mf & (mf - 1)) {
+                        const int j0 = __ffsll((unsigned long long)mf) - 1;
+                        mf &= mf - 1;
+                        const int j1 = __ffsll((unsigned long long)mf) - 1;
+                        mf &= mf - 1;
+                        const float4 t0 = tw[j0 * E + e], t1 = tw[j1 * E + e];
+                        const float a0 = fine(t0), a1 = fine(t1);
+                        pf[i] = fmaf(t0.w, a0, pf[i]);
+                        pf[i] = fmaf(t1.w, a1, pf[i]);
+                    }
+                    if (mf) {
+                        const float4 tv = tw[(__ffsll((unsigned long long)mf) - 1) * E + e];
+                        pf[i] = fmaf(tv.w, fine(tv), pf[i]);
                     }
                 }
             }
@@ -666,7 +692,7 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
             for (int i = 0; i < EPT; i++) {
                 const int e = h + G * i;
-                if (WITH_A && e < E && tq < Q) hst[e * Q + tq] += pc[i];
+                if (WITH_A && e < E && tq < Q) hst[e * Q + tq] += pc[i] * dqc;
                 pc[i] = 0.f;
                 pf[i] = 0.f;
             }
