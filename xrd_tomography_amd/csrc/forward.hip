@@ -29,6 +29,7 @@ namespace xrd {
 
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kRs2pi = 0.39894228040143267794;  // 1/sqrt(2 pi)
+constexpr double kBandScale = 9.419280180123797;    // sqrt(128 / log2(e))
 
 // ---------------------------------------------------------------------------
 // preparation
@@ -575,26 +576,31 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const double w = ec.wcoef * vs.s;
                 const float wh = (float)w;
                 const float wl = (float)(w - (double)wh);
-                const double c2 = -kLog2e / (2.0 * sig2);
-                const double P = pinpout * vs.gR * kRs2pi / sqrt(sig2);
+                // from one 1/sigma: c2 = -log2(e)/(2 sigma^2), the prefactor's
+                // 1/(sqrt(2 pi) sigma), and the band half-width sqrt(64/-c2)
+                const double isg = 1.0 / sqrt(sig2);
+                const double c2 = (-0.5 * kLog2e) * (isg * isg);
+                const double P = pinpout * vs.gR * kRs2pi * isg;
                 tw[j * E + e] = make_float4(wh, wl, (float)c2, (float)P);
-                tb[j * E + e] = (float)sqrt(64.0 / -c2);
-                // Compton pieces (integrate_diffraction.cu:318-356) without RF and channel_match
+                tb[j * E + e] = (float)(kBandScale * sig2 * isg);
+                // Compton pieces (integrate_diffraction.cu:318-356) without RF and
+                // channel_match; one reciprocal of er serves every division by it
                 const double ein = ec.ein;
                 const double er = 1.0 + ein * (1.0 - vs.cth);
-                const double eout = ein / er;
+                const double ier = 1.0 / er;
+                const double eout = ein * ier;
                 const double wc = ein * vs.s;
-                const double l2w = log2(wc * sqrt(er + wc * wc) / er);
+                const double l2w = log2(wc * sqrt(er + wc * wc) * ier);
                 const int nq = lut_index((l2w - a.lut_q_min) / a.dq_lut, a.lut_nq);
                 const double g1 = eout * eout * eout, g2 = f2_kn(a, eout);
                 const double pin = exp_neg(vs.lin0 * ec.inv_f1 + vs.lin1 * ec.inv_f2);
                 const double pc_ = exp_neg(vs.lout0 / g1 + vs.lout1 / g2);
-                const double cf = er * er * (er + 1.0 / er - 1.0 + vs.cth * vs.cth) / 2.0;
+                const double cf = er * er * (er + ier - 1.0 + vs.cth * vs.cth) / 2.0;
                 const double K = pin * pc_ * vs.gp * cf * vs.a2c *
                                  (double)a.s2u8[nq + a.lut_nq * vs.na];
                 // Compton-shifted source bin [es1, es2] / er (channel_match.cu): the
                 // detector lanes add (overlap with their bin) * K / (bin width)
-                const double eo1 = ec.es1 / er, eo2 = ec.es2 / er;
+                const double eo1 = ec.es1 * ier, eo2 = ec.es2 * ier;
                 teo[j * E + e] = make_double2(eo1, eo2);
                 tk[j * E + e] = (float)(K / (eo2 - eo1));
             }
