@@ -618,11 +618,14 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ double red[(NT / kWave) * EPT + 64];
     __shared__ float bet[MAXB * 64];
     __shared__ float Lsrc[KC * MAXB];
+    // table rows of kTS entries: the ballot's per-lane reads of one energy across
+    // the chunk's voxels then fall into different banks
+    constexpr int kTS = 65;
     __shared__ union {
         WaveList cruns[NT / kWave];
         struct {
-            float4 tw[KC * 64];
-            float tb[KC * 64];
+            float4 tw[KC * kTS];
+            float tb[KC * kTS];
         } tab;
     } ov;
     WaveList *cruns = ov.cruns;
@@ -696,19 +699,30 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
             float band;
             double P;
             table_entry(a, vox[j], Lsrc + j * MAXB, Ld + j * MAXB, bet, e, tv, band, P);
-            tw[j * 64 + e] = tv;
-            tb[j * 64 + e] = band;
+            tw[j * kTS + e] = tv;
+            tb[j * kTS + e] = band;
         }
         __syncthreads();
-        for (int j = 0; j < nk; j++) {
-            const float wq = dqc * Fq[j * TQ + tq];
+        // per source energy, one ballot gives the chunk's voxels whose band reaches
+        // the wave's bins (the range and the energy are wave-uniform); the wave walks
+        // those only, in ascending voxel order
+        {
+            const int ln = t & (kWave - 1);
 #pragma unroll
             for (int i = 0; i < EPT; i++) {
                 const int e = h + G * i;
                 if (e >= E) continue;
-                const float4 tv = tw[j * 64 + e];
-                const float band = tb[j * 64 + e];
-                if (tv.x + band >= lo && tv.x - band <= hi) {
+                bool in = false;
+                if (ln < nk) {
+                    const float x = tw[ln * kTS + e].x, band = tb[ln * kTS + e];
+                    in = x + band >= lo && x - band <= hi;
+                }
+                uint64_t m = __ballot(in);
+                while (m) {
+                    const int j = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const float4 tv = tw[j * kTS + e];
+                    const float wq = dqc * Fq[j * TQ + tq];
                     const float d = (tv.x - qch) + (tv.y - qcl);
                     pf[i] = fmaf(tv.w * wq, __builtin_amdgcn_exp2f(d * d * tv.z), pf[i]);
                 }
