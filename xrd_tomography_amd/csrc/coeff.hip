@@ -718,13 +718,25 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
                     in = x + band >= lo && x - band <= hi;
                 }
                 uint64_t m = __ballot(in);
-                while (m) {
-                    const int j = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
+                auto term = [&](int j) {
                     const float4 tv = tw[j * kTS + e];
                     const float wq = dqc * Fq[j * TQ + tq];
                     const float d = (tv.x - qch) + (tv.y - qcl);
-                    pf[i] = fmaf(tv.w * wq, __builtin_amdgcn_exp2f(d * d * tv.z), pf[i]);
+                    return make_float2(tv.w * wq, __builtin_amdgcn_exp2f(d * d * tv.z));
+                };
+                // two voxels per step while two are left, their reads in flight together
+                while (m & (m - 1)) {
+                    const int j0 = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const int j1 = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const float2 u0 = term(j0), u1 = term(j1);
+                    pf[i] = fmaf(u0.x, u0.y, pf[i]);
+                    pf[i] = fmaf(u1.x, u1.y, pf[i]);
+                }
+                if (m) {
+                    const float2 u = term(__ffsll((unsigned long long)m) - 1);
+                    pf[i] = fmaf(u.x, u.y, pf[i]);
                 }
             }
         }
