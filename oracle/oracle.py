@@ -6,6 +6,7 @@ product path (xrd_tomography_amd) never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -35,47 +36,64 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.xo_get_intersection.restype = ctypes.c_int
-        L.xo_get_intersection.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [_dp, _dp]
-        L.xo_trace_ray.restype = ctypes.c_int
-        L.xo_trace_ray.argtypes = ([ctypes.c_double] * 4 + [ctypes.c_int] * 2 +
-                                   [ctypes.c_double, _dp, _dp, _ip, ctypes.c_int32, _ip])
-        L.xo_line_integrals.restype = None
-        L.xo_line_integrals.argtypes = ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_double] + [ctypes.c_double] * 4 + [_dp])
-        L.xo_bilinear.restype = ctypes.c_double
-        L.xo_bilinear.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                  ctypes.c_double]
-        L.xo_find_src_nrg_range.restype = None
-        L.xo_find_src_nrg_range.argtypes = [ctypes.c_int] * 3 + [_ip, _ip]
-        L.xo_wvt_value.restype = None
-        L.xo_wvt_value.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp]
-        L.xo_scatter_geometry.restype = None
-        L.xo_scatter_geometry.argtypes = [ctypes.POINTER(Geometry)] + [ctypes.c_int] * 5 + [_dp]
-        L.xo_forward.restype = ctypes.c_int
-        L.xo_forward.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
-                                 _dp, _dp, _ip, _dp, _dp, _dp] + [ctypes.c_double] * 4 + \
-                                [_ip, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
-        for name in ("xo_project", "xo_backproject"):
-            f = getattr(L, name)
-            f.restype = None
-            f.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, ctypes.c_int]
-        L.xo_sensitivity.restype = None
-        L.xo_sensitivity.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int]
-        L.xo_mlem.restype = None
-        L.xo_mlem.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
-                              ctypes.c_int]
-        L.xo_mlem_update.restype = None
-        L.xo_mlem_update.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, _dp,
-                                     ctypes.c_int]
-        L.xo_loglik.restype = ctypes.c_double
-        L.xo_loglik.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
-        L.xo_max_threads.restype = ctypes.c_int
-        L.xo_channel_match.restype = ctypes.c_double
-        L.xo_channel_match.argtypes = [ctypes.c_double] * 5
-        _lib = L
+        _lib = _configure(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def using_library(path: str):
+    """Route every call of this module to another build of the oracle (for example
+    one compiled with a planted mistake, to show that a pin detects it)."""
+    global _lib
+    saved = _lib
+    _lib = _configure(ctypes.CDLL(os.path.abspath(path)))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _configure(L):
+    """Declare the argument and result types of the oracle's exports."""
+    L.xo_get_intersection.restype = ctypes.c_int
+    L.xo_get_intersection.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [_dp, _dp]
+    L.xo_trace_ray.restype = ctypes.c_int
+    L.xo_trace_ray.argtypes = ([ctypes.c_double] * 4 + [ctypes.c_int] * 2 +
+                               [ctypes.c_double, _dp, _dp, _ip, ctypes.c_int32, _ip])
+    L.xo_line_integrals.restype = None
+    L.xo_line_integrals.argtypes = ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double] + [ctypes.c_double] * 4 + [_dp])
+    L.xo_bilinear.restype = ctypes.c_double
+    L.xo_bilinear.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_double]
+    L.xo_find_src_nrg_range.restype = None
+    L.xo_find_src_nrg_range.argtypes = [ctypes.c_int] * 3 + [_ip, _ip]
+    L.xo_wvt_value.restype = None
+    L.xo_wvt_value.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp]
+    L.xo_scatter_geometry.restype = None
+    L.xo_scatter_geometry.argtypes = [ctypes.POINTER(Geometry)] + [ctypes.c_int] * 5 + [_dp]
+    L.xo_forward.restype = ctypes.c_int
+    L.xo_forward.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
+                             _dp, _dp, _ip, _dp, _dp, _dp] + [ctypes.c_double] * 4 + \
+                            [_ip, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+    for name in ("xo_project", "xo_backproject"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, ctypes.c_int]
+    L.xo_sensitivity.restype = None
+    L.xo_sensitivity.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int]
+    L.xo_mlem.restype = None
+    L.xo_mlem.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+                          ctypes.c_int]
+    L.xo_mlem_update.restype = None
+    L.xo_mlem_update.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                 ctypes.c_int]
+    L.xo_loglik.restype = ctypes.c_double
+    L.xo_loglik.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+    L.xo_max_threads.restype = ctypes.c_int
+    L.xo_channel_match.restype = ctypes.c_double
+    L.xo_channel_match.argtypes = [ctypes.c_double] * 5
+    return L
 
 
 def _d(a):

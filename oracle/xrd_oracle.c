@@ -299,6 +299,20 @@ void xo_scatter_geometry(const xrd_geometry *g, int view, int col, int row, int 
     o->det_z = dz;
 }
 
+/* Centre and variance of the q-resolution Gaussian of source energy Es
+ * (integrate_diffraction.cu:284-288): wvt = 2 Es sin(theta/2)/hbarc and
+ * sigma^2 = (2 sin(theta/2)/hbarc)^2 dE^2/12
+ *         + (Es/hbarc)^2 (sS2 + vV2 + dD2) / (2 sin(theta/2))^2 / |a|^2 / |b|^2. */
+static void xo_q_resolution(const xrd_discretization *d, const xo_scatter_geom *sg, double Es,
+                            double *wvt, double *sig2) {
+    const double s = sg->sin_half;
+    const double k1 = 2.0 * s / d->hbarc;
+    const double k2 = Es / d->hbarc;
+    *wvt = 2.0 * Es / d->hbarc * s;
+    *sig2 = k1 * k1 * d->e_width * d->e_width / 12.0 +
+            k2 * k2 * sg->res2 / (2.0 * s) / (2.0 * s) / sg->anorm2 / sg->bnorm2;
+}
+
 /* floor(t) clamped to [0, n-1]; NaN -> 0 (the clamps of integrate_diffraction.cu:187-189,
  * :337-339; cvt of NaN to int gives 0). */
 static int xo_lut_index(double t, int n) {
@@ -517,12 +531,8 @@ static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
                 for (int e = 0; e < E; e++) {
                     const double Es = Esm[e];
                     const double ein = Es / d->mc2;
-                    const double wvt = 2.0 * Es / d->hbarc * s;
-                    const double k1 = 2.0 * s / d->hbarc;
-                    const double k2 = Es / d->hbarc;
-                    const double sig2 = k1 * k1 * d->e_width * d->e_width / 12.0 +
-                                        k2 * k2 * sg.res2 / (2.0 * s) / (2.0 * s) / sg.anorm2 /
-                                            sg.bnorm2;
+                    double wvt, sig2;
+                    xo_q_resolution(d, &sg, Es, &wvt, &sig2);
                     const double f1 = ein * ein * ein, f2 = xo_f2(d, ein);
                     const double Pin = exp(-(Lin[0] / f1 + Lin[1] / f2));
                     const double Pout = exp(-(Lout[0] / f1 + Lout[1] / f2));
@@ -789,11 +799,7 @@ static void xo_cm_triple(const xrd_geometry *g, const xrd_discretization *d,
         double mu = 0.0;
         for (int b = 0; b < B; b++) mu += beta[(long)b * E + e] * (Ls[b] + Ld[b]);
         const double Es = s->Esm[e];
-        const double k1 = 2.0 * sg->sin_half / d->hbarc, k2 = Es / d->hbarc;
-        sig2[e] = k1 * k1 * d->e_width * d->e_width / 12.0 +
-                  k2 * k2 * sg->res2 / (2.0 * sg->sin_half) / (2.0 * sg->sin_half) / sg->anorm2 /
-                      sg->bnorm2;
-        wvt[e] = 2.0 * Es / d->hbarc * sg->sin_half;
+        xo_q_resolution(d, sg, Es, &wvt[e], &sig2[e]);
         P[e] = exp(-mu) * sg->geom * sg->rayleigh * rs2pi / sqrt(sig2[e]);
     }
 }
