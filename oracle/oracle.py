@@ -75,7 +75,7 @@ def _configure(L):
     L.xo_forward.restype = ctypes.c_int
     L.xo_forward.argtypes = [ctypes.POINTER(Geometry), ctypes.POINTER(Discretization),
                              _dp, _dp, _ip, _dp, _dp, _dp] + [ctypes.c_double] * 4 + \
-                            [_ip, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+                            [_ip, ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
     for name in ("xo_project", "xo_backproject"):
         f = getattr(L, name)
         f.restype = None
@@ -157,14 +157,25 @@ def scatter_geometry(g: Geometry, view, col, row, ix, iy):
     return dict(zip(GEOM_FIELDS, out))
 
 
-def forward(prob, views=None, with_A=True, nthreads=0, disc=None):
-    """Oracle forward model of a synthetic.Problem for the listed views.
-    Returns (N, C, A) float64 with SRC replaced by len(views); A is [rows, cols]."""
+def _col_list(g, cols):
+    if cols is None:
+        return None, 0, g.n_cols
+    cl = np.ascontiguousarray(cols, np.int32)
+    if cl.size == 0 or cl.min() < 0 or cl.max() >= g.n_cols:
+        raise ValueError("column list out of range")
+    return cl, cl.size, cl.size
+
+
+def forward(prob, views=None, with_A=True, nthreads=0, disc=None, cols=None):
+    """Oracle forward model of a synthetic.Problem for the listed views and detector
+    columns.  Returns (N, C, A) float64 with SRC replaced by len(views) and COL by
+    len(cols); A is [rows, cols]."""
     g = prob.geometry
     d = disc if disc is not None else prob.disc
     vl = np.arange(g.n_views, dtype=np.int32) if views is None else np.asarray(views, np.int32)
     nvl = len(vl)
-    rows = nvl * g.n_cols * g.n_rows * d.n_energy
+    cl, ncl, ncol = _col_list(g, cols)
+    rows = nvl * ncol * g.n_rows * d.n_energy
     N = np.zeros(rows)
     C = np.zeros(rows)
     A = np.zeros(rows * d.n_q * d.n_mat) if with_A else None
@@ -176,7 +187,9 @@ def forward(prob, views=None, with_A=True, nthreads=0, disc=None):
     s2, s2p = _d(prob.flat("s2_table"))
     vv, vvp = _i(vl)
     rc = lib().xo_forward(ctypes.byref(g), ctypes.byref(d), spp, fip, sgp, a1p, a2p, s2p,
-                          *[float(v) for v in prob.lut], vvp, nvl, N.ctypes.data_as(_dp),
+                          *[float(v) for v in prob.lut], vvp, nvl,
+                          cl.ctypes.data_as(_ip) if cl is not None else None, ncl,
+                          N.ctypes.data_as(_dp),
                           C.ctypes.data_as(_dp),
                           A.ctypes.data_as(_dp) if with_A else None, nthreads)
     if rc != 0:
@@ -362,13 +375,14 @@ def line_integrals_3d(vols, nx, ny, nz, step, x1, y1, z1, x2, y2, z2):
     return out
 
 
-def forward_slice(vp, n_slices, slice_, views=None, with_A=True, nthreads=0):
+def forward_slice(vp, n_slices, slice_, views=None, with_A=True, nthreads=0, cols=None):
     """Oracle forward model of slice slice_ of a volume problem (segmentation, a1, a2
     [nx, ny, n_slices]).  Returns (N, C, A) like forward()."""
     g, d = vp.geometry, vp.disc
     vl = np.arange(g.n_views, dtype=np.int32) if views is None else np.asarray(views, np.int32)
     nvl = len(vl)
-    rows = nvl * g.n_cols * g.n_rows * d.n_energy
+    cl, ncl, ncol = _col_list(g, cols)
+    rows = nvl * ncol * g.n_rows * d.n_energy
     N = np.zeros(rows)
     C = np.zeros(rows)
     A = np.zeros(rows * d.n_q * d.n_mat) if with_A else None
@@ -384,6 +398,7 @@ def forward_slice(vp, n_slices, slice_, views=None, with_A=True, nthreads=0):
     rc = L.xo_forward_slice(ctypes.byref(g), ctypes.byref(d), spp, fip, sgp, a1p, a2p,
                             ctypes.c_int(n_slices), ctypes.c_int(slice_), s2p,
                             *[ctypes.c_double(float(v)) for v in vp.lut], vvp, ctypes.c_int(nvl),
+                            cl.ctypes.data_as(_ip) if cl is not None else None, ctypes.c_int(ncl),
                             N.ctypes.data_as(_dp), C.ctypes.data_as(_dp),
                             A.ctypes.data_as(_dp) if with_A else None, ctypes.c_int(nthreads))
     if rc != 0:

@@ -362,39 +362,41 @@ static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
                            const double *spectrum, const double *ifine, const int32_t *seg,
                            const double *a1, const double *a2, const double *s2_table,
                            double lut_q_min, double lut_q_max, double lut_a_min,
-                           double lut_a_max, const int32_t *views, int nvl, double *out_N,
-                           double *out_C, double *out_A, int nthreads, const xo_volume *volm);
+                           double lut_a_max, const int32_t *views, int nvl,
+                           const int32_t *cols, int ncl, double *out_N, double *out_C,
+                           double *out_A, int nthreads, const xo_volume *volm);
 
 int xo_forward(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
                const double *ifine, const int32_t *seg, const double *a1, const double *a2,
                const double *s2_table, double lut_q_min, double lut_q_max, double lut_a_min,
-               double lut_a_max, const int32_t *views, int nvl, double *out_N, double *out_C,
-               double *out_A, int nthreads) {
+               double lut_a_max, const int32_t *views, int nvl, const int32_t *cols, int ncl,
+               double *out_N, double *out_C, double *out_A, int nthreads) {
     return xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min, lut_q_max,
-                           lut_a_min, lut_a_max, views, nvl, out_N, out_C, out_A, nthreads, NULL);
+                           lut_a_min, lut_a_max, views, nvl, cols, ncl, out_N, out_C, out_A,
+                           nthreads, NULL);
 }
 
 int xo_forward_slice(const xrd_geometry *g, const xrd_discretization *d, const double *spectrum,
                      const double *ifine, const int32_t *seg_vol, const double *a1_vol,
                      const double *a2_vol, int n_slices, int slice, const double *s2_table,
                      double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
-                     const int32_t *views, int nvl, double *out_N, double *out_C,
-                     double *out_A, int nthreads) {
+                     const int32_t *views, int nvl, const int32_t *cols, int ncl,
+                     double *out_N, double *out_C, double *out_A, int nthreads) {
     if (n_slices <= 0 || slice < 0 || slice >= n_slices) return XRD_ERR_DIMS;
     const long plane = (long)g->nx * g->ny;
     const int32_t *seg = seg_vol + plane * slice;
     const double *a1 = a1_vol + plane * slice, *a2 = a2_vol + plane * slice;
     if (n_slices == 1)
         return xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min,
-                               lut_q_max, lut_a_min, lut_a_max, views, nvl, out_N, out_C, out_A,
-                               nthreads, NULL);
+                               lut_q_max, lut_a_min, lut_a_max, views, nvl, cols, ncl, out_N,
+                               out_C, out_A, nthreads, NULL);
     double *vol = (double *)malloc(sizeof(double) * 2 * plane * n_slices);
     memcpy(vol, a1_vol, sizeof(double) * plane * n_slices);
     memcpy(vol + plane * n_slices, a2_vol, sizeof(double) * plane * n_slices);
     xo_volume vm = {vol, n_slices, (double)slice + 0.5};
     const int rc = xo_forward_impl(g, d, spectrum, ifine, seg, a1, a2, s2_table, lut_q_min,
-                                   lut_q_max, lut_a_min, lut_a_max, views, nvl, out_N, out_C,
-                                   out_A, nthreads, &vm);
+                                   lut_q_max, lut_a_min, lut_a_max, views, nvl, cols, ncl,
+                                   out_N, out_C, out_A, nthreads, &vm);
     free(vol);
     return rc;
 }
@@ -403,13 +405,16 @@ static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
                            const double *spectrum, const double *ifine, const int32_t *seg,
                            const double *a1, const double *a2, const double *s2_table,
                            double lut_q_min, double lut_q_max, double lut_a_min,
-                           double lut_a_max, const int32_t *views, int nvl, double *out_N,
-                           double *out_C, double *out_A, int nthreads, const xo_volume *volm) {
+                           double lut_a_max, const int32_t *views, int nvl,
+                           const int32_t *cols, int ncl, double *out_N, double *out_C,
+                           double *out_A, int nthreads, const xo_volume *volm) {
     const int NX = g->nx, NY = g->ny, V = g->n_views, COL = g->n_cols, ROW = g->n_rows;
-    const int DET = COL * ROW;
+    /* detector pixels computed: the listed columns (all when cols is NULL) of every row */
+    const int NCL = cols ? ncl : COL;
+    const int DET = NCL * ROW;
     const int E = d->n_energy, DN = d->energy_halfwidth, Q = d->n_q, QF = d->n_qf;
     const int M = d->n_mat, LQ = d->lut_nq, LA = d->lut_na;
-    if (NX <= 0 || NY <= 0 || V <= 0 || COL <= 0 || ROW <= 0 || E <= 2 * DN || Q <= 0 ||
+    if (NX <= 0 || NY <= 0 || V <= 0 || COL <= 0 || ROW <= 0 || NCL <= 0 || E <= 2 * DN || Q <= 0 ||
         QF <= 0 || M <= 0 || LQ <= 0 || LA <= 0)
         return XRD_ERR_DIMS;
     int32_t *vlist = (int32_t *)malloc(sizeof(int32_t) * (views ? nvl : V));
@@ -501,7 +506,8 @@ static int xo_forward_impl(const xrd_geometry *g, const xrd_discretization *d,
 #pragma omp for schedule(dynamic, 1)
         for (long vp = 0; vp < (long)nvl * DET; vp++) {
             const int vi = (int)(vp / DET), det = (int)(vp % DET);
-            const int col = det % COL, row = det / COL;
+            const int ci = det % NCL, row = det / NCL;
+            const int col = cols ? cols[ci] : ci;
             const int view = vlist[vi];
             if (accA) memset(accA, 0, sizeof(double) * (size_t)E * M * Q);
             memset(accN, 0, sizeof(double) * E * 2);

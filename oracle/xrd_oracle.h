@@ -92,13 +92,14 @@ int xo_forward_slice(const xrd_geometry *g, const xrd_discretization *d, const d
                      const double *ifine, const int32_t *seg_vol, const double *a1_vol,
                      const double *a2_vol, int n_slices, int slice, const double *s2_table,
                      double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
-                     const int32_t *views, int nvl, double *out_N, double *out_C,
-                     double *out_A, int nthreads);
+                     const int32_t *views, int nvl, const int32_t *cols, int ncl,
+                     double *out_N, double *out_C, double *out_A, int nthreads);
 
 /* Full forward model (integrate_diffraction.cu:9-409 with the launch of
  * Forward_XRD.cu:244-259) for the views listed in `views` (n_views_list of them;
- * NULL = all views).  Outputs use the reference layouts with SRC replaced by
- * n_views_list: row r = vi + nvl*(col + COL*(row + ROW*e_det)).
+ * NULL = all views) and the detector columns listed in `cols` (n_cols_list of them;
+ * NULL = all columns).  Outputs use the reference layouts with SRC replaced by
+ * n_views_list and COL by n_cols_list: row r = vi + nvl*(ci + ncl*(row + ROW*e_det)).
  *   out_N, out_C: [rows]; out_A: [NWT*MAT*rows] or NULL.
  * Inputs are the xrd_forward inputs converted to double.  nthreads <= 0 uses
  * OpenMP's default.  Returns 0 or a negative XRD_ERR_* code. */
@@ -106,7 +107,8 @@ int xo_forward(const xrd_geometry *g, const xrd_discretization *d, const double 
                const double *ifine, const int32_t *segmentation, const double *a1,
                const double *a2, const double *s2_table, double lut_q_min, double lut_q_max,
                double lut_a_min, double lut_a_max, const int32_t *views, int n_views_list,
-               double *out_N, double *out_C, double *out_A, int nthreads);
+               const int32_t *cols, int n_cols_list, double *out_N, double *out_C,
+               double *out_A, int nthreads);
 
 /* Explicit-matrix operators (main.m:162), A row-major [n_rows x n_cols]. */
 void xo_project(const double *A, int64_t n_rows, int n_cols, const double *x, double *out,
