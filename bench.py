@@ -25,14 +25,40 @@ METRIC = "MLEM fwd+adj iters/s @256²×360 views; % HBM roofline; rel err vs FP6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the streaming pass, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (profiles/mlem_fused_fan256_hbm_traffic.md): FETCH_SIZE scaled by 2.0 and
-# WRITE_SIZE by 1.0, the factors that map each counter onto a 2 GiB copy kernel.
-MEASURED_TRAFFIC = {("fan256", "f32", "fused"): 23_076_164_551,
-                    ("fan256", "bf24", "fused"): 17_321_587_470}
+# WRITE_SIZE by 1.0, the factors that map each counter onto a 2 GiB copy kernel.  Each
+# figure carries the hash of the streaming kernels' sources it was measured on
+# (stream_sources_sha16); once those sources change it no longer applies and the bench
+# line reports traffic null until it is measured again.
+MEASURED_TRAFFIC = {
+    ("fan256", "f32", "fused"): (23_076_164_594, "5caf26e605accaac",
+                                 "profiles/mlem_fused_fan256_hbm_traffic.md, launch 2,4"),
+    ("fan256", "bf24", "fused"): (17_321_586_916, "5caf26e605accaac",
+                                  "profiles/mlem_fused_fan256_hbm_traffic.md, launch 3,1"),
+}
+STREAM_SOURCES = ("xrd_tomography_amd/csrc/matrix_stream.hpp", "xrd_tomography_amd/csrc/mlem.hip")
 ESIZE = {"f32": 4, "bf16": 2, "f16": 2, "bf24": 3}  # bytes per stored element of A
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def stream_sources_sha16() -> str:
+    """First 16 hex digits of the SHA-256 of the streaming kernels' sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in STREAM_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(key):
+    """(bytes, profile note) of a stamped traffic measurement that still applies."""
+    entry = MEASURED_TRAFFIC.get(key)
+    if entry is None or entry[1] != stream_sources_sha16():
+        return None, None
+    return entry[0], entry[2]
 
 
 def main():
@@ -130,6 +156,15 @@ def main():
     torch.cuda.synchronize()
     a_build_s = time.perf_counter() - t0
     log(f"[bench] xrd_forward (N, C, A) built in {a_build_s:.2f} s")
+    # rows of view 0 at a few evenly spaced detector columns, kept for the comparison
+    # with the FP64 oracle forward that is also the CPU A-build baseline
+    a_cols = np.linspace(0, g.n_cols - 1, max(1, min(args.cpu_a_cols, g.n_cols))).round()
+    a_cols = np.unique(a_cols.astype(np.int64))
+    sel = lambda X, w=None: (X.reshape((d.n_energy, g.n_rows, g.n_cols, g.n_views) +  # noqa: E731
+                                       ((w,) if w else ()))
+                             .index_select(2, torch.from_numpy(a_cols).to(dev))[:, :, :, 0]
+                             .double().cpu().numpy().reshape((-1, w) if w else (-1,)))
+    a_rows = (sel(N), sel(C), sel(A, ncol))
     A = A.view(nrow, ncol)
     N = N.reshape(-1)
     C = C.reshape(-1)
@@ -190,13 +225,14 @@ def main():
     bytes_step = nrow * ncol * esize + nrow * 8 + ncol * 12
     passes = 2 if args.unfused else 1
     achieved = bytes_step * passes / (ms_pass * 1e-3) / 1e9
-    traffic = MEASURED_TRAFFIC.get((args.workload, args.dtype,
-                                    "unfused" if args.unfused else "fused"))
+    traffic, traffic_src = measured_traffic((args.workload, args.dtype,
+                                             "unfused" if args.unfused else "fused"))
 
     # ---- error vs FP64 and CPU baseline on a view subsample (whole views, all rows of them)
     rel_err = None
     cpu = None
     cpu_a = None
+    a_build_err = None
     if not args.no_cpu:
         from oracle import oracle as O
         nv = max(1, min(args.cpu_views, g.n_views))
@@ -248,19 +284,20 @@ def main():
         log(f"[bench] cpu baseline {cpu_iters_s:.3f} iters/s on {threads} threads; "
             f"GPU-vs-FP64 MLEM rel err after {k_err} steps {rel_err:.2e}")
         del A64
-        # A-build baseline: the FP64 oracle forward (N, C and A) on one view and a few
-        # detector columns of the same phantom, scaled to all views and columns (the
-        # cost per (view, column) row is the same)
-        pa = make_problem(args.workload, n_views=1, n_cols=args.cpu_a_cols)
+        # A-build baseline: the FP64 oracle forward (N, C and A) on view 0 and a few
+        # detector columns of the same problem, scaled to all views and columns (the
+        # cost per (view, column) row is the same); its rows are compared with the GPU's
         t0 = time.perf_counter()
-        O.forward(pa, with_A=True)
+        No, Co, Ao = O.forward(prob, views=[0], cols=a_cols, with_A=True)
         dt_a = time.perf_counter() - t0
-        k_a = g.n_views * g.n_cols / args.cpu_a_cols
+        k_a = g.n_views * g.n_cols / len(a_cols)
         cpu_a = {"value": round(dt_a * k_a, 1), "unit": "s per A build (N, C and A)",
                  "cores": threads, "kind": "port",
-                 "sample": f"FP64 oracle forward on 1 view x {args.cpu_a_cols} detector "
-                           f"columns of the same phantom ({dt_a:.2f} s), scaled x{k_a:.0f}"}
+                 "sample": f"FP64 oracle forward on view 0 x {len(a_cols)} detector "
+                           f"columns of the same problem ({dt_a:.2f} s), scaled x{k_a:.0f}"}
         log(f"[bench] cpu A build {dt_a * k_a:.0f} s (scaled) against {a_build_s:.2f} s")
+        a_build_err = _abuild_errors(a_rows, (No, Co, Ao))
+        log(f"[bench] A build vs FP64 on those rows: {a_build_err}")
 
     out = {
         "metric": METRIC,
@@ -288,16 +325,33 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel_ms": round(ms_pass, 4),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_ms": round(ms_pass, 4),
                      "algorithmic_bytes": bytes_step * passes},
         "cpu_baseline": cpu,
         "rel_err_vs_fp64": rel_err,
         "a_build_s": round(a_build_s, 3),
         "cpu_a_build": cpu_a,
+        "a_build_rel_err_vs_fp64": a_build_err,
         "bytes_per_step": bytes_step * passes,
     }
     plan.close()
     print(json.dumps(out), flush=True)
+
+
+def _abuild_errors(gpu, ora):
+    """Worst relative error of the GPU's N, C (per row, rows above 1e-9 of the largest)
+    and A (per-row Frobenius) against the FP64 oracle's rows."""
+    import numpy as np
+    out = {}
+    for name, g_, o_ in zip(("N", "C"), gpu[:2], ora[:2]):
+        live = np.abs(o_) > 1e-9 * np.abs(o_).max()
+        out[name] = float(np.max(np.abs(g_[live] - o_[live]) / np.abs(o_[live]), initial=0.0))
+    rn = np.linalg.norm(ora[2], axis=1)
+    live = rn > 1e-9 * rn.max()
+    out["A_row"] = float(np.max(np.linalg.norm(gpu[2] - ora[2], axis=1)[live] / rn[live],
+                                initial=0.0))
+    return {k: float(f"{v:.3e}") for k, v in out.items()}
 
 
 def _scaled_counts(api, prob, dev):

@@ -219,10 +219,12 @@ int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discre
  * Line integrals are recomputed, not stored.  grad_coef [nx*ny x K] (FP64,
  * deterministic) and grad_maps [B x nx*ny] (FP64, accumulated with atomics) are
  * overwritten; either may be NULL.  With B > 4 the map gradient's detector legs are
- * scattered in bundles of adjacent pixels from a per-view gamma workspace of at most
- * 16 GiB (views processed in slabs), unless one view's gamma exceeds it or the
- * environment sets XRD_VJP_SCATTER=inplace; then they are scattered inside the VJP
- * launch (same results up to atomic order). */
+ * scattered in bundles of adjacent pixels from a per-view gamma workspace (views
+ * processed in slabs) of at most 16 GiB and at most half the free device memory; it
+ * stays in the context (xrd_context_workspace_bytes counts it).  When one view's
+ * gamma exceeds that, the allocation fails, or the environment sets
+ * XRD_VJP_SCATTER=inplace, they are scattered inside the VJP launch instead (same
+ * results up to atomic order). */
 int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
                    const float *spectrum, const int32_t *segmentation, const float *coef,
                    int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,

@@ -99,3 +99,104 @@ def test_num_rows_matches_layout():
     p = make_problem("fan256")
     assert num_rows(p.geometry, p.disc) == 360 * 256 * 61 == p.n_rows
     assert p.n_cols == 1024
+
+
+# ---------------------------------------------------------------------------
+# host-side argument checks of the Python mirror: every bad tensor is refused with
+# TypeError/ValueError before a context is created or a pointer reaches the library
+# (on this CPU-only host a context cannot even be created, so reaching that point
+# would raise RuntimeError instead)
+
+def _small():
+    import torch
+    from xrd_tomography_amd.synthetic import make_problem
+    p = make_problem("small")
+    t = lambda name, dt: torch.from_numpy(p.flat(name)).to(dt)  # noqa: E731
+    args = [t("spectrum", torch.float32), t("ifine", torch.float32),
+            t("segmentation", torch.int32), t("a1", torch.float32), t("a2", torch.float32),
+            t("s2_table", torch.float32)]
+    return p, args
+
+
+@pytest.mark.parametrize("which,bad", [
+    (0, "short"), (1, "short"), (2, "dtype"), (3, "short"), (4, "dtype"), (5, "short"),
+    (0, "host"), (2, "not_tensor")])
+def test_forward_rejects_bad_inputs_before_launch(which, bad):
+    import torch
+    from xrd_tomography_amd import api
+    p, args = _small()
+    if bad == "short":
+        args[which] = args[which][:-1]
+    elif bad == "dtype":
+        args[which] = args[which].to(torch.float64)
+    elif bad == "not_tensor":
+        args[which] = args[which].numpy()
+    with pytest.raises((TypeError, ValueError)) as ei:
+        api.Forward_XRD(*args, *p.lut, geometry=p.geometry, disc=p.disc)
+    if bad == "host":  # right sizes and types: the device check refuses host memory
+        assert "GPU tensors only" in str(ei.value)
+
+
+def test_forward_rejects_bad_outputs_and_slices():
+    import torch
+    from xrd_tomography_amd import api
+    p, args = _small()
+    rows = p.n_rows
+    good = (torch.empty(rows), torch.empty(rows), torch.empty(rows * p.n_cols))
+    for out in ((torch.empty(rows - 1), good[1], good[2]),
+                (good[0], torch.empty(rows, dtype=torch.float64), good[2]),
+                (good[0], good[1], torch.empty(rows * p.n_cols - 4))):
+        with pytest.raises((TypeError, ValueError)):
+            api.Forward_XRD(*args, *p.lut, geometry=p.geometry, disc=p.disc, out=out)
+    for n_slices, sl in ((2, 2), (2, -1), (3, 0)):  # out of range; volume sizes short
+        with pytest.raises(ValueError):
+            api.forward_slice(*args, *p.lut, n_slices, sl, geometry=p.geometry, disc=p.disc)
+
+
+def test_matrix_operators_reject_bad_inputs():
+    import torch
+    from xrd_tomography_amd import api
+    A = torch.zeros(10, 8)
+    for call in (lambda: api.project(A, torch.zeros(7)),
+                 lambda: api.project(A, torch.zeros(8, dtype=torch.float64)),
+                 lambda: api.backproject(A, torch.zeros(9)),
+                 lambda: api.project(A.double(), torch.zeros(8)),
+                 lambda: api.project(A[:, 0], torch.zeros(8)),
+                 lambda: api.convert_matrix(A.double(), torch.float16),
+                 lambda: api.convert_matrix_scaled(A, torch.float32),
+                 lambda: api.MlemPlan(A, torch.zeros(10), torch.zeros(9), torch.zeros(8)),
+                 lambda: api.MlemPlan(A, torch.zeros(10), torch.zeros(10), torch.zeros(8),
+                                      col_scale=torch.zeros(7)),
+                 lambda: api.MlemBatch([(A, torch.zeros(10), torch.zeros(10), torch.zeros(9))]),
+                 lambda: api.project(A, torch.zeros(8))):  # right sizes, host memory
+        with pytest.raises((TypeError, ValueError)):
+            call()
+
+
+def test_coefficient_and_ct_wrappers_reject_bad_inputs():
+    import torch
+    from xrd_tomography_amd import api
+    from xrd_tomography_amd.synthetic import make_coeff_problem, make_problem
+    cp = make_coeff_problem(make_problem("small"), K=4, full_mu=False)
+    g, d = cp.base.geometry, cp.base.disc
+    f = lambda a: torch.from_numpy(a.copy())  # noqa: E731
+    sp = f(cp.base.flat("spectrum"))
+    seg = f(cp.base.flat("segmentation"))
+    good = [sp, seg, f(cp.coef), f(cp.basis), f(cp.maps), f(cp.beta)]
+    bads = {2: f(cp.coef[:-1]), 3: f(cp.basis[:, :-1]), 4: f(cp.maps[:, :-1]),
+            5: f(cp.beta[:1]), 1: seg.to(torch.int64)}
+    for i, b in bads.items():
+        args = list(good)
+        args[i] = b
+        with pytest.raises((TypeError, ValueError)):
+            api.forward_coeffs(*args, geometry=g, disc=d)
+    with pytest.raises(ValueError):
+        api.vjp_coeffs(*good, torch.zeros(cp.base.n_rows - 1), geometry=g, disc=d)
+    with pytest.raises(ValueError):
+        api.forward_ct(sp, f(cp.maps), f(cp.beta[:, :-1]), geometry=g, disc=d)
+    with pytest.raises(ValueError):
+        api.vjp_ct(sp, f(cp.maps), f(cp.beta), torch.zeros(3), geometry=g, disc=d)
+    with pytest.raises(ValueError):
+        api.poisson_nll(torch.zeros(5), torch.zeros(4))
+    with pytest.raises(ValueError):
+        api.trace_rays(torch.zeros(3, 3, dtype=torch.float64), 4, 4)

@@ -70,13 +70,36 @@ def _stream(stream=None):
 
 
 def _require_gpu(*ts):
+    """Every tensor given (None entries skipped) is a contiguous tensor on one GPU."""
+    dev = None
     for t in ts:
         if t is None:
             continue
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"expected a torch.Tensor, got {type(t).__name__}")
         if not t.is_cuda:
             raise ValueError("xrd_tomography_amd operates on GPU tensors only")
         if not t.is_contiguous():
             raise ValueError("tensors must be contiguous")
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError(f"tensors on different devices: {dev} and {t.device}")
+
+
+def _check(t, name: str, dtype, numel: Optional[int] = None, optional: bool = False):
+    """Type, dtype and element count of one argument, checked on the host before the
+    library sees its pointer (the C ABI cannot see tensor sizes)."""
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: expected {dtype}, got {t.dtype}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} elements, got {t.numel()}")
 
 
 class Context:
@@ -132,21 +155,22 @@ def Forward_XRD(spectrum: torch.Tensor, Ifine: torch.Tensor, segmentation: torch
     segmentation [NX*NY] int32, a1/a2 [NX*NY] float32 in 1/voxel, S2_grid [NQ*NA]).
     Returns (Nphotons, Cphotons, Amatrix); Amatrix is None if with_A is False."""
     g, d = geometry, disc
-    ctx = ctx or default_context(spectrum.device.index or 0)
-    _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid)
-    for t, dt in ((spectrum, torch.float32), (Ifine, torch.float32),
-                  (segmentation, torch.int32), (a1, torch.float32), (a2, torch.float32),
-                  (S2_grid, torch.float32)):
-        if t.dtype != dt:
-            raise TypeError(f"expected {dt}, got {t.dtype}")
+    _check_forward_inputs(g, d, spectrum, Ifine, segmentation, a1, a2, S2_grid, 1)
     shape_n = (d.n_energy, g.n_rows, g.n_cols, g.n_views)
+    rows = g.n_views * g.n_cols * g.n_rows * d.n_energy
     if out is None:
+        _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid)
         dev = spectrum.device
         N = torch.empty(shape_n, dtype=torch.float32, device=dev)
         C = torch.empty(shape_n, dtype=torch.float32, device=dev)
         A = torch.empty(shape_n + (d.n_mat, d.n_q), dtype=torch.float32, device=dev) if with_A else None
     else:
         N, C, A = out
+        _check(N, "out[0] (Nphotons)", torch.float32, rows)
+        _check(C, "out[1] (Cphotons)", torch.float32, rows)
+        _check(A, "out[2] (Amatrix)", torch.float32, rows * d.n_q * d.n_mat, optional=True)
+        _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid, N, C, A)
+    ctx = ctx or default_context(spectrum.device.index or 0)
     rc = lib().xrd_forward(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
                            _ptr(Ifine), _ptr(segmentation), _ptr(a1), _ptr(a2), _ptr(S2_grid),
                            float(q_min), float(q_max), float(a_min), float(a_max),
@@ -155,14 +179,27 @@ def Forward_XRD(spectrum: torch.Tensor, Ifine: torch.Tensor, segmentation: torch
     return N, C, A
 
 
+def _check_forward_inputs(g, d, spectrum, Ifine, segmentation, a1, a2, S2_grid, n_slices):
+    npix = g.nx * g.ny * n_slices
+    _check(spectrum, "spectrum", torch.float32, d.n_energy * d.n_energy)
+    _check(Ifine, "Ifine", torch.float32, d.n_qf * d.n_mat)
+    _check(segmentation, "segmentation", torch.int32, npix)
+    _check(a1, "a1", torch.float32, npix)
+    _check(a2, "a2", torch.float32, npix)
+    _check(S2_grid, "S2_grid", torch.float32, d.lut_nq * d.lut_na)
+
+
 def forward_slice(spectrum, Ifine, segmentation, a1, a2, S2_grid, q_min, q_max, a_min, a_max,
                   n_slices: int, slice_: int, *, geometry: Geometry, disc: Discretization,
                   with_A: bool = True, ctx: Optional[Context] = None, stream=None):
     """xrd_forward_slice: slice slice_ of a volume (segmentation, a1, a2 flat
     [NX*NY*n_slices] device tensors, x fastest); returns (N, C, A) of that slice."""
     g, d = geometry, disc
-    ctx = ctx or default_context(spectrum.device.index or 0)
+    if not 0 <= int(slice_) < int(n_slices):
+        raise ValueError(f"slice {slice_} outside [0, {n_slices})")
+    _check_forward_inputs(g, d, spectrum, Ifine, segmentation, a1, a2, S2_grid, int(n_slices))
     _require_gpu(spectrum, Ifine, segmentation, a1, a2, S2_grid)
+    ctx = ctx or default_context(spectrum.device.index or 0)
     shape_n = (d.n_energy, g.n_rows, g.n_cols, g.n_views)
     dev = spectrum.device
     N = torch.empty(shape_n, dtype=torch.float32, device=dev)
@@ -204,8 +241,17 @@ def trace_rays(segments: torch.Tensor, nx: int, ny: int, step: float = 0.25,
     """GetIntersection + attenuator sampling for n segments [n, 4] float64 (x1,y1,x2,y2).
     Returns (side int32 [n], entry float64 [n,2], count int32 [n], texels int32
     [n, max_samples, 2] or None, integrals float32 [n_basis, n] or None)."""
-    ctx = ctx or default_context(segments.device.index or 0)
+    _check(segments, "segments", torch.float64)
+    if segments.dim() != 2 or segments.shape[1] != 4:
+        raise ValueError("segments must be [n, 4] (x1, y1, x2, y2)")
+    if maps is not None:
+        _check(maps, "maps", torch.float32)
+        if maps.numel() == 0 or maps.numel() % (nx * ny) != 0:
+            raise ValueError("maps must hold n_basis whole [nx*ny] maps")
+    if max_samples < 0:
+        raise ValueError("max_samples < 0")
     _require_gpu(segments, maps)
+    ctx = ctx or default_context(segments.device.index or 0)
     n = segments.shape[0]
     dev = segments.device
     side = torch.empty(n, dtype=torch.int32, device=dev)
@@ -222,21 +268,30 @@ def trace_rays(segments: torch.Tensor, nx: int, ny: int, step: float = 0.25,
 
 
 def _mat(A):
+    """(dtype code, rows, cols) of a model matrix on the GPU."""
     if isinstance(A, Bf24Matrix):
         _require_gpu(A.data)
         return DTYPE_BF24, A.shape[0], A.shape[1]
-    _require_gpu(A)
+    if not isinstance(A, torch.Tensor):
+        raise TypeError(f"A: expected a torch.Tensor or Bf24Matrix, got {type(A).__name__}")
     if A.dim() != 2:
         raise ValueError("A must be 2-D [rows, cols]")
     if A.dtype not in _DTYPES:
         raise TypeError(f"unsupported A dtype {A.dtype}")
+    _require_gpu(A)
     return _DTYPES[A.dtype], A.shape[0], A.shape[1]
+
+
+def _mat_tensor(A):
+    return A.data if isinstance(A, Bf24Matrix) else A
 
 
 def project(A: torch.Tensor, x: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
     """A x (FP32 output)."""
-    ctx = ctx or default_context(A.device.index or 0)
     dt, nr, nc = _mat(A)
+    _check(x, "x", torch.float32, nc)
+    _require_gpu(_mat_tensor(A), x)
+    ctx = ctx or default_context(A.device.index or 0)
     out = torch.empty(nr, dtype=torch.float32, device=A.device)
     check(lib().xrd_project(ctx.handle, _ptr(A), dt, nr, nc, _ptr(x), _ptr(out),
                             _stream(stream)), "xrd_project")
@@ -245,8 +300,10 @@ def project(A: torch.Tensor, x: torch.Tensor, ctx=None, stream=None) -> torch.Te
 
 def backproject(A: torch.Tensor, w: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
     """A' w with FP64 column sums."""
-    ctx = ctx or default_context(A.device.index or 0)
     dt, nr, nc = _mat(A)
+    _check(w, "w", torch.float32, nr)
+    _require_gpu(_mat_tensor(A), w)
+    ctx = ctx or default_context(A.device.index or 0)
     out = torch.empty(nc, dtype=torch.float64, device=A.device)
     check(lib().xrd_backproject(ctx.handle, _ptr(A), dt, nr, nc, _ptr(w), _ptr(out),
                                 _stream(stream)), "xrd_backproject")
@@ -255,8 +312,8 @@ def backproject(A: torch.Tensor, w: torch.Tensor, ctx=None, stream=None) -> torc
 
 def sensitivity(A: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
     """sum(A, 1)' with FP64 column sums."""
-    ctx = ctx or default_context(A.device.index or 0)
     dt, nr, nc = _mat(A)
+    ctx = ctx or default_context(A.device.index or 0)
     out = torch.empty(nc, dtype=torch.float64, device=A.device)
     check(lib().xrd_sensitivity(ctx.handle, _ptr(A), dt, nr, nc, _ptr(out), _stream(stream)),
           "xrd_sensitivity")
@@ -266,8 +323,9 @@ def sensitivity(A: torch.Tensor, ctx=None, stream=None) -> torch.Tensor:
 def convert_matrix(A: torch.Tensor, dtype, ctx=None, stream=None):
     """FP32 -> BF16/F16 (torch tensor) or BF24 (``dtype=BF24``: a Bf24Matrix) storage,
     round to nearest even, with the library's kernel."""
-    ctx = ctx or default_context(A.device.index or 0)
+    _check(A, "A", torch.float32)
     _require_gpu(A)
+    ctx = ctx or default_context(A.device.index or 0)
     if dtype == BF24:
         if A.dim() != 2:
             raise ValueError("A must be 2-D [rows, cols]")
@@ -288,14 +346,10 @@ class MlemPlan:
     def __init__(self, A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, x: torch.Tensor,
                  fused: bool = True, col_scale: Optional[torch.Tensor] = None, ctx=None,
                  stream=None):
-        self.ctx = ctx or default_context(A.device.index or 0)
         dt, nr, nc = _mat(A)
-        _require_gpu(y, c, x)
-        if y.numel() != nr or c.numel() != nr or x.numel() != nc:
-            raise ValueError("y, c must have n_rows elements and x n_cols elements")
-        for t in (y, c, x):
-            if t.dtype != torch.float32:
-                raise TypeError("y, c and x must be float32")
+        _check_mlem_vectors(nr, nc, y, c, x, col_scale)
+        _require_gpu(_mat_tensor(A), y, c, x, col_scale)
+        self.ctx = ctx or default_context(A.device.index or 0)
         self._keep = (A, y, c, x, col_scale)
         self.x = x
         h = ctypes.c_void_p()
@@ -374,6 +428,13 @@ class MlemPlan:
             pass
 
 
+def _check_mlem_vectors(nr, nc, y, c, x, col_scale):
+    _check(y, "y", torch.float32, nr)
+    _check(c, "c", torch.float32, nr)
+    _check(x, "x", torch.float32, nc)
+    _check(col_scale, "col_scale", torch.float32, nc, optional=True)
+
+
 def _wrap_device_ptr(ptr: int, n: int, dtype, device):
     """Non-owning torch view of n elements at a device pointer owned by the library."""
     class _Holder:
@@ -389,8 +450,13 @@ def _wrap_device_ptr(ptr: int, n: int, dtype, device):
 def convert_matrix_scaled(A: torch.Tensor, dtype: torch.dtype, ctx=None, stream=None):
     """(A16, col_scale): BF16/F16 storage of A*col_scale with power-of-two per-column
     scales chosen by the library (xrd_convert_matrix_scaled)."""
-    ctx = ctx or default_context(A.device.index or 0)
+    _check(A, "A", torch.float32)
+    if A.dim() != 2:
+        raise ValueError("A must be 2-D [rows, cols]")
+    if dtype not in (torch.bfloat16, torch.float16):
+        raise TypeError("dtype must be torch.bfloat16 or torch.float16")
     _require_gpu(A)
+    ctx = ctx or default_context(A.device.index or 0)
     out = torch.empty(A.shape, dtype=dtype, device=A.device)
     sc = torch.empty(A.shape[1], dtype=torch.float32, device=A.device)
     check(lib().xrd_convert_matrix_scaled(ctx.handle, _ptr(A), A.shape[0], A.shape[1],
@@ -416,9 +482,10 @@ def forward_coeffs(spectrum, segmentation, coef, basis, maps, beta, *, geometry:
                    disc: Discretization, ctx=None, stream=None) -> torch.Tensor:
     """Counts of the coefficient model (xrd_forward_coeffs): coef [nx*ny, K],
     basis [K, NWT], maps [B, nx*ny], beta [B, NRG], float32 device tensors."""
-    ctx = ctx or default_context(spectrum.device.index or 0)
-    _require_gpu(spectrum, segmentation, coef, basis, maps, beta)
     g, d = geometry, disc
+    _check_coeff_inputs(g, d, spectrum, segmentation, coef, basis, maps, beta)
+    _require_gpu(spectrum, segmentation, coef, basis, maps, beta)
+    ctx = ctx or default_context(spectrum.device.index or 0)
     out = torch.empty((d.n_energy, g.n_rows, g.n_cols, g.n_views), dtype=torch.float32,
                       device=spectrum.device)
     check(lib().xrd_forward_coeffs(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
@@ -428,17 +495,39 @@ def forward_coeffs(spectrum, segmentation, coef, basis, maps, beta, *, geometry:
     return out
 
 
+def _check_coeff_inputs(g, d, spectrum, segmentation, coef, basis, maps, beta):
+    npix = g.nx * g.ny
+    _check(spectrum, "spectrum", torch.float32, d.n_energy * d.n_energy)
+    _check(segmentation, "segmentation", torch.int32, npix)
+    for t, name in ((coef, "coef"), (basis, "basis"), (maps, "maps"), (beta, "beta")):
+        _check(t, name, torch.float32)
+        if t.dim() != 2:
+            raise ValueError(f"{name} must be 2-D")
+    K, B = coef.shape[1], maps.shape[0]
+    if coef.shape[0] != npix:
+        raise ValueError(f"coef must be [nx*ny = {npix}, K]")
+    if tuple(basis.shape) != (K, d.n_q):
+        raise ValueError(f"basis must be [K = {K}, NWT = {d.n_q}]")
+    if maps.shape[1] != npix:
+        raise ValueError(f"maps must be [B, nx*ny = {npix}]")
+    if tuple(beta.shape) != (B, d.n_energy):
+        raise ValueError(f"beta must be [B = {B}, NRG = {d.n_energy}]")
+
+
 def vjp_coeffs(spectrum, segmentation, coef, basis, maps, beta, grad_counts, *,
                geometry: Geometry, disc: Discretization, want_coef=True, want_maps=True,
                ctx=None, stream=None):
     """(grad_coef [nx*ny, K] float64, grad_maps [B, nx*ny] float64) for the upstream
     gradient grad_counts [rows] (xrd_vjp_coeffs)."""
-    ctx = ctx or default_context(spectrum.device.index or 0)
+    g, d = geometry, disc
+    _check_coeff_inputs(g, d, spectrum, segmentation, coef, basis, maps, beta)
+    _check(grad_counts, "grad_counts", torch.float32,
+           g.n_views * g.n_cols * g.n_rows * d.n_energy)
     _require_gpu(spectrum, segmentation, coef, basis, maps, beta, grad_counts)
+    ctx = ctx or default_context(spectrum.device.index or 0)
     dev = spectrum.device
     gc = torch.empty(coef.shape, dtype=torch.float64, device=dev) if want_coef else None
     gm = torch.empty(maps.shape, dtype=torch.float64, device=dev) if want_maps else None
-    g, d = geometry, disc
     check(lib().xrd_vjp_coeffs(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
                                _ptr(segmentation), _ptr(coef), coef.shape[1], _ptr(basis),
                                maps.shape[0], _ptr(maps), _ptr(beta), _ptr(grad_counts),
@@ -450,9 +539,10 @@ def forward_ct(spectrum, maps, beta, *, geometry: Geometry, disc: Discretization
                stream=None) -> torch.Tensor:
     """CT transmission [V*COL*NRG] (view fastest) of the attenuation maps [B, nx*ny]
     with energy coefficients beta [B, NRG] (xrd_forward_ct)."""
-    ctx = ctx or default_context(spectrum.device.index or 0)
-    _require_gpu(spectrum, maps, beta)
     g, d = geometry, disc
+    _check_ct_inputs(g, d, spectrum, maps, beta)
+    _require_gpu(spectrum, maps, beta)
+    ctx = ctx or default_context(spectrum.device.index or 0)
     out = torch.empty(g.n_views * g.n_cols * d.n_energy, dtype=torch.float32,
                       device=spectrum.device)
     check(lib().xrd_forward_ct(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
@@ -461,13 +551,25 @@ def forward_ct(spectrum, maps, beta, *, geometry: Geometry, disc: Discretization
     return out
 
 
+def _check_ct_inputs(g, d, spectrum, maps, beta):
+    _check(spectrum, "spectrum", torch.float32, d.n_energy * d.n_energy)
+    _check(maps, "maps", torch.float32)
+    _check(beta, "beta", torch.float32)
+    if maps.dim() != 2 or maps.shape[1] != g.nx * g.ny:
+        raise ValueError(f"maps must be [B, nx*ny = {g.nx * g.ny}]")
+    if tuple(beta.shape) != (maps.shape[0], d.n_energy):
+        raise ValueError(f"beta must be [B = {maps.shape[0]}, NRG = {d.n_energy}]")
+
+
 def vjp_ct(spectrum, maps, beta, grad_t, *, geometry: Geometry, disc: Discretization, ctx=None,
            stream=None) -> torch.Tensor:
     """grad_maps [B, nx*ny] float64 of <grad_t, forward_ct(maps)> (xrd_vjp_ct)."""
-    ctx = ctx or default_context(spectrum.device.index or 0)
-    _require_gpu(spectrum, maps, beta, grad_t)
-    gm = torch.empty(maps.shape, dtype=torch.float64, device=spectrum.device)
     g, d = geometry, disc
+    _check_ct_inputs(g, d, spectrum, maps, beta)
+    _check(grad_t, "grad_t", torch.float32, g.n_views * g.n_cols * d.n_energy)
+    _require_gpu(spectrum, maps, beta, grad_t)
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    gm = torch.empty(maps.shape, dtype=torch.float64, device=spectrum.device)
     check(lib().xrd_vjp_ct(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
                            maps.shape[0], _ptr(maps), _ptr(beta), _ptr(grad_t), _ptr(gm),
                            _stream(stream)), "xrd_vjp_ct")
@@ -476,8 +578,11 @@ def vjp_ct(spectrum, maps, beta, grad_t, *, geometry: Geometry, disc: Discretiza
 
 def poisson_nll(counts, y, background=None, want_grad=True, ctx=None, stream=None):
     """(loss: float, grad: float32 tensor 1 - y/f or None), f = counts + background."""
-    ctx = ctx or default_context(counts.device.index or 0)
+    _check(counts, "counts", torch.float32)
+    _check(y, "y", torch.float32, counts.numel())
+    _check(background, "background", torch.float32, counts.numel(), optional=True)
     _require_gpu(counts, y, background)
+    ctx = ctx or default_context(counts.device.index or 0)
     grad = torch.empty_like(counts) if want_grad else None
     v = ctypes.c_double()
     check(lib().xrd_poisson_nll(ctx.handle, counts.numel(), _ptr(counts), _ptr(background),
@@ -504,7 +609,6 @@ class MlemBatch:
     def __init__(self, problems, ctx=None, stream=None):
         if not problems:
             raise ValueError("no problems")
-        self.ctx = ctx or default_context(problems[0][0].device.index or 0)
         dts = {p[0].dtype for p in problems}
         if len(dts) != 1:
             raise TypeError("all matrices of a batch must share one dtype")
@@ -516,11 +620,13 @@ class MlemBatch:
             Ak, yk, ck, xk = p[:4]
             s_k = p[4] if len(p) > 4 else None
             dt, nr, nc = _mat(Ak)
-            _require_gpu(yk, ck, xk, s_k)
+            _check_mlem_vectors(nr, nc, yk, ck, xk, s_k)
+            _require_gpu(_mat_tensor(Ak), yk, ck, xk, s_k)
             A[k], y[k], c[k], x[k] = Ak.data_ptr(), yk.data_ptr(), ck.data_ptr(), xk.data_ptr()
             sc[k] = s_k.data_ptr() if s_k is not None else None
             rows[k], cols[k] = nr, nc
         self._keep = problems
+        self.ctx = ctx or default_context(problems[0][0].device.index or 0)
         h = ctypes.c_void_p()
         dt = dts.pop()
         check(lib().xrd_mlem_batch_create(self.ctx.handle, n,

@@ -1020,14 +1020,24 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
         HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
         ca.n_vox = nv;
         const size_t per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
-        // one view's gamma must fit the budget; larger problems scatter in place
-        if (nv > 0 && per_view * sizeof(float) <= kGamBudget) {
+        // The budget is also held to half of the device memory that is free (counting
+        // the buffer the context already holds), so the slab leaves room for the
+        // caller.  One view's gamma must fit it, and the allocation must succeed;
+        // otherwise the detector legs are scattered in place (same results up to the
+        // atomic order).
+        size_t budget = kGamBudget;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            budget = std::min(budget, (free_b + ctx->cm_gam.bytes) / 2);
+        if (nv > 0 && per_view * sizeof(float) <= budget) {
             const size_t views = std::max<size_t>(
-                1, std::min<size_t>((size_t)g->n_views, kGamBudget / (per_view * sizeof(float))));
-            HIP_TRY(ctx->cm_gam.ensure(views * per_view * sizeof(float)), "workspace alloc");
-            ca.gam_buf = (float *)ctx->cm_gam.ptr;
-            HIP_TRY(launch_coeff_vjp_bundled(ca, views * per_view, s), "bundled map VJP");
-            return XRD_OK;
+                1, std::min<size_t>((size_t)g->n_views, budget / (per_view * sizeof(float))));
+            if (ctx->cm_gam.ensure(views * per_view * sizeof(float)) == hipSuccess) {
+                ca.gam_buf = (float *)ctx->cm_gam.ptr;
+                HIP_TRY(launch_coeff_vjp_bundled(ca, views * per_view, s), "bundled map VJP");
+                return XRD_OK;
+            }
+            (void)hipGetLastError();  // the failed allocation is not the caller's error
         }
     }
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
