@@ -332,6 +332,25 @@ int xrd_mlem_plan_set_launch(xrd_mlem_plan *plan, int32_t blocks_per_cu,
 const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *plan);
 int xrd_mlem_plan_destroy(xrd_mlem_plan *plan);
 
+/* ---- MLEM without allocation ----------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * n_iter MLEM updates of x in place with the kernels, arithmetic and default launch
+ * shape of xrd_mlem_plan_create + xrd_mlem_plan_iterate (XRD_MLEM_FUSED, no graph):
+ * x bit-identical to that plan's.  The sensitivity sum(A,1)' is recomputed on every
+ * call (one extra pass over A) and kept, with the per-workgroup FP64 partials, in a
+ * fixed device array of XRD_MLEM_STATIC_SCRATCH_BYTES that the library holds on each
+ * GPU until the process ends; the call makes no hipMalloc/hipFree.  A problem needs
+ * (workgroups + 1) * n_cols * 8 bytes of it, where workgroups = min(2 * CUs,
+ * ceil(n_rows / 16)): up to 1,024 columns at any row count on a 256-CU MI355X.
+ * Larger problems get XRD_ERR_DIMS.  Every call on one GPU shares that GPU's array,
+ * and it keeps the last call's contents: calls that overlap in time on one GPU (other
+ * streams or host threads) race.  ctx selects the device; its workspace is not used. */
+#define XRD_MLEM_STATIC_SCRATCH_BYTES 4259840
+int xrd_mlem_iterate_static(xrd_context *ctx, const void *A, int a_dtype, int64_t n_rows,
+                            int32_t n_cols, const float *col_scale, const float *y,
+                            const float *c, float *x, int32_t n_iter, void *stream);
+
 /* ---- batched MLEM ------------------------------------------------------------
  * MLEM for n_problems independent explicit matrices (phantoms or slices of uneven
  * size and view count) with one persistent launch per pass over all of them.

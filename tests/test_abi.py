@@ -85,6 +85,7 @@ def test_error_paths_without_device(L):
     assert L.xrd_mlem_plan_create(None, None, 0, 10, 8, None, None, None, None, 0, None,
                                   ctypes.byref(ctypes.c_void_p())) == -2
     assert L.xrd_mlem_plan_iterate(None, 1, 0, None) == -2
+    assert L.xrd_mlem_iterate_static(None, None, 0, 10, 8, None, None, None, None, 1, None) == -2
     assert L.xrd_trace_rays(None, 4, 4, 0.25, 1, None, None, None, None, 0, None, 0, None,
                             None, None) == -2
     assert L.xrd_version() >= 100
@@ -167,6 +168,8 @@ def test_matrix_operators_reject_bad_inputs():
                  lambda: api.MlemPlan(A, torch.zeros(10), torch.zeros(9), torch.zeros(8)),
                  lambda: api.MlemPlan(A, torch.zeros(10), torch.zeros(10), torch.zeros(8),
                                       col_scale=torch.zeros(7)),
+                 lambda: api.mlem_iterate_static(A, torch.zeros(10), torch.zeros(10),
+                                                 torch.zeros(8, dtype=torch.float16), 1),
                  lambda: api.MlemBatch([(A, torch.zeros(10), torch.zeros(10), torch.zeros(9))]),
                  lambda: api.project(A, torch.zeros(8))):  # right sizes, host memory
         with pytest.raises((TypeError, ValueError)):

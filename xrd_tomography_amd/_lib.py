@@ -34,8 +34,9 @@ EXPORTS = (
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
     "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
     "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups", "xrd_forward_ct", "xrd_vjp_ct",
-    "xrd_forward_slice",
+    "xrd_forward_slice", "xrd_mlem_iterate_static",
 )
+MLEM_STATIC_SCRATCH_BYTES = 4259840  # XRD_MLEM_STATIC_SCRATCH_BYTES
 
 
 class XrdError(RuntimeError):
@@ -97,6 +98,8 @@ def _declare(L):
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),
+        "xrd_mlem_iterate_static": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp,
+                                                   _vp, _vp, _vp, _i32, _vp]),
         "xrd_mlem_plan_loglik": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl), _vp]),
         "xrd_mlem_plan_sensitivity": (_vp, [_vp]),
         "xrd_mlem_plan_destroy": (ctypes.c_int, [_vp]),

@@ -465,6 +465,22 @@ def convert_matrix_scaled(A: torch.Tensor, dtype: torch.dtype, ctx=None, stream=
     return out, sc
 
 
+def mlem_iterate_static(A, y: torch.Tensor, c: torch.Tensor, x: torch.Tensor, n_iter: int,
+                        col_scale: Optional[torch.Tensor] = None, ctx=None, stream=None):
+    """n_iter MLEM updates of x in place through xrd_mlem_iterate_static: the fused plan's
+    kernels and default launch shape with every buffer in the library's fixed device
+    scratch, so the call allocates nothing (usable under graph capture).  Calls that
+    overlap on one GPU race on that scratch."""
+    dt, nr, nc = _mat(A)
+    _check_mlem_vectors(nr, nc, y, c, x, col_scale)
+    _require_gpu(_mat_tensor(A), y, c, x, col_scale)
+    ctx = ctx or default_context(x.device.index or 0)
+    check(lib().xrd_mlem_iterate_static(ctx.handle, _ptr(A), dt, nr, nc, _ptr(col_scale),
+                                        _ptr(y), _ptr(c), _ptr(x), int(n_iter),
+                                        _stream(stream)), "xrd_mlem_iterate_static")
+    return x
+
+
 def mlem(A: torch.Tensor, y: torch.Tensor, c: torch.Tensor, n_iter: int, x0: float = 10.0,
          fused: bool = True, use_graph: bool = False, col_scale=None, ctx=None) -> torch.Tensor:
     """n_iter MLEM updates from x = x0 (main.m:154) on device; returns x (float32)."""

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/xrd_tomo.h"
@@ -61,6 +62,44 @@ struct DevBuf {
         bytes = 0;
     }
 };
+
+// ---- device scratch of xrd_mlem_iterate_static -------------------------------------
+// A fixed array in every GPU's copy of the code object: the FP64 sensitivity, then the
+// per-workgroup FP64 partials of the fused MLEM pass.  In host code `scratch` is only a
+// symbol handle; kernels get the device address that hipGetSymbolAddress returns for
+// the current device (scratch_ptr).
+constexpr size_t kScratchDoubles = XRD_MLEM_STATIC_SCRATCH_BYTES / sizeof(double);
+static_assert(XRD_MLEM_STATIC_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+bool g_scratch_resolved[kMaxDevices] = {};
+double *g_scratch_dev[kMaxDevices] = {};
+hipError_t g_scratch_status[kMaxDevices] = {};
+
+// With `device` current: looks up that device's address of `scratch` once.
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_resolved[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_status[device] = e;
+    g_scratch_dev[device] = e == hipSuccess ? static_cast<double *>(p) : nullptr;
+    g_scratch_resolved[device] = true;
+}
+
+// The device address of `device`'s scratch (NULL if the lookup failed or never ran).
+double *scratch_ptr(int device, hipError_t *status) {
+    if (device < 0 || device >= kMaxDevices) {
+        *status = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    *status = g_scratch_resolved[device] ? g_scratch_status[device] : hipErrorNotInitialized;
+    return g_scratch_dev[device];
+}
 
 }  // namespace
 
@@ -176,6 +215,8 @@ int xrd_context_create(int device, xrd_context **out) {
     if (device < 0 || device >= n)
         return fail(XRD_ERR_INVALID_ARG, "xrd_context_create: device %d of %d", device, n);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    // set-up of the allocation-free MLEM entry point: this device's scratch address
+    resolve_scratch(device);
     xrd_context *c = new xrd_context();
     c->device = device;
     hipError_t e = hipEventCreateWithFlags(&c->staged, hipEventDisableTiming);
@@ -879,6 +920,48 @@ extern "C" int xrd_mlem_plan_set_launch(xrd_mlem_plan *p, int32_t blocks_per_cu,
     if (p->graph) {
         (void)hipGraphDestroy(p->graph);
         p->graph = nullptr;
+    }
+    return XRD_OK;
+}
+
+extern "C" int xrd_mlem_iterate_static(xrd_context *ctx, const void *A, int a_dtype,
+                                       int64_t n_rows, int32_t n_cols, const float *col_scale,
+                                       const float *y, const float *c, float *x,
+                                       int32_t n_iter, void *stream) {
+    const char *fn = "xrd_mlem_iterate_static";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    int rc = check_matrix(A, a_dtype, n_rows, n_cols, fn);
+    if (rc) return rc;
+    if (!y || !c || !x) return fail(XRD_ERR_NULL_POINTER, "%s: y/c/x is NULL", fn);
+    if (!aligned(x, 16)) return fail(XRD_ERR_ALIGN, "%s: x must be 16-byte aligned", fn);
+    if (col_scale && !aligned(col_scale, 16))
+        return fail(XRD_ERR_ALIGN, "%s: col_scale must be 16-byte aligned", fn);
+    if (n_iter < 0) return fail(XRD_ERR_INVALID_ARG, "%s: n_iter < 0", fn);
+    hipError_t st = hipSuccess;
+    double *base = scratch_ptr(ctx->device, &st);
+    if (!base)
+        return fail(XRD_ERR_HIP, "%s: no device scratch on device %d (%s)", fn, ctx->device,
+                    hipGetErrorString(st));
+    // the plan's default launch shape; sensitivity first, partials 256-byte aligned after it
+    const MatArgs m{A, a_dtype, n_rows, n_cols, col_scale};
+    const int nb = mat_num_blocks(m);
+    const size_t sens_doubles = ((size_t)n_cols + 31) / 32 * 32;
+    const size_t need = sens_doubles + (size_t)nb * n_cols;
+    if (need > kScratchDoubles)
+        return fail(XRD_ERR_DIMS,
+                    "%s: %d workgroups x %d columns need %zu bytes of device scratch, more "
+                    "than the %d bytes of XRD_MLEM_STATIC_SCRATCH_BYTES",
+                    fn, nb, n_cols, need * sizeof(double), XRD_MLEM_STATIC_SCRATCH_BYTES);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    double *sens = base;
+    double *partial = base + sens_doubles;
+    hipStream_t s = (hipStream_t)stream;
+    // every workgroup writes all n_cols of its partial slot, so nothing is cleared
+    HIP_TRY(launch_colsum_partials(m, nullptr, partial, s), "sensitivity kernel");
+    HIP_TRY(launch_reduce_partials(partial, nb, n_cols, sens, s, col_scale), "reduce kernel");
+    for (int i = 0; i < n_iter; i++) {
+        HIP_TRY(launch_mlem_fused(m, x, y, c, partial, s), "mlem pass");
+        HIP_TRY(launch_mlem_update(partial, nb, n_cols, sens, x, s, col_scale), "mlem update");
     }
     return XRD_OK;
 }
