@@ -76,6 +76,10 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="B,R: fix the fused pass to B workgroups per CU and R rows per wave "
                          "(no autotuning)")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "xrd_tomography_amd", "data",
+                                                         "launch_cache_mi355x.json"),
+                    help="JSON cache of autotuned launch shapes (projector and MLEM pass), "
+                         "read and extended; '' disables it")
     ap.add_argument("--cpu-views", type=int, default=36, help="views in the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="repeat the CPU-baseline update until this much time has passed")
@@ -149,7 +153,12 @@ def main():
     log(f"[bench] workload {args.workload}: rows {nrow}, cols {ncol}, non-air voxels "
         f"{int((prob.segmentation > 0).sum())}")
 
-    # ---- build the model matrix on the GPU (not in the timed region)
+    # ---- build the model matrix on the GPU (not in the timed region), in the
+    # projector launch shape the autotuner picked for this problem shape
+    fwd_tuned = None
+    if not args.no_autotune:
+        fwd_tuned = api.autotune_forward(prob, cache=args.tune_cache or None)
+        log(f"[bench] projector launch: {fwd_tuned}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     N, C, A = api.forward_problem(prob, device=dev)
@@ -202,7 +211,7 @@ def main():
         plan.set_launch(b, r)
         tuned = {"blocks_per_cu": b, "rows_per_wave": r, "fixed": True}
     elif not args.unfused and not args.no_autotune:
-        tuned = plan.autotune(reps=2)
+        tuned = plan.autotune(reps=2, cache=args.tune_cache or None)
         log(f"[bench] autotuned launch: {tuned}")
     use_graph = not args.no_graph
     plan.iterate(args.warmup, use_graph=use_graph)
@@ -322,6 +331,7 @@ def main():
                         f"MLEM{', hipGraph replay' if use_graph else ''}",
             "rows": nrow, "cols": ncol,
             "launch": tuned,
+            "forward_launch": fwd_tuned,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -494,11 +504,11 @@ def run_volume(args, dev):
     view, and the 30-iteration MLEM image with A stored in FP32, BF24, BF16 and F16
     against the FP64 oracle MLEM on the same FP32 matrix (36 of the 720 views).  The
     counts are also reported with the form-factor and attenuation tables in BF16 with
-    a per-material FP32 scale."""
+    a per-material FP32 scale (xrd_forward_bf16), whose A build is timed as well."""
     import numpy as np
     import torch
     from xrd_tomography_amd import api
-    from xrd_tomography_amd.synthetic import make_volume_problem, with_bf16_tables
+    from xrd_tomography_amd.synthetic import decoded_bf16_tables, make_volume_problem
     from oracle import oracle as O
 
     nz, sl = 32, args.volume_slice
@@ -525,8 +535,20 @@ def run_volume(args, dev):
     expected = (N * scale + c).double().cpu().numpy()
     y = torch.from_numpy(np.random.default_rng(11).poisson(expected).astype(np.float32)).to(dev)
 
+    # the same slice with BF16 tables (xrd_forward_bf16: form factors and maps as
+    # bfloat16 with per-material FP32 scales, decoded by the kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Nq, _, Aq = api.forward_problem_bf16(vp, nz, sl)
+    torch.cuda.synchronize()
+    t_build_bf16 = time.perf_counter() - t0
+    del Aq
+    log(f"[vol] A build with BF16 tables {t_build_bf16:.2f} s")
+    Nq = Nq.view(d.n_energy, g.n_rows * g.n_cols, g.n_views)[:, :, 0].reshape(-1)
+    Nq = Nq.double().cpu().numpy()
+
     # counts against the FP64 oracle on one view (all pixels, energies)
-    cnt_err = cnt_err_bf16 = None
+    cnt_err = cnt_err_bf16 = cnt_err_bf16_kernel = None
     if not args.no_cpu:
         t0 = time.perf_counter()
         No, Co, _ = O.forward_slice(vp, nz, sl, views=[0], with_A=False)
@@ -537,15 +559,15 @@ def run_volume(args, dev):
         cnt_err = float(np.max(np.abs(Ng[live] - No[live]) / No[live]))
         log(f"[vol] counts of view 0 against FP64: max rel err {cnt_err:.2e} "
             f"(oracle {t_oracle:.1f} s)")
-        # the same with the form-factor and attenuation tables in BF16 with a
-        # per-material FP32 scale, against the FP64 counts of the FP32 tables
-        Nq, _, _ = api.forward_volume_slice(with_bf16_tables(vp), nz, sl, with_A=False)
-        Nq = Nq.view(d.n_energy, g.n_rows * g.n_cols, g.n_views)[:, :, 0].reshape(-1)
-        Nq = Nq.double().cpu().numpy()
+        # BF16 tables against the FP64 counts of the FP32 tables (storage and kernel
+        # error) and of the decoded tables (the kernels' own error)
         cnt_err_bf16 = float(np.max(np.abs(Nq[live] - No[live]) / No[live]))
-        del Nq
+        Nd, _, _ = O.forward_slice(decoded_bf16_tables(vp), nz, sl, views=[0], with_A=False)
+        lived = Nd > 1e-9 * Nd.max()
+        cnt_err_bf16_kernel = float(np.max(np.abs(Nq[lived] - Nd[lived]) / Nd[lived]))
         log(f"[vol] counts of view 0 with BF16 tables against FP64: max rel err "
-            f"{cnt_err_bf16:.2e}")
+            f"{cnt_err_bf16:.2e} (FP32 tables), {cnt_err_bf16_kernel:.2e} (decoded tables)")
+    del Nq
 
     # MLEM image per storage format on 36 of the 720 views, against FP64 MLEM
     views = np.linspace(0, g.n_views - 1, 36).round().astype(np.int64)
@@ -599,11 +621,13 @@ def run_volume(args, dev):
                                f"pixels x 61 energies, 128 q-bins x 10 materials; 3-D "
                                f"trilinear attenuation legs", "rows": nrow, "cols": ncol},
         "a_build_s": round(t_build, 3),
+        "a_build_bf16_tables_s": round(t_build_bf16, 3),
         "roofline": {"bound": "hbm", "achieved": round(bytes_it / ms_it / 1e6, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_it / ms_it / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
         "counts_rel_err_vs_fp64_view0": cnt_err,
         "counts_rel_err_bf16_tables_vs_fp64_view0": cnt_err_bf16,
+        "counts_rel_err_bf16_tables_vs_fp64_decoded_view0": cnt_err_bf16_kernel,
         "mlem30_rel_err_vs_fp64": study,
         "mlem_sample": "36 of 720 views of the slice; FP64 oracle MLEM on the FP32 matrix",
     }

@@ -180,6 +180,54 @@ int xrd_forward_slice(xrd_context *ctx, const xrd_geometry *g, const xrd_discret
                       double lut_a_min, double lut_a_max, float *out_counts, float *out_compton,
                       float *out_A, void *stream);
 
+/* ---- projector launch shape ----------------------------------------------------
+ * No reference counterpart: the reference's launch shape is fixed
+ * (Forward_XRD.cu:244-246, one block per (view, pixel, energy), 256 threads).
+ * Later forward calls on ctx (xrd_forward, xrd_forward_slice, xrd_forward_bf16) use
+ * voxels_per_chunk voxels per chunk of the accumulation kernel's per-(voxel, energy)
+ * table (1..32; default 16; the chunk's FP32 partials add up at most that many
+ * terms, so the last bits of the results depend on it) and at most views_per_batch
+ * views per batch of (view, pixel, voxel) records (default: as many as fit 4 GiB).
+ * 0 restores a default.  Host-only. */
+int xrd_forward_set_launch(xrd_context *ctx, int32_t voxels_per_chunk, int32_t views_per_batch);
+/* Autotuner: runs xrd_forward (with A) on the first sample_views views of the scan
+ * for voxels_per_chunk in {8, 12, 16} and views_per_batch in {2, 8, 32}, timing each
+ * with HIP events on `stream`, keeps the fastest on ctx and reports it with its time
+ * in ms.  Allocates and frees scratch outputs; synchronises `stream`. */
+int xrd_forward_autotune(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                         const float *spectrum, const float *ifine, const int32_t *segmentation,
+                         const float *a1, const float *a2, const float *s2_table,
+                         double lut_q_min, double lut_q_max, double lut_a_min, double lut_a_max,
+                         int32_t sample_views, void *stream, int32_t *out_voxels_per_chunk,
+                         int32_t *out_views_per_batch, float *out_ms);
+
+/* ---- BF16 tables --------------------------------------------------------------
+ * The reference reads its form factors and its (a1,a2) attenuation texture in FP32
+ * (integrate_diffraction.cu:50-53, Forward_XRD.cu:216-237).  xrd_forward_bf16 takes
+ * them as bfloat16 (the upper 16 bits of an IEEE binary32, rounded to nearest even)
+ * with one FP32 scale per material, decoded on load (value = bf16 * scale, an FP32
+ * product) and then used exactly as the FP32 tables of xrd_forward_slice.  A map scale
+ * is picked by the voxel's own segmentation id, so a material of constant (a1,a2)
+ * decodes exactly when its scale is that value.  All pointers are device pointers,
+ * 16-byte aligned; the struct itself is read on the host during the call. */
+typedef struct xrd_bf16_tables {
+    const uint16_t *ifine;      /* [NQF x MAT]: Ifine(qf,mat) = bf16(ifine[qf + NQF*mat]) * ifine_scale[mat] */
+    const float *ifine_scale;   /* [MAT]                                                     */
+    const uint16_t *a1, *a2;    /* [nx x ny x n_slices] (x fastest), 1/voxel before the scale */
+    const float *mu_scale;      /* [2 x (MAT+1)]: a1(i) = bf16(a1[i]) * mu_scale[id(i)],
+                                   a2(i) = bf16(a2[i]) * mu_scale[MAT+1 + id(i)], id(i) the
+                                   segmentation id of voxel i (ids outside [0,MAT]: 0, air) */
+} xrd_bf16_tables;
+
+/* xrd_forward_slice (n_slices = 1, slice = 0: xrd_forward) with BF16 tables; the
+ * segmentation covers the same voxels as the maps.  Outputs and limits as there. */
+int xrd_forward_bf16(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
+                     const float *spectrum, const xrd_bf16_tables *tables,
+                     const int32_t *segmentation, int32_t n_slices, int32_t slice,
+                     const float *s2_table, double lut_q_min, double lut_q_max,
+                     double lut_a_min, double lut_a_max, float *out_counts, float *out_compton,
+                     float *out_A, void *stream);
+
 /* ---- ray traversal (the GetIntersection + attenuator sampling, on its own) ----
  * For `n` segments (x1,y1)->(x2,y2) given in `seg` as 4 doubles each, computes
  * in FP64 the entry point into [0,nx]x[0,ny] with GetIntersection's rules

@@ -86,6 +86,9 @@ def test_error_paths_without_device(L):
                                   ctypes.byref(ctypes.c_void_p())) == -2
     assert L.xrd_mlem_plan_iterate(None, 1, 0, None) == -2
     assert L.xrd_mlem_iterate_static(None, None, 0, 10, 8, None, None, None, None, 1, None) == -2
+    assert L.xrd_forward_set_launch(None, 16, 0) == -2
+    assert L.xrd_forward_bf16(None, ctypes.byref(g), ctypes.byref(d), None, None, None, 1, 0,
+                              None, 0.0, 1.0, 0.0, 1.0, None, None, None, None) == -2
     assert L.xrd_trace_rays(None, 4, 4, 0.25, 1, None, None, None, None, 0, None, 0, None,
                             None, None) == -2
     assert L.xrd_version() >= 100

@@ -404,22 +404,43 @@ def test_paper_fixture_rebuilds_from_the_reference(tmp_path):
 
 
 # ---------------------------------------------------------------------------
-# BF16 tables (with_bf16_tables)
+# BF16 tables (encode_bf16_tables / decode_bf16_tables, the xrd_bf16_tables layout)
 
 def test_bf16_round_matches_torch_and_bounds_tables():
-    """bf16_round is round-to-nearest-even to bfloat16 (torch's conversion, including
-    ties and negative values); with_bf16_tables moves each form-factor entry by at most
-    2^-8 of itself and leaves per-material-constant maps unchanged."""
+    """bf16_round and bf16_bits are round-to-nearest-even to bfloat16 (torch's
+    conversion, including ties and negative values); the encoded tables decode to
+    within 2^-8 of each form-factor entry, per-material-constant maps decode exactly,
+    and a map that varies inside a material moves by at most 2^-8 per voxel."""
     import torch
-    from xrd_tomography_amd.synthetic import bf16_round, with_bf16_tables
+    from xrd_tomography_amd.synthetic import (bf16_bits, bf16_float, bf16_round,
+                                              decode_bf16_tables, decoded_bf16_tables,
+                                              encode_bf16_tables, make_volume_problem)
     rng = np.random.default_rng(3)
     x = np.concatenate([rng.normal(size=20000) * 10.0 ** rng.integers(-20, 20, 20000),
                         [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 0.0]])
     x = x.astype(np.float32)
-    assert np.array_equal(bf16_round(x), torch.from_numpy(x).to(torch.bfloat16).float().numpy())
+    ref = torch.from_numpy(x).to(torch.bfloat16)
+    assert np.array_equal(bf16_round(x), ref.float().numpy())
+    assert np.array_equal(bf16_bits(x), ref.view(torch.int16).numpy().view(np.uint16))
+    assert np.array_equal(bf16_float(bf16_bits(x)), bf16_round(x))
     p = make_problem("small")
-    q = with_bf16_tables(p)
+    t = encode_bf16_tables(p)
+    M = p.disc.n_mat
+    assert t.ifine.dtype == np.uint16 and t.ifine.size == p.disc.n_qf * M
+    assert t.mu_scale.shape == (2 * (M + 1),) and t.ifine_scale.shape == (M,)
+    q = decode_bf16_tables(p, t)
     rel = np.abs(q.ifine.astype(np.float64) - p.ifine) / np.abs(p.ifine)
     assert rel.max() <= 2.0 ** -8 * (1 + 1e-6) and rel.max() > 2.0 ** -12
     assert np.array_equal(q.a1, p.a1) and np.array_equal(q.a2, p.a2)
     assert np.array_equal(q.spectrum, p.spectrum) and np.array_equal(q.s2_table, p.s2_table)
+    # decoding is the FP32 product bf16 * scale, element by element
+    sc = t.mu_scale[:M + 1][p.flat("segmentation")]
+    assert np.array_equal(q.flat("a1"), (bf16_float(t.a1) * sc).astype(np.float32))
+    # maps that vary inside a material, and a volume
+    v = make_volume_problem("small", n_slices=3)
+    v.a1 = (v.a1 * rng.uniform(0.7, 1.3, v.a1.shape)).astype(np.float32)
+    w = decoded_bf16_tables(v)
+    assert w.a1.shape == v.a1.shape and not np.array_equal(w.a1, v.a1)
+    live = v.a1 != 0
+    assert np.max(np.abs(w.a1[live] - v.a1[live]) / v.a1[live]) <= 2.0 ** -8 * (1 + 1e-6)
+    assert np.array_equal(w.a1[~live], v.a1[~live]) and np.array_equal(w.a2, v.a2)

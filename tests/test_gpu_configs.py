@@ -10,7 +10,8 @@
   oracle; dot-product test <J x, u> = <x, J' u> <= 1e-6 on the 256x256 geometry at
   12 of the 360 views.
 * vol192 (slice 16 of 192x192x32 voxels, 720 views, 256 pixels x 61 energies, 10
-  materials): N, C and A rows of 2 views x 8 columns against the oracle.
+  materials): N, C and A rows of 2 views x 8 columns against the oracle, with FP32
+  tables and with BF16 tables decoded by the kernels.
 * batch48 (48 phantoms of 64^2..256^2 voxels, 90..360 views, 2..10 materials, BF24
   storage): every batched x equals its own single-problem plan's x, and the 4
   smallest match the FP64 oracle MLEM on the stored matrix (2e-5).
@@ -187,19 +188,25 @@ VOL_VIEWS = [0, 359]
 VOL_COLS = [0, 40, 97, 128, 129, 170, 211, 255]
 
 
-@pytest.mark.parametrize("tables", ["f32"])
+@pytest.mark.parametrize("tables", ["f32", "bf16"])
 def test_vol192_slice_vs_oracle(xrd, oracle, tables):
     """Slice 16 of the 192x192x32 volume with all 720 views on the GPU; 2 views x 8
-    columns of N, C and A against the oracle."""
+    columns of N, C and A against the oracle.  With BF16 tables the form factors and
+    the attenuation maps are stored as bfloat16 with a per-material FP32 scale and
+    decoded by the kernels (xrd_forward_bf16); the oracle runs on the decoded values."""
     import torch
-    from xrd_tomography_amd.synthetic import make_volume_problem
+    from xrd_tomography_amd.synthetic import decoded_bf16_tables, make_volume_problem
     nz, sl = 32, 16
     vp = make_volume_problem("fan256", n_slices=nz, seed=7, nx=192, voxel_mm=2.0, n_views=720,
                              n_mat=10, mlem_iterations=30)
     d = vp.disc
     print(f"\n[vol192 {tables}] building slice {sl} on the GPU", flush=True)
-    N, C, A = xrd.forward_volume_slice(vp, nz, sl)
-    ref = vp
+    if tables == "bf16":
+        N, C, A = xrd.forward_problem_bf16(vp, nz, sl)
+        ref = decoded_bf16_tables(vp)
+    else:
+        N, C, A = xrd.forward_volume_slice(vp, nz, sl)
+        ref = vp
     torch.cuda.synchronize()
     Ng = _select(vp, N, VOL_VIEWS, VOL_COLS)
     Cg = _select(vp, C, VOL_VIEWS, VOL_COLS)

@@ -116,30 +116,59 @@ def test_forward_paper100(xrd, oracle, flags):
 
 
 def test_forward_bf16_tables(xrd, oracle):
-    """Form-factor and attenuation tables stored as BF16 with a per-material FP32 scale
-    (with_bf16_tables).  The GPU matches the FP64 oracle on the same rounded tables to
-    the bars above.  A does not depend on the form factors and the per-material maps
-    are unchanged, so A is bit-identical to the FP32-table A.  Each form-factor entry
-    moves by at most 2^-8 of itself and N is a positive combination of the entries, so
-    N moves by at most 2^-8 from the FP32-table counts."""
+    """xrd_forward_bf16: form factors and attenuation maps stored as bfloat16 with a
+    per-material FP32 scale, decoded by the kernels.  The GPU matches the FP64 oracle
+    on the decoded tables to the bars above.  A does not depend on the form factors
+    and per-material constant maps decode exactly, so A is bit-identical to the
+    FP32-table A; each form-factor entry moves by at most 2^-8 of itself and N is a
+    positive combination of the entries, so N moves by at most 2^-8 from the
+    FP32-table counts."""
     import torch
-    from xrd_tomography_amd.synthetic import make_problem, with_bf16_tables
+    from xrd_tomography_amd.synthetic import decoded_bf16_tables, make_problem
     p = make_problem("small")
-    q = with_bf16_tables(p)
+    q = decoded_bf16_tables(p)
     assert np.array_equal(q.a1, p.a1) and np.array_equal(q.a2, p.a2)
     assert not np.array_equal(q.ifine, p.ifine)
-    Ng, Cg, Ag = _run(xrd, q)
+    Nb, Cb, Ab = xrd.forward_problem_bf16(p)
+    Ng, Cg, Ag = _host(p, Nb, Cb, Ab)
     No, Co, Ao = oracle.forward(q, nthreads=4)
     _compare("small, BF16 tables", Ng, Cg, Ag, No, Co, Ao)
     _, _, Ap = xrd.forward_problem(p)
-    _, _, Aq = xrd.forward_problem(q)
     torch.cuda.synchronize()
-    assert torch.equal(Ap, Aq)
+    assert torch.equal(Ap, Ab)
     Np, _, _ = oracle.forward(p, with_A=False, nthreads=4)
     live = Np > 1e-9 * Np.max()
     rel = float(np.max(np.abs(np.asarray(Ng, np.float64)[live] - Np[live]) / Np[live]))
     print(f"[BF16 tables] counts vs FP64 on the FP32 tables: {rel:.2e} (bound 2^-8 = 3.9e-3)")
     assert rel <= 2.0 ** -8 + RTOL_COUNTS
+
+
+def test_forward_bf16_tables_varying_maps_and_volume(xrd, oracle):
+    """Maps that vary inside each material (so their BF16 rounding reaches A), in 2-D
+    and as slice 2 of a 4-slice volume: GPU against the oracle on the decoded tables;
+    the 2-D BF16 path with the FP32 tables' exact values is xrd_forward itself."""
+    import torch
+    from xrd_tomography_amd.synthetic import (decoded_bf16_tables, make_problem,
+                                              make_volume_problem)
+    rng = np.random.default_rng(12)
+    p = make_problem("small")
+    p.a1 = (p.a1 * rng.uniform(0.6, 1.4, p.a1.shape)).astype(np.float32)
+    p.a2 = (p.a2 * rng.uniform(0.6, 1.4, p.a2.shape)).astype(np.float32)
+    q = decoded_bf16_tables(p)
+    assert not np.array_equal(q.a1, p.a1)
+    Ng, Cg, Ag = _host(p, *xrd.forward_problem_bf16(p))
+    No, Co, Ao = oracle.forward(q, nthreads=4)
+    _compare("small, BF16 tables, varying maps", Ng, Cg, Ag, No, Co, Ao)
+    # the FP32 path on the decoded tables computes with the same values: same bits
+    N2, C2, A2 = _host(q, *xrd.forward_problem(q))
+    assert np.array_equal(Ng, N2) and np.array_equal(Cg, C2) and np.array_equal(Ag, A2)
+    vp = make_volume_problem("small", n_slices=4)
+    vp.a2 = (vp.a2 * rng.uniform(0.6, 1.4, vp.a2.shape)).astype(np.float32)
+    vq = decoded_bf16_tables(vp)
+    Ng, Cg, Ag = _host(vp, *xrd.forward_problem_bf16(vp, 4, 2))
+    No, Co, Ao = oracle.forward_slice(vq, 4, 2, nthreads=4)
+    _compare("volume slice 2/4, BF16 tables", Ng, Cg, Ag, No, Co, Ao)
+    torch.cuda.synchronize()
 
 
 def _host(prob, N, C, A):
@@ -260,3 +289,36 @@ def test_line_integrals_vs_oracle(xrd, oracle):
     rel = np.abs(ints - ref) / np.maximum(np.abs(ref), 1e-30)
     print(f"\n[line integrals] max rel err vs FP64 {rel.max():.2e}")
     assert rel.max() <= 2e-6
+
+
+def test_forward_launch_shapes_and_autotune_cache(xrd, oracle, tmp_path):
+    """Every projector launch shape the autotuner may pick (voxel chunk, view batch)
+    meets the forward bars against the oracle; the tuned shape is kept on the context
+    and a cache file gives it back without re-timing."""
+    import torch
+    from xrd_tomography_amd.synthetic import make_problem
+    prob = make_problem("reference", flags=3, n_views=8)
+    No, Co, Ao = oracle.forward(prob, nthreads=4)
+    try:
+        for kc, vpb in ((8, 2), (12, 8), (16, 32), (32, 1), (5, 3)):
+            xrd.set_forward_launch(kc, vpb)
+            Ng, Cg, Ag = _host(prob, *xrd.forward_problem(prob))
+            _compare(f"reference, chunk {kc}, view batch {vpb}", Ng, Cg, Ag, No, Co, Ao)
+        with pytest.raises(xrd.XrdError):
+            xrd.set_forward_launch(33, 0)
+        cache = str(tmp_path / "tune.json")
+        r1 = xrd.autotune_forward(prob, sample_views=8, cache=cache)
+        assert r1["cached"] is False and r1["voxels_per_chunk"] in (8, 12, 16)
+        assert r1["views_per_batch"] in (2, 8, 32)
+        r2 = xrd.autotune_forward(prob, sample_views=8, cache=cache)
+        assert r2["cached"] is True
+        assert (r2["voxels_per_chunk"], r2["views_per_batch"]) == (r1["voxels_per_chunk"],
+                                                                  r1["views_per_batch"])
+        a = xrd.forward_problem(prob)
+        b = xrd.forward_problem(prob)
+        torch.cuda.synchronize()
+        for u, v in zip(a, b):
+            assert torch.equal(u, v)
+        print(f"\n[autotune] {r1}")
+    finally:
+        xrd.set_forward_launch(0, 0)

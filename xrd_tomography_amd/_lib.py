@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .descriptors import Discretization, Geometry
+from .descriptors import Bf16TablesStruct, Discretization, Geometry
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libxrd_tomo.so")
 
@@ -34,7 +34,8 @@ EXPORTS = (
     "xrd_mlem_batch_create", "xrd_mlem_batch_iterate", "xrd_mlem_batch_set_launch",
     "xrd_mlem_batch_destroy", "xrd_mlem_plan_time_pass", "xrd_mlem_batch_autotune",
     "xrd_mlem_batch_time_pass", "xrd_mlem_batch_groups", "xrd_forward_ct", "xrd_vjp_ct",
-    "xrd_forward_slice", "xrd_mlem_iterate_static",
+    "xrd_forward_slice", "xrd_mlem_iterate_static", "xrd_forward_bf16",
+    "xrd_forward_set_launch", "xrd_forward_autotune",
 )
 MLEM_STATIC_SCRATCH_BYTES = 4259840  # XRD_MLEM_STATIC_SCRATCH_BYTES
 
@@ -66,6 +67,14 @@ def _declare(L):
                                        _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp]),
         "xrd_forward_slice": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
                                              _vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp]),
+        "xrd_forward_bf16": (ctypes.c_int, [_vp, G, D, _vp, ctypes.POINTER(Bf16TablesStruct),
+                                            _vp, _i32, _i32, _vp, _dbl, _dbl, _dbl, _dbl, _vp,
+                                            _vp, _vp, _vp]),
+        "xrd_forward_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
+        "xrd_forward_autotune": (ctypes.c_int, [_vp, G, D, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
+                                                _dbl, _dbl, _dbl, _i32, _vp,
+                                                ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                                ctypes.POINTER(ctypes.c_float)]),
         "xrd_trace_rays": (ctypes.c_int, [_vp, _i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp,
                                           _i32, _vp, _i32, _vp, _vp, _vp]),
         "xrd_project": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i64, _i32, _vp, _vp, _vp]),

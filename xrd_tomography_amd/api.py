@@ -22,8 +22,8 @@ import torch  # loaded before libxrd_tomo.so so both share one HIP runtime
 
 from . import _lib
 from ._lib import (DTYPE_BF16, DTYPE_BF24, DTYPE_F16, DTYPE_F32, MLEM_FUSED, MLEM_UNFUSED,
-                   check, lib)
-from .descriptors import Discretization, Geometry
+                   XrdError, check, lib)
+from .descriptors import Bf16TablesStruct, Discretization, Geometry
 
 _DTYPES = {torch.float32: DTYPE_F32, torch.bfloat16: DTYPE_BF16, torch.float16: DTYPE_F16}
 
@@ -213,6 +213,119 @@ def forward_slice(spectrum, Ifine, segmentation, a1, a2, S2_grid, q_min, q_max, 
     return N, C, A
 
 
+def Forward_XRD_bf16(spectrum, Ifine_bf16, Ifine_scale, segmentation, a1_bf16, a2_bf16,
+                     mu_scale, S2_grid, q_min, q_max, a_min, a_max, n_slices: int = 1,
+                     slice_: int = 0, *, geometry: Geometry, disc: Discretization,
+                     with_A: bool = True, ctx: Optional[Context] = None, stream=None):
+    """xrd_forward_bf16: Forward_XRD (n_slices = 1) or forward_slice with the form
+    factors and the attenuation maps stored as bfloat16 (torch.bfloat16 tensors) with
+    per-material FP32 scales, decoded on load by the kernels.  Ifine_scale is [MAT],
+    mu_scale [2*(MAT+1)] (a1 scales by segmentation id, then a2's)."""
+    g, d = geometry, disc
+    if not 0 <= int(slice_) < int(n_slices):
+        raise ValueError(f"slice {slice_} outside [0, {n_slices})")
+    npix = g.nx * g.ny * int(n_slices)
+    _check(spectrum, "spectrum", torch.float32, d.n_energy * d.n_energy)
+    _check(Ifine_bf16, "Ifine_bf16", torch.bfloat16, d.n_qf * d.n_mat)
+    _check(Ifine_scale, "Ifine_scale", torch.float32, d.n_mat)
+    _check(segmentation, "segmentation", torch.int32, npix)
+    _check(a1_bf16, "a1_bf16", torch.bfloat16, npix)
+    _check(a2_bf16, "a2_bf16", torch.bfloat16, npix)
+    _check(mu_scale, "mu_scale", torch.float32, 2 * (d.n_mat + 1))
+    _check(S2_grid, "S2_grid", torch.float32, d.lut_nq * d.lut_na)
+    _require_gpu(spectrum, Ifine_bf16, Ifine_scale, segmentation, a1_bf16, a2_bf16, mu_scale,
+                 S2_grid)
+    ctx = ctx or default_context(spectrum.device.index or 0)
+    shape_n = (d.n_energy, g.n_rows, g.n_cols, g.n_views)
+    dev = spectrum.device
+    N = torch.empty(shape_n, dtype=torch.float32, device=dev)
+    C = torch.empty(shape_n, dtype=torch.float32, device=dev)
+    A = torch.empty(shape_n + (d.n_mat, d.n_q), dtype=torch.float32, device=dev) if with_A else None
+    tab = Bf16TablesStruct(Ifine_bf16.data_ptr(), Ifine_scale.data_ptr(), a1_bf16.data_ptr(),
+                           a2_bf16.data_ptr(), mu_scale.data_ptr())
+    rc = lib().xrd_forward_bf16(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(spectrum),
+                                ctypes.byref(tab), _ptr(segmentation), int(n_slices), int(slice_),
+                                _ptr(S2_grid), float(q_min), float(q_max), float(a_min),
+                                float(a_max), _ptr(N), _ptr(C), _ptr(A), _stream(stream))
+    check(rc, "xrd_forward_bf16")
+    return N, C, A
+
+
+def forward_problem_bf16(prob, n_slices: int = 1, slice_: int = 0, device="cuda",
+                         with_A=True, ctx=None):
+    """Forward_XRD_bf16 on a synthetic.Problem (2-D, or a volume with n_slices) whose
+    tables are encoded by synthetic.encode_bf16_tables."""
+    from .synthetic import encode_bf16_tables
+    t = encode_bf16_tables(prob)
+    f32 = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dt)  # noqa: E731
+    b16 = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).to(device).view(torch.bfloat16)  # noqa: E731
+    return Forward_XRD_bf16(f32(prob.flat("spectrum")), b16(t.ifine), f32(t.ifine_scale),
+                            f32(prob.flat("segmentation"), torch.int32), b16(t.a1), b16(t.a2),
+                            f32(t.mu_scale), f32(prob.flat("s2_table")), *prob.lut, n_slices,
+                            slice_, geometry=prob.geometry, disc=prob.disc, with_A=with_A, ctx=ctx)
+
+
+def _cache_table(cache: Optional[str]) -> dict:
+    import json
+    import os
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            return json.load(f)
+    return {}
+
+
+def _cache_store(cache: Optional[str], key: str, value: dict):
+    import json
+    import os
+    if not cache:
+        return
+    table = _cache_table(cache)
+    table[key] = value
+    os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+    with open(cache, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def set_forward_launch(voxels_per_chunk: int = 0, views_per_batch: int = 0, ctx=None):
+    """xrd_forward_set_launch on the (default) context: the projector's voxel chunk and
+    view batch (0 = default) for the forward calls that follow."""
+    ctx = ctx or default_context(0)
+    check(lib().xrd_forward_set_launch(ctx.handle, int(voxels_per_chunk), int(views_per_batch)),
+          "xrd_forward_set_launch")
+
+
+def autotune_forward(prob, sample_views: int = 32, cache: Optional[str] = None, device="cuda",
+                     ctx=None) -> dict:
+    """Pick the projector's launch shape for a synthetic.Problem (xrd_forward_autotune on
+    its first sample_views views) and keep it on the context for later forward calls.
+    With `cache` (a JSON file) an earlier result for the same device and problem shape
+    is reused instead of re-timing, and new results are added to it."""
+    g, d = prob.geometry, prob.disc
+    ctx = ctx or default_context(0)
+    nvox = int(((prob.segmentation > 0) & (prob.segmentation <= d.n_mat)).sum())
+    key = (f"forward|{torch.cuda.get_device_name(ctx.device)}|{g.nx}x{g.ny}|cols {g.n_cols}"
+           f"|rows {g.n_rows}|E {d.n_energy}|Q {d.n_q}|QF {d.n_qf}|M {d.n_mat}|vox {nvox}")
+    hit = _cache_table(cache).get(key)
+    if hit is not None:
+        set_forward_launch(hit["voxels_per_chunk"], hit["views_per_batch"], ctx=ctx)
+        return dict(hit, cached=True)
+
+    def dev(name, dt):
+        return torch.from_numpy(prob.flat(name)).to(device=device, dtype=dt)
+    sp, fi, sg = dev("spectrum", torch.float32), dev("ifine", torch.float32), dev("segmentation", torch.int32)
+    a1, a2, s2 = dev("a1", torch.float32), dev("a2", torch.float32), dev("s2_table", torch.float32)
+    kc, vpb, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+    check(lib().xrd_forward_autotune(ctx.handle, ctypes.byref(g), ctypes.byref(d), _ptr(sp),
+                                     _ptr(fi), _ptr(sg), _ptr(a1), _ptr(a2), _ptr(s2),
+                                     *[float(v) for v in prob.lut], int(sample_views),
+                                     _stream(None), ctypes.byref(kc), ctypes.byref(vpb),
+                                     ctypes.byref(ms)), "xrd_forward_autotune")
+    res = {"voxels_per_chunk": kc.value, "views_per_batch": vpb.value,
+           "ms_sample": round(ms.value, 3), "sample_views": min(int(sample_views), g.n_views)}
+    _cache_store(cache, key, res)
+    return dict(res, cached=False)
+
+
 def forward_volume_slice(vp, n_slices: int, slice_: int, device="cuda", with_A=True, ctx=None):
     """forward_slice on a volume Problem (segmentation, a1, a2 [nx, ny, n_slices])."""
     def dev(name, dt):
@@ -370,28 +483,20 @@ class MlemPlan:
         """Pick the launch shape of the fused pass (xrd_mlem_plan_autotune).  With
         `cache` (a JSON file path) an earlier result for the same matrix shape, dtype and
         device is reused instead of re-timing."""
-        import json
-        import os
-        key = f"{torch.cuda.get_device_name(self.x.device)}|{self._keep[0].shape[0]}x" \
+        key = f"mlem|{torch.cuda.get_device_name(self.x.device)}|{self._keep[0].shape[0]}x" \
               f"{self.n_cols}|{self._keep[0].dtype}"
-        table = {}
-        if cache and os.path.exists(cache):
-            with open(cache) as f:
-                table = json.load(f)
-        if key in table:
-            b, r = table[key]["blocks_per_cu"], table[key]["rows_per_wave"]
+        hit = _cache_table(cache).get(key)
+        if hit is not None:
+            b, r = hit["blocks_per_cu"], hit["rows_per_wave"]
             check(lib().xrd_mlem_plan_set_launch(self.handle, b, r), "xrd_mlem_plan_set_launch")
-            return dict(table[key], cached=True)
+            return dict(hit, cached=True)
         b, r, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
         check(lib().xrd_mlem_plan_autotune(self.handle, int(reps), _stream(stream),
                                            ctypes.byref(b), ctypes.byref(r), ctypes.byref(ms)),
               "xrd_mlem_plan_autotune")
-        res = {"blocks_per_cu": b.value, "rows_per_wave": r.value, "ms_fused_pass": ms.value}
-        if cache:
-            table[key] = res
-            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
-            with open(cache, "w") as f:
-                json.dump(table, f, indent=1)
+        res = {"blocks_per_cu": b.value, "rows_per_wave": r.value,
+               "ms_fused_pass": round(ms.value, 4)}
+        _cache_store(cache, key, res)
         return dict(res, cached=False)
 
     def set_launch(self, blocks_per_cu: int = 0, rows_per_wave: int = 0):

@@ -42,8 +42,8 @@ __global__ void prepare_maps_kernel(PrepArgs a) {
         float2 v = make_float2(0.f, 0.f);
         if (px >= 1 && px <= a.nx && py >= 1 && py <= a.ny) {
             const int pix = (px - 1) + a.nx * (py - 1);
-            v.x = a.a1[pix];
-            v.y = a.a2[pix];
+            v.x = a.a1.at(pix);
+            v.y = a.a2.at(pix);
         }
         a.mu_padded[i] = v;
     }
@@ -105,11 +105,11 @@ __global__ __launch_bounds__(1024) void prepare_list_kernel(PrepArgs a) {
         const int pix = a.ref_list[j];
         if (stale) {
             const int prev = j > 0 ? a.ref_list[j - 1] : -1;
-            a.cpx1[pix] = prev >= 0 ? a.a1[prev] : 0.f;
-            a.cpx2[pix] = prev >= 0 ? a.a2[prev] : 0.f;
+            a.cpx1[pix] = prev >= 0 ? a.a1.at(prev) : 0.f;
+            a.cpx2[pix] = prev >= 0 ? a.a2.at(prev) : 0.f;
         } else {
-            a.cpx1[pix] = a.a1[pix];
-            a.cpx2[pix] = a.a2[pix];
+            a.cpx1[pix] = a.a1.at(pix);
+            a.cpx2[pix] = a.a2.at(pix);
         }
     }
     __syncthreads();
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(1024) void prepare_list_kernel(PrepArgs a) {
     if (threadIdx.x == 0) a.mat_offsets[a.n_mat] = off;
 }
 
-__global__ void prepare_volume_kernel(const float *a1, const float *a2, int nx, int ny, int nz,
+__global__ void prepare_volume_kernel(MapSrc a1, MapSrc a2, int nx, int ny, int nz,
                                       float2 *mu_vol) {
     const int pw = nx + 2, ph = ny + 2;
     const size_t n = (size_t)pw * ph * (nz + 2);
@@ -151,14 +151,14 @@ __global__ void prepare_volume_kernel(const float *a1, const float *a2, int nx, 
         float2 v = make_float2(0.f, 0.f);
         if (px >= 1 && px <= nx && py >= 1 && py <= ny && pz >= 1 && pz <= nz) {
             const size_t vox = (size_t)(px - 1) + (size_t)nx * ((py - 1) + (size_t)ny * (pz - 1));
-            v.x = a1[vox];
-            v.y = a2[vox];
+            v.x = a1.at(vox);
+            v.y = a2.at(vox);
         }
         mu_vol[i] = v;
     }
 }
 
-hipError_t launch_prepare_volume(const float *a1, const float *a2, int nx, int ny, int nz,
+hipError_t launch_prepare_volume(const MapSrc &a1, const MapSrc &a2, int nx, int ny, int nz,
                                  float2 *mu_vol, hipStream_t s) {
     hipLaunchKernelGGL(prepare_volume_kernel, dim3(4096), dim3(256), 0, s, a1, a2, nx, ny, nz,
                        mu_vol);
@@ -532,7 +532,11 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
         for (int s = 0; s < FS; s++) {
             const int n = tq * FS + s;
-            wf[s] = (n < QF) ? (float)(a.dq_fine * (double)a.ifine[n + (size_t)QF * m]) : 0.f;
+            float iv = 0.f;
+            if (n < QF)
+                iv = a.ifine ? a.ifine[n + (size_t)QF * m]
+                             : bf16_bits_to_float(a.ifine_b16[n + (size_t)QF * m]) * a.ifine_scale[m];
+            wf[s] = (n < QF) ? (float)(a.dq_fine * (double)iv) : 0.f;
         }
         for (int k0 = lo; k0 < hi; k0 += kc) {
             const int nk = min(kc, hi - k0);
@@ -756,7 +760,7 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
 template <int TQ, int G, int EPT, int FS, bool WITH_A>
 static hipError_t launch_fwd_t(const ForwardArgs &a, hipStream_t s) {
     constexpr int NT = G * TQ;
-    const int kc = 16;
+    const int kc = a.kc > 0 ? a.kc : 16;
     const int E = a.n_energy;
     const int Gc = NT / E;
     const size_t vox_bytes = ((kc * sizeof(VoxShared) + 15) & ~size_t(15));

@@ -118,6 +118,8 @@ struct xrd_context {
     int *pinned_word = nullptr;
     hipEvent_t staged = nullptr;
     bool staged_pending = false;
+    // projector launch shape (xrd_forward_set_launch / xrd_forward_autotune; 0 = default)
+    int fwd_kc = 0, fwd_vpb = 0;
 };
 
 struct xrd_mlem_plan {
@@ -410,24 +412,34 @@ int setup_forward_common(xrd_context *ctx, const xrd_geometry *g, const xrd_disc
 
 namespace {
 
-// xrd_forward for the slice `slice` of an n_slices-slice volume (segmentation, a1,
-// a2 point at the volumes); n_slices = 1 is the 2-D model.
+// The form-factor and attenuation tables of a forward call: FP32 (ifine, a1, a2) or
+// bfloat16 with per-material FP32 scales (the xrd_bf16_tables fields).
+struct TableArgs {
+    const float *ifine = nullptr, *a1 = nullptr, *a2 = nullptr;
+    const xrd_bf16_tables *b16 = nullptr;
+};
+
+// xrd_forward for the slice `slice` of an n_slices-slice volume (segmentation and the
+// maps point at the volumes); n_slices = 1 is the 2-D model.
 int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
-                 const float *spectrum, const float *ifine, const int32_t *seg_vol,
-                 const float *a1_vol, const float *a2_vol, int n_slices, int slice,
-                 const float *s2_table, double lut_q_min, double lut_q_max, double lut_a_min,
-                 double lut_a_max, float *out_counts, float *out_compton, float *out_A,
-                 void *stream, const char *fn) {
+                 const float *spectrum, const TableArgs &tab, const int32_t *seg_vol,
+                 int n_slices, int slice, const float *s2_table, double lut_q_min,
+                 double lut_q_max, double lut_a_min, double lut_a_max, float *out_counts,
+                 float *out_compton, float *out_A, void *stream, const char *fn) {
     if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
     const int32_t *segmentation = seg_vol;
-    const float *a1 = a1_vol, *a2 = a2_vol;
     int rc = check_geometry(g, d, fn);
     if (rc) return rc;
-    if (!spectrum || !ifine || !segmentation || !a1 || !a2 || !s2_table || !out_counts ||
-        !out_compton)
+    const xrd_bf16_tables *bt = tab.b16;
+    const bool have_tables =
+        bt ? (bt->ifine && bt->ifine_scale && bt->a1 && bt->a2 && bt->mu_scale)
+           : (tab.ifine && tab.a1 && tab.a2);
+    if (!spectrum || !have_tables || !segmentation || !s2_table || !out_counts || !out_compton)
         return fail(XRD_ERR_NULL_POINTER, "%s: a required input/output pointer is NULL", fn);
-    const void *ptrs[] = {spectrum, ifine, segmentation, a1, a2, s2_table, out_counts,
-                          out_compton, out_A};
+    const void *ptrs[] = {spectrum, segmentation, s2_table, out_counts, out_compton, out_A,
+                          tab.ifine, tab.a1, tab.a2, bt ? bt->ifine : nullptr,
+                          bt ? bt->ifine_scale : nullptr, bt ? bt->a1 : nullptr,
+                          bt ? bt->a2 : nullptr, bt ? bt->mu_scale : nullptr};
     for (const void *p : ptrs)
         if (p && !aligned(p, 16)) return fail(XRD_ERR_ALIGN, "%s: pointers must be 16-byte aligned", fn);
     const double lims[] = {lut_q_min, lut_q_max, lut_a_min, lut_a_max};
@@ -451,15 +463,32 @@ int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizati
     const int nx = g->nx, ny = g->ny, V = g->n_views;
     const size_t npix = (size_t)nx * ny;
     const size_t npad = (size_t)(nx + 2) * (ny + 2);
-    segmentation += npix * slice;
-    a1 += npix * slice;
-    a2 += npix * slice;
+    // the maps as the preparation kernels load them: the whole volume, and the slice
+    MapSrc a1v, a2v;
+    if (bt) {
+        a1v.b16 = bt->a1;
+        a2v.b16 = bt->a2;
+        a1v.scale = bt->mu_scale;
+        a2v.scale = bt->mu_scale + (d->n_mat + 1);
+        a1v.seg = a2v.seg = seg_vol;
+        a1v.n_ids = a2v.n_ids = d->n_mat + 1;
+    } else {
+        a1v.f32 = tab.a1;
+        a2v.f32 = tab.a2;
+    }
+    MapSrc a1s = a1v, a2s = a2v;
+    const size_t off = npix * slice;
+    for (MapSrc *m : {&a1s, &a2s}) {
+        if (m->f32) m->f32 += off;
+        if (m->b16) m->b16 += off;
+        if (m->seg) m->seg += off;
+    }
+    segmentation += off;
     HIP_TRY(ctx->mu_padded.ensure(npad * sizeof(float2)), "workspace alloc");
     if (n_slices > 1) {
         HIP_TRY(ctx->mu_vol.ensure(npad * (size_t)(n_slices + 2) * sizeof(float2)),
                 "workspace alloc");
-        HIP_TRY(launch_prepare_volume(a1_vol, a2_vol, nx, ny, n_slices, (float2 *)ctx->mu_vol.ptr,
-                                      s),
+        HIP_TRY(launch_prepare_volume(a1v, a2v, nx, ny, n_slices, (float2 *)ctx->mu_vol.ptr, s),
                 "volume preparation");
     }
     HIP_TRY(ctx->s2u8.ensure((size_t)d->lut_nq * d->lut_na), "workspace alloc");
@@ -478,8 +507,8 @@ int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizati
     pa.n_mat = d->n_mat;
     pa.flags = d->flags;
     pa.seg = segmentation;
-    pa.a1 = a1;
-    pa.a2 = a2;
+    pa.a1 = a1s;
+    pa.a2 = a2s;
     pa.s2_table = s2_table;
     pa.s2_count = d->lut_nq * d->lut_na;
     pa.s2u8 = (uint8_t *)ctx->s2u8.ptr;
@@ -491,7 +520,9 @@ int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizati
     pa.mat_offsets = (int *)ctx->offsets.ptr;
     HIP_TRY(launch_prepare(pa, s), "prepare kernels");
 
-    fa.ifine = ifine;
+    fa.ifine = bt ? nullptr : tab.ifine;
+    fa.ifine_b16 = bt ? bt->ifine : nullptr;
+    fa.ifine_scale = bt ? bt->ifine_scale : nullptr;
     fa.lut_q_min = lut_q_min;
     fa.lut_a_min = lut_a_min;
     fa.dq_lut = (lut_q_max - lut_q_min) / (double)d->lut_nq;
@@ -520,6 +551,8 @@ int forward_impl(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizati
     const int nvox = *ctx->pinned_word;
     const size_t per_view = (size_t)fa.n_det * (size_t)(nvox > 0 ? nvox : 1) * kPixelVoxelRecordBytes;
     int nvb = (int)std::min<size_t>((size_t)V, std::max<size_t>(1, kPixelVoxelBudget / per_view));
+    if (ctx->fwd_vpb > 0) nvb = std::min(nvb, ctx->fwd_vpb);
+    fa.kc = ctx->fwd_kc;
     const size_t nrec = (size_t)nvb * fa.n_det * (size_t)(nvox > 0 ? nvox : 1);
     HIP_TRY(ctx->pv.ensure(nrec * kPixelVoxelRecordBytes), "workspace alloc");
     fa.nvox = nvox;
@@ -547,9 +580,13 @@ extern "C" int xrd_forward(xrd_context *ctx, const xrd_geometry *g, const xrd_di
                            const float *s2_table, double lut_q_min, double lut_q_max,
                            double lut_a_min, double lut_a_max, float *out_counts,
                            float *out_compton, float *out_A, void *stream) {
-    return forward_impl(ctx, g, d, spectrum, ifine, segmentation, a1, a2, 1, 0, s2_table,
-                        lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts, out_compton,
-                        out_A, stream, "xrd_forward");
+    TableArgs tab;
+    tab.ifine = ifine;
+    tab.a1 = a1;
+    tab.a2 = a2;
+    return forward_impl(ctx, g, d, spectrum, tab, segmentation, 1, 0, s2_table, lut_q_min,
+                        lut_q_max, lut_a_min, lut_a_max, out_counts, out_compton, out_A, stream,
+                        "xrd_forward");
 }
 
 extern "C" int xrd_forward_slice(xrd_context *ctx, const xrd_geometry *g,
@@ -559,9 +596,121 @@ extern "C" int xrd_forward_slice(xrd_context *ctx, const xrd_geometry *g,
                                  const float *s2_table, double lut_q_min, double lut_q_max,
                                  double lut_a_min, double lut_a_max, float *out_counts,
                                  float *out_compton, float *out_A, void *stream) {
-    return forward_impl(ctx, g, d, spectrum, ifine, segmentation, a1, a2, n_slices, slice,
-                        s2_table, lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts,
-                        out_compton, out_A, stream, "xrd_forward_slice");
+    TableArgs tab;
+    tab.ifine = ifine;
+    tab.a1 = a1;
+    tab.a2 = a2;
+    return forward_impl(ctx, g, d, spectrum, tab, segmentation, n_slices, slice, s2_table,
+                        lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts, out_compton,
+                        out_A, stream, "xrd_forward_slice");
+}
+
+extern "C" int xrd_forward_bf16(xrd_context *ctx, const xrd_geometry *g,
+                                const xrd_discretization *d, const float *spectrum,
+                                const xrd_bf16_tables *tables, const int32_t *segmentation,
+                                int32_t n_slices, int32_t slice, const float *s2_table,
+                                double lut_q_min, double lut_q_max, double lut_a_min,
+                                double lut_a_max, float *out_counts, float *out_compton,
+                                float *out_A, void *stream) {
+    if (!tables) return fail(XRD_ERR_NULL_POINTER, "xrd_forward_bf16: tables is NULL");
+    TableArgs tab;
+    tab.b16 = tables;
+    return forward_impl(ctx, g, d, spectrum, tab, segmentation, n_slices, slice, s2_table,
+                        lut_q_min, lut_q_max, lut_a_min, lut_a_max, out_counts, out_compton,
+                        out_A, stream, "xrd_forward_bf16");
+}
+
+extern "C" int xrd_forward_set_launch(xrd_context *ctx, int32_t voxels_per_chunk,
+                                      int32_t views_per_batch) {
+    const char *fn = "xrd_forward_set_launch";
+    if (!ctx) return fail(XRD_ERR_NULL_POINTER, "%s: context is NULL", fn);
+    if (voxels_per_chunk < 0 || voxels_per_chunk > kMaxVoxelsPerChunk || views_per_batch < 0)
+        return fail(XRD_ERR_INVALID_ARG,
+                    "%s: voxels_per_chunk in [0,%d], views_per_batch >= 0 (0 = default)", fn,
+                    kMaxVoxelsPerChunk);
+    ctx->fwd_kc = voxels_per_chunk;
+    ctx->fwd_vpb = views_per_batch;
+    return XRD_OK;
+}
+
+extern "C" int xrd_forward_autotune(xrd_context *ctx, const xrd_geometry *g,
+                                    const xrd_discretization *d, const float *spectrum,
+                                    const float *ifine, const int32_t *segmentation,
+                                    const float *a1, const float *a2, const float *s2_table,
+                                    double lut_q_min, double lut_q_max, double lut_a_min,
+                                    double lut_a_max, int32_t sample_views, void *stream,
+                                    int32_t *out_voxels_per_chunk, int32_t *out_views_per_batch,
+                                    float *out_ms) {
+    const char *fn = "xrd_forward_autotune";
+    if (!ctx || !g || !d) return fail(XRD_ERR_NULL_POINTER, "%s: NULL argument", fn);
+    if (sample_views <= 0) return fail(XRD_ERR_INVALID_ARG, "%s: sample_views must be > 0", fn);
+    int rc = check_geometry(g, d, fn);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    // the first sample_views views of the scan, into scratch outputs
+    xrd_geometry gs = *g;
+    gs.n_views = std::min(sample_views, g->n_views);
+    const size_t rows = (size_t)gs.n_views * g->n_cols * g->n_rows * d->n_energy;
+    DevBuf out;
+    HIP_TRY(out.ensure(rows * ((size_t)d->n_q * d->n_mat + 2) * sizeof(float)),
+            "autotune output alloc");
+    float *n_out = (float *)out.ptr, *c_out = n_out + rows, *a_out = c_out + rows;
+    TableArgs tab;
+    tab.ifine = ifine;
+    tab.a1 = a1;
+    tab.a2 = a2;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    const int kcs[] = {8, 12, 16};
+    const int vpbs[] = {2, 8, 32};
+    const int saved_kc = ctx->fwd_kc, saved_vpb = ctx->fwd_vpb;
+    float best = 1e30f;
+    int best_kc = 16, best_vpb = 0;
+    rc = XRD_OK;
+    hipError_t err = hipSuccess;
+    // one untimed call first: workspace allocation and code loading
+    rc = forward_impl(ctx, &gs, d, spectrum, tab, segmentation, 1, 0, s2_table, lut_q_min,
+                      lut_q_max, lut_a_min, lut_a_max, n_out, c_out, a_out, stream, fn);
+    for (int kc : kcs) {
+        for (int vpb : vpbs) {
+            if (rc != XRD_OK || err != hipSuccess) break;
+            if (vpb > gs.n_views && vpb != vpbs[0]) continue;
+            ctx->fwd_kc = kc;
+            ctx->fwd_vpb = vpb;
+            err = hipEventRecord(e0, s);
+            if (err != hipSuccess) break;
+            rc = forward_impl(ctx, &gs, d, spectrum, tab, segmentation, 1, 0, s2_table,
+                              lut_q_min, lut_q_max, lut_a_min, lut_a_max, n_out, c_out, a_out,
+                              stream, fn);
+            if (rc != XRD_OK) break;
+            err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err == hipSuccess && ms < best) {
+                best = ms;
+                best_kc = kc;
+                best_vpb = vpb;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    out.release();
+    if (rc != XRD_OK || err != hipSuccess) {
+        ctx->fwd_kc = saved_kc;
+        ctx->fwd_vpb = saved_vpb;
+        return rc != XRD_OK ? rc : hip_fail(err, fn);
+    }
+    ctx->fwd_kc = best_kc;
+    ctx->fwd_vpb = best_vpb;
+    if (out_voxels_per_chunk) *out_voxels_per_chunk = best_kc;
+    if (out_views_per_batch) *out_views_per_batch = best_vpb;
+    if (out_ms) *out_ms = best;
+    return XRD_OK;
 }
 
 extern "C" int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step,

@@ -16,6 +16,33 @@ struct VoxelEntry {
     float a1c, a2c;  // (a1,a2) used by the Compton term
 };
 
+// bfloat16 bits (the upper half of an IEEE binary32) as FP32, exactly.
+__host__ __device__ inline float bf16_bits_to_float(uint16_t b) {
+    union {
+        uint32_t u;
+        float f;
+    } v;
+    v.u = (uint32_t)b << 16;
+    return v.f;
+}
+
+// A per-voxel table (a1 or a2 map, one slice or a volume) as the preparation kernels
+// load it: FP32 values, or bfloat16 values times an FP32 scale chosen by the voxel's
+// segmentation id (ids outside [0, n_ids) use scale[0], the air scale).
+struct MapSrc {
+    const float *f32 = nullptr;
+    const uint16_t *b16 = nullptr;
+    const float *scale = nullptr;
+    const int32_t *seg = nullptr;  // segmentation of the same voxels as b16
+    int n_ids = 0;
+    __device__ __forceinline__ float at(size_t i) const {
+        if (f32) return f32[i];
+        int id = seg[i];
+        if (id < 0 || id >= n_ids) id = 0;
+        return bf16_bits_to_float(b16[i]) * scale[id];
+    }
+};
+
 // Per-energy-bin constants, host-computed in FP64.
 struct EnergyConst {
     double es1, es2, esm;  // source-bin edges and centre, keV
@@ -50,7 +77,9 @@ struct ForwardArgs {
     double dq_fine, lut_q_min, lut_a_min, dq_lut, da_lut;
     // inputs
     const float *spectrum;      // [E x E]
-    const float *ifine;         // [QF x M]
+    const float *ifine;         // [QF x M] FP32, or NULL when ifine_b16 is set
+    const uint16_t *ifine_b16;  // [QF x M] bfloat16, times ifine_scale[mat]
+    const float *ifine_scale;   // [M]
     const uint8_t *s2u8;        // [NQ x NA] quantised Compton table
     const float2 *mu_padded;    // [(nx+2) x (ny+2)] (a1,a2), zero border
     const VoxelEntry *voxels;   // material-major voxel list
@@ -73,13 +102,15 @@ struct ForwardArgs {
     const float2 *mu_vol;
     int nz;
     double z_slice;
+    // voxels per chunk of forward_kernel (its per-(voxel, energy) LDS table); 0 = 16
+    int kc;
 };
 
 struct PrepArgs {
     int nx, ny, n_mat;
     unsigned flags;
     const int32_t *seg;
-    const float *a1, *a2;
+    MapSrc a1, a2;          // the slice's maps [nx x ny], decoded on load
     const float *s2_table;
     int s2_count;
     uint8_t *s2u8;
@@ -92,10 +123,13 @@ struct PrepArgs {
 
 hipError_t launch_prepare(const PrepArgs &a, hipStream_t s);
 // (a1, a2) volume [nx x ny x nz] into the padded volume with a zero border.
-hipError_t launch_prepare_volume(const float *a1, const float *a2, int nx, int ny, int nz,
+hipError_t launch_prepare_volume(const MapSrc &a1, const MapSrc &a2, int nx, int ny, int nz,
                                  float2 *mu_vol, hipStream_t s);
 hipError_t launch_source_integrals(const ForwardArgs &a, float2 *src_int, hipStream_t s);
 hipError_t launch_pixel_voxel(const ForwardArgs &a, hipStream_t s);
+// Largest voxel chunk of forward_kernel: its ballots cover one wave of voxels, and
+// 32 voxels' tables still fit a workgroup's LDS.
+constexpr int kMaxVoxelsPerChunk = 32;
 // Bytes of one (view, pixel, voxel) record, and the workspace the records of one
 // batch of views may take.
 constexpr size_t kPixelVoxelRecordBytes = 2 * sizeof(double) + sizeof(float4) + sizeof(float2);

@@ -80,6 +80,12 @@ class Discretization(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Bf16TablesStruct(ctypes.Structure):
+    """ctypes mirror of xrd_bf16_tables (device pointers)."""
+    _fields_ = [("ifine", ctypes.c_void_p), ("ifine_scale", ctypes.c_void_p),
+                ("a1", ctypes.c_void_p), ("a2", ctypes.c_void_p), ("mu_scale", ctypes.c_void_p)]
+
+
 def num_rows(g: Geometry, d: Discretization) -> int:
     """Rows of the model matrix: SRC*COL*ROW*NRG (Forward_XRD.cu:143)."""
     return g.n_views * g.n_cols * g.n_rows * d.n_energy

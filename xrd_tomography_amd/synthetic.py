@@ -333,31 +333,81 @@ def bf16_round(x: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).reshape(np.shape(x))
 
 
-def with_bf16_tables(prob: Problem) -> Problem:
-    """A copy of prob whose form-factor table and attenuation maps are stored as
-    bfloat16 with a per-material FP32 scale s_m: ifine[:, m] = s_m bf16(ifine[:, m] /
-    s_m) with s_m the column's largest magnitude, and a1, a2 likewise over material m's
-    voxels.  BF16 keeps 8 significant bits, so round-to-nearest moves every table
-    entry by at most 2^-8 of itself; the projector then computes in FP32 with exactly
-    these values.  Maps that are constant per material (the phantoms here) are
-    unchanged."""
-    ifine = np.array(prob.ifine, dtype=np.float32)
-    for m in range(ifine.shape[1]):
-        sm = np.float32(np.abs(ifine[:, m]).max())
-        if sm > 0:
-            ifine[:, m] = bf16_round(ifine[:, m] / sm) * sm
-    maps = []
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """bfloat16 bits (uint16) of float32 values, rounded to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16).reshape(np.shape(x))
+
+
+def bf16_float(b: np.ndarray) -> np.ndarray:
+    """float32 values of bfloat16 bits (exact)."""
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+@dataclasses.dataclass
+class Bf16Tables:
+    """A problem's form factors and attenuation maps in BF16 storage with per-material
+    FP32 scales: the fields of xrd_bf16_tables, as host arrays in the ABI's order."""
+    ifine: np.ndarray        # uint16 [QF*M], q fastest
+    ifine_scale: np.ndarray  # float32 [M]
+    a1: np.ndarray           # uint16 [nx*ny(*n_slices)], x fastest
+    a2: np.ndarray
+    mu_scale: np.ndarray     # float32 [2*(M+1)]: a1 scales by segmentation id, then a2's
+
+
+def _ids(prob: Problem) -> np.ndarray:
+    """Segmentation id of every voxel in ABI order, ids outside [0, M] mapped to 0 (air)."""
+    seg = prob.flat("segmentation").astype(np.int64)
+    seg[(seg < 0) | (seg > prob.disc.n_mat)] = 0
+    return seg
+
+
+def encode_bf16_tables(prob: Problem) -> Bf16Tables:
+    """BF16 storage of prob's form-factor table and attenuation maps: each form-factor
+    column m and each map's voxels of segmentation id s are divided by a scale (the
+    column's or the group's largest magnitude, 1 if that is 0) and rounded to
+    bfloat16.  Round-to-nearest keeps 8 significant bits, so decoding moves every
+    entry by at most 2^-8 of itself; a material of constant (a1, a2) decodes exactly."""
+    M = prob.disc.n_mat
+    ifine = np.asarray(prob.flat("ifine"), np.float32).reshape(M, -1)  # [M, QF]
+    isc = np.array([np.abs(col).max() for col in ifine], np.float32)
+    isc[~(isc > 0)] = 1.0
+    ifine_b = bf16_bits(ifine / isc[:, None]).reshape(-1)
+    ids = _ids(prob)
+    maps, scales = [], []
     for name in ("a1", "a2"):
-        a = np.array(getattr(prob, name), dtype=np.float32)
-        for m in np.unique(prob.segmentation):
-            if m <= 0:
-                continue
-            sel = prob.segmentation == m
-            sm = np.float32(np.abs(a[sel]).max())
-            if sm > 0:
-                a[sel] = bf16_round(a[sel] / sm) * sm
-        maps.append(a)
-    return dataclasses.replace(prob, ifine=ifine, a1=maps[0], a2=maps[1])
+        a = np.asarray(prob.flat(name), np.float32)
+        sc = np.ones(M + 1, np.float32)
+        for i in range(M + 1):
+            sel = ids == i
+            if sel.any():
+                mx = np.float32(np.abs(a[sel]).max())
+                if mx > 0:
+                    sc[i] = mx
+        maps.append(bf16_bits(a / sc[ids]))
+        scales.append(sc)
+    return Bf16Tables(ifine=ifine_b, ifine_scale=isc, a1=maps[0], a2=maps[1],
+                      mu_scale=np.concatenate(scales).astype(np.float32))
+
+
+def decode_bf16_tables(prob: Problem, t: Bf16Tables) -> Problem:
+    """A copy of prob whose FP32 tables hold exactly the values the kernels decode from
+    t (bf16 times scale, an FP32 product): the FP32 problem that xrd_forward_bf16 on
+    t computes, for the oracle."""
+    M = prob.disc.n_mat
+    ifine = (bf16_float(t.ifine).reshape(M, -1) * t.ifine_scale[:, None]).astype(np.float32)
+    ids = _ids(prob)
+    shape = prob.segmentation.shape
+    a1 = (bf16_float(t.a1) * t.mu_scale[:M + 1][ids]).astype(np.float32)
+    a2 = (bf16_float(t.a2) * t.mu_scale[M + 1:][ids]).astype(np.float32)
+    unflat = lambda v: v.reshape(shape, order="F")  # noqa: E731
+    return dataclasses.replace(prob, ifine=np.ascontiguousarray(ifine.T),
+                               a1=unflat(a1), a2=unflat(a2))
+
+
+def decoded_bf16_tables(prob: Problem) -> Problem:
+    """decode_bf16_tables(prob, encode_bf16_tables(prob))."""
+    return decode_bf16_tables(prob, encode_bf16_tables(prob))
 
 
 def make_volume_problem(name: str = "small", n_slices: int = 4, seed: Optional[int] = None,
