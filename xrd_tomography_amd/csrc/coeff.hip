@@ -25,41 +25,6 @@ namespace xrd {
 
 namespace {
 
-template <int CTRL> __device__ __forceinline__ float dpp_get(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-
-// Sums 16 values per lane over the wave so that lane l ends up with the wave total
-// of value (l >> 2) & 15 (pairs of lanes differing in one bit exchange halves:
-// bits 5, 4 by shuffles, 3 by row_mirror, 2 by row_half_mirror; bits 1 and 0 add
-// the remaining value).
-__device__ __forceinline__ float transpose_sum16(const float *v, int lane) {
-    float w[8];
-    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float keep = b5 ? v[8 + k] : v[k], send = b5 ? v[k] : v[8 + k];
-        w[k] = keep + __shfl_xor(send, 32, kWave);
-    }
-    float u[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float keep = b4 ? w[4 + k] : w[k], send = b4 ? w[k] : w[4 + k];
-        u[k] = keep + __shfl_xor(send, 16, kWave);
-    }
-    float z[2];
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const float keep = b3 ? u[2 + k] : u[k], send = b3 ? u[k] : u[2 + k];
-        z[k] = keep + dpp_get<0x140>(send);
-    }
-    const float keep = b2 ? z[1] : z[0], send = b2 ? z[0] : z[1];
-    float r = keep + dpp_get<0x141>(send);
-    r += dpp_get<0x4E>(r);
-    r += dpp_get<0xB1>(r);
-    return r;
-}
-
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kRs2pi = 0.39894228040143267794;
 constexpr int TQ = 128, G = 4, NT = TQ * G, EPT = 16, KC = 16;
@@ -71,65 +36,20 @@ struct VoxC {
     int ix, iy, pix, k;
 };
 
-struct PixG {
-    double sp, cp, sx, sy, dx, dy, dz, ndx, ndy, nsx, nsy, nsz;
-};
+using PixG = PixelGeom;
 
 __device__ __forceinline__ PixG pixel_setup(const CoeffArgs &a, int v, int p) {
-#pragma clang fp contract(off)
-    PixG pg;
-    const int col = p % a.f.n_cols, row = p / a.f.n_cols;
-    const ViewConst vc = a.f.views[v];
-    pg.sp = vc.sp;
-    pg.cp = vc.cp;
-    pg.sx = vc.src_x;
-    pg.sy = vc.src_y;
-    const double rdet = ((double)col - (double)a.f.n_cols / 2.0 + 0.5) * a.f.col_pitch;
-    pg.dx = a.f.center_x - a.f.r_det * vc.sp + rdet * vc.cp;
-    pg.dy = a.f.center_y + a.f.r_det * vc.cp + rdet * vc.sp;
-    pg.dz = a.f.det_z0 + (double)row * a.f.row_pitch;
-    pg.ndx = vc.sp;
-    pg.ndy = -vc.cp;
-    pg.nsx = -vc.sp * a.f.sta;
-    pg.nsy = vc.cp * a.f.sta;
-    pg.nsz = -a.f.cta;
-    return pg;
+    return pixel_geometry(a.f, v, p);
 }
 
-// integrate_diffraction.cu:192-255 in FP64 (same order as the FP64 oracle)
+// integrate_diffraction.cu:192-255 in FP64 (triple_geometry)
 __device__ __forceinline__ void geom_c(const ForwardArgs &a, const PixG &pg, int ix, int iy,
                                        VoxC &o) {
-#pragma clang fp contract(off)
-    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
-    const double ax = vx - pg.sx, ay = vy - pg.sy;
-    const double proj = (ax * pg.nsx + ay * pg.nsy) / sqrt(pg.nsx * pg.nsx + pg.nsy * pg.nsy);
-    const double vz = -proj * a.tan_w2;
-    const double hvox = proj * a.tan_w;
-    const double az = vz - 0.0;
-    const double bx = pg.dx - vx, by = pg.dy - vy, bz = pg.dz - vz;
-    const double ab = ax * bx + ay * by + az * bz;
-    const double an2 = ax * ax + ay * ay + az * az;
-    const double bn2 = bx * bx + by * by + bz * bz;
-    const double cth = ab * (1.0 / sqrt(an2)) * (1.0 / sqrt(bn2));
-    const double sth = sqrt(fabs(1.0 - cth) / 2.0);
-    const double Sx = bx - ab * ax / an2, Sy = by - ab * ay / an2, Sz = bz - ab * az / an2;
-    const double Dx = -ax + ab * bx / bn2, Dy = -ay + ab * by / bn2, Dz = -az + ab * bz / bn2;
-    const double Vx = -(Sx + Dx), Vy = -(Sy + Dy), Vz = -(Sz + Dz);
-    const double Dn = Dx * pg.ndx + Dy * pg.ndy + Dz * 0.0;
-    const double Sn = Sx * pg.nsx + Sy * pg.nsy + Sz * pg.nsz;
-    const double dD2 = (Dx * Dx + Dy * Dy + Dz * Dz - Dn * Dn) * a.adet / 12.0;
-    const double vV2 =
-        (a.voxel_size * a.voxel_size * (Vx * Vx + Vy * Vy) + hvox * hvox * Vz * Vz) / 12.0;
-    const double sS2 = (Sx * Sx + Sy * Sy + Sz * Sz - Sn * Sn) * a.focal_spot * a.focal_spot / 12.0;
-    const double rdotn = -pg.sp * ax + pg.cp * ay;
-    const double cospsi = rdotn * (1.0 / sqrt(ax * ax + ay * ay));
-    const double g_sv = cospsi * (1.0 / sqrt(an2)) * a.tan_w;
-    const double g_vd = -(pg.ndx * bx + pg.ndy * by) * (1.0 / sqrt(bn2 * bn2 * bn2));
-    const double k1 = 2.0 * sth / a.hbarc;
-    o.s = sth;
-    o.As = k1 * k1 * a.e_width * a.e_width / 12.0;
-    o.Bs = (sS2 + vV2 + dD2) / (2.0 * sth) / (2.0 * sth) / an2 / bn2;
-    o.gR = g_sv * g_vd * ((1.0 + cth * cth) / 2.0);
+    const TripleGeom tg = triple_geometry(a, pg, ix, iy);
+    o.s = tg.s;
+    o.As = tg.As;
+    o.Bs = tg.Bs;
+    o.gR = tg.gR;
 }
 
 // Bilinear sample of padded planar map b at (px,py): zero border by index clamping.

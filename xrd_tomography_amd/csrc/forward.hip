@@ -248,112 +248,20 @@ struct VoxShared {
     int na, pad;
 };
 
-// Per-(view, pixel, voxel) scattering geometry in FP64
-// (integrate_diffraction.cu:192-255), same operation order as the FP64 oracle.
-struct PixelGeom {
-    double sp, cp, sx, sy, dx, dy, dz, ndx, ndy, nsx, nsy, nsz;
-};
-
+// Per-(view, pixel, voxel) scattering geometry of the record (triple_geometry).
 __device__ __forceinline__ void voxel_geometry(const ForwardArgs &a, const PixelGeom &pg, int ix,
                                                int iy, VoxShared &o) {
-#pragma clang fp contract(off)
-    const double vx = (double)ix + 0.5, vy = (double)iy + 0.5;
-    const double ax = vx - pg.sx, ay = vy - pg.sy;
-    const double proj = (ax * pg.nsx + ay * pg.nsy) / sqrt(pg.nsx * pg.nsx + pg.nsy * pg.nsy);
-    const double vz = -proj * a.tan_w2;
-    const double hvox = proj * a.tan_w;
-    const double az = vz - 0.0;
-    const double bx = pg.dx - vx, by = pg.dy - vy, bz = pg.dz - vz;
-    const double ab = ax * bx + ay * by + az * bz;
-    const double an2 = ax * ax + ay * ay + az * az;
-    const double bn2 = bx * bx + by * by + bz * bz;
-    const double cth = ab * (1.0 / sqrt(an2)) * (1.0 / sqrt(bn2));
-    const double sth = sqrt(fabs(1.0 - cth) / 2.0);
-    const double Sx = bx - ab * ax / an2, Sy = by - ab * ay / an2, Sz = bz - ab * az / an2;
-    const double Dx = -ax + ab * bx / bn2, Dy = -ay + ab * by / bn2, Dz = -az + ab * bz / bn2;
-    const double Vx = -(Sx + Dx), Vy = -(Sy + Dy), Vz = -(Sz + Dz);
-    const double Dn = Dx * pg.ndx + Dy * pg.ndy + Dz * 0.0;
-    const double Sn = Sx * pg.nsx + Sy * pg.nsy + Sz * pg.nsz;
-    const double dD2 = (Dx * Dx + Dy * Dy + Dz * Dz - Dn * Dn) * a.adet / 12.0;
-    const double vV2 =
-        (a.voxel_size * a.voxel_size * (Vx * Vx + Vy * Vy) + hvox * hvox * Vz * Vz) / 12.0;
-    const double sS2 = (Sx * Sx + Sy * Sy + Sz * Sz - Sn * Sn) * a.focal_spot * a.focal_spot / 12.0;
-    const double rdotn = -pg.sp * ax + pg.cp * ay;
-    const double cospsi = rdotn * (1.0 / sqrt(ax * ax + ay * ay));
-    const double g_sv = cospsi * (1.0 / sqrt(an2)) * a.tan_w;
-    const double g_vd = -(pg.ndx * bx + pg.ndy * by) * (1.0 / sqrt(bn2 * bn2 * bn2));
-    const double res2 = sS2 + vV2 + dD2;
-    const double k1 = 2.0 * sth / a.hbarc;
-    o.cth = cth;
-    o.s = sth;
-    o.As = k1 * k1 * a.e_width * a.e_width / 12.0;
-    o.Bs = res2 / (2.0 * sth) / (2.0 * sth) / an2 / bn2;
-    o.gp = g_sv * g_vd;
-    o.gR = o.gp * ((1.0 + cth * cth) / 2.0);
+    const TripleGeom tg = triple_geometry(a, pg, ix, iy);
+    o.cth = tg.cth;
+    o.s = tg.s;
+    o.As = tg.As;
+    o.Bs = tg.Bs;
+    o.gp = tg.gp;
+    o.gR = tg.gR;
 }
 
 __device__ __forceinline__ double f2_kn(const ForwardArgs &a, double e) {
     return a.kn0 + a.kn1 * e + a.kn2 * e * e;
-}
-
-// v[k] of lane `from` for DPP control CTRL (a permutation within 16 lanes).
-template <int CTRL> __device__ __forceinline__ float dpp_get(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-
-// Sums 16 values per lane over the wave so that lane l ends up with the wave total
-// of value (l >> 2) & 15.  Each step pairs lanes that differ in one lane bit; the
-// lane with the bit clear keeps the lower half of its values, its partner the upper
-// half, and each adds what the other sends: bits 5 and 4 by shuffles, bit 3 by
-// row_mirror (lane i <-> 15 - i), bit 2 by row_half_mirror (i <-> 7 - i); bits 1
-// and 0 then add the one remaining value (quad_perm swaps).
-__device__ __forceinline__ float transpose_sum16(const float *v, int lane) {
-    float w[8];
-    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float keep = b5 ? v[8 + k] : v[k], send = b5 ? v[k] : v[8 + k];
-        w[k] = keep + __shfl_xor(send, 32, kWave);
-    }
-    float u[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float keep = b4 ? w[4 + k] : w[k], send = b4 ? w[k] : w[4 + k];
-        u[k] = keep + __shfl_xor(send, 16, kWave);
-    }
-    float z[2];
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const float keep = b3 ? u[2 + k] : u[k], send = b3 ? u[k] : u[2 + k];
-        z[k] = keep + dpp_get<0x140>(send);
-    }
-    const float keep = b2 ? z[1] : z[0], send = b2 ? z[0] : z[1];
-    float r = keep + dpp_get<0x141>(send);
-    r += dpp_get<0x4E>(r);
-    r += dpp_get<0xB1>(r);
-    return r;
-}
-
-// Per-(view, pixel) geometry (integrate_diffraction.cu:81-128), FP64.
-__device__ __forceinline__ PixelGeom pixel_geometry(const ForwardArgs &a, int v, int p) {
-#pragma clang fp contract(off)
-    const int col = p % a.n_cols, row = p / a.n_cols;
-    PixelGeom pg;
-    const ViewConst vc = a.views[v];
-    pg.sp = vc.sp;
-    pg.cp = vc.cp;
-    pg.sx = vc.src_x;
-    pg.sy = vc.src_y;
-    const double rdet = ((double)col - (double)a.n_cols / 2.0 + 0.5) * a.col_pitch;
-    pg.dx = a.center_x - a.r_det * vc.sp + rdet * vc.cp;
-    pg.dy = a.center_y + a.r_det * vc.cp + rdet * vc.sp;
-    pg.dz = a.det_z0 + (double)row * a.row_pitch;
-    pg.ndx = vc.sp;
-    pg.ndy = -vc.cp;
-    pg.nsx = -vc.sp * a.sta;
-    pg.nsy = vc.cp * a.sta;
-    pg.nsz = -a.cta;
-    return pg;
 }
 
 // One thread per (view of the batch, pixel, voxel): the detector leg's line
