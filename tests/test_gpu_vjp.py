@@ -150,3 +150,50 @@ def test_autograd_function_poisson_loss(xrd, oracle):
     ec, em = _rel(coef.grad.cpu().numpy(), gco), _rel(maps.grad.cpu().numpy(), gmo)
     print(f"\n[autograd] coef grad rel err {ec:.2e}, maps grad rel err {em:.2e}")
     assert ec <= 1e-5 and em <= 1e-5
+
+
+@pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "inplace"),
+                                             (False, "bundled")])
+def test_vjp_run_twice(xrd, full_mu, scatter, monkeypatch):
+    """Two runs of the VJP on the same inputs.  The coefficient gradient adds its terms
+    in a fixed order: the same bits.  The map gradient adds each texel's FP64
+    contributions with atomics, so its last bits follow their order; any two orders
+    agree to (n - 1) 2^-53 of the sum of the contributions' magnitudes, n the
+    contributions per texel.  That sum is J'|w| (J is one-signed), and n is below
+    2^20 here, so the runs agree to 2^-33 of J'|w| element by element."""
+    import torch
+    if scatter == "inplace":
+        monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
+    cp = _case(full_mu=full_mu)
+    args = xrd.coeff_problem_tensors(cp)
+    g, d = cp.base.geometry, cp.base.disc
+    w = torch.from_numpy(np.random.default_rng(7).normal(size=cp.base.n_rows).astype(np.float32)).cuda()
+    gc1, gm1 = xrd.vjp_coeffs(*args, w, geometry=g, disc=d)
+    gc2, gm2 = xrd.vjp_coeffs(*args, w, geometry=g, disc=d)
+    _, sm = xrd.vjp_coeffs(*args, w.abs(), geometry=g, disc=d, want_coef=False)
+    torch.cuda.synchronize()
+    assert torch.equal(gc1, gc2)
+    diff = (gm1 - gm2).abs()
+    bound = 2.0 ** -33 * sm.abs()
+    print(f"\n[vjp run twice B={cp.B}, {scatter}] map gradient: {int((diff > 0).sum())} of "
+          f"{diff.numel()} elements differ, max {float((diff / sm.abs().clamp_min(1e-300)).max()):.1e} "
+          f"of J'|w|")
+    assert bool((diff <= bound).all())
+
+
+def test_vjp_ct_run_twice(xrd):
+    """The CT map gradient's atomics: two runs agree to 2^-33 of J'|w| (as above)."""
+    import torch
+    from xrd_tomography_amd.synthetic import make_coeff_problem, make_problem
+    p = make_problem("fan64")
+    cp = make_coeff_problem(p, K=2, full_mu=True)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    sp, mp, be = t(p.flat("spectrum")), t(cp.maps), t(cp.beta)
+    kw = dict(geometry=p.geometry, disc=p.disc)
+    w = torch.from_numpy(np.random.default_rng(8).normal(
+        size=p.geometry.n_views * p.geometry.n_cols * p.disc.n_energy).astype(np.float32)).cuda()
+    g1 = xrd.vjp_ct(sp, mp, be, w, **kw)
+    g2 = xrd.vjp_ct(sp, mp, be, w, **kw)
+    s = xrd.vjp_ct(sp, mp, be, w.abs(), **kw)
+    torch.cuda.synchronize()
+    assert bool(((g1 - g2).abs() <= 2.0 ** -33 * s.abs()).all())

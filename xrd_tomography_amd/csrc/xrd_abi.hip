@@ -1235,7 +1235,12 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
             ca.grad_tm = (double *)ctx->cm_gtm.ptr;
             HIP_TRY(launch_zero_f64(ca.grad_tm, npix * 64, s), "zero kernel");
         } else {
-            HIP_TRY(launch_zero_f64(grad_maps, npix * n_basis, s), "zero kernel");
+            // planar maps: the atomics add into a context buffer (ordinary device
+            // memory, where FP64 atomics are supported whatever memory the caller
+            // passes), copied out at the end
+            HIP_TRY(ctx->cm_gtm.ensure(npix * n_basis * sizeof(double)), "workspace alloc");
+            ca.grad_maps = (double *)ctx->cm_gtm.ptr;
+            HIP_TRY(launch_zero_f64(ca.grad_maps, npix * n_basis, s), "zero kernel");
         }
     }
     // XRD_VJP_SCATTER=inplace keeps the detector legs' scatter inside the VJP launch
@@ -1273,6 +1278,10 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
         }
     }
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
+    if (grad_maps && !ca.maps_tm)
+        HIP_TRY(hipMemcpyAsync(grad_maps, ca.grad_maps, npix * n_basis * sizeof(double),
+                               hipMemcpyDeviceToDevice, s),
+                "gradient copy");
     return XRD_OK;
 }
 
