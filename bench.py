@@ -419,6 +419,17 @@ def run_batch(args, dev):
                    for p in stored)
     batch = api.MlemBatch(stored)
     tuned = batch.autotune(reps=2)
+    # algorithmic bytes per row-span group (one persistent launch each)
+    spans = (256, 384, 512, 768, 1024, 1536, 2048)
+    per_span = {}
+    for p in stored:
+        nr, nc = p[0].shape
+        sp = next(v for v in spans if nc <= v)
+        b = per_span.setdefault(sp, [0, 0])
+        b[0] += 1
+        b[1] += nr * nc * esize + nr * 8 + nc * 12
+    for gr in tuned["groups"]:
+        gr["problems"], gr["bytes"] = per_span.get(gr["span"], (0, 0))
     log(f"[batch] autotuned launch: {tuned}")
     for p in stored:
         p[3].fill_(10.0)
