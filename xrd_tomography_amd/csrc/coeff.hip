@@ -573,13 +573,12 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         lo = fminf(lo, __shfl_xor(lo, o, kWave));
         hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
     }
-    float pf[EPT];
-    double ft[EPT];
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-        pf[i] = 0.f;
-        ft[i] = 0.0;
-    }
+    // fine totals per wave and energy slot, FP64 in LDS: each chunk's FP32 partials
+    // are summed over the wave (16-value transposed sum) and added once per chunk,
+    // so no per-lane total is held in registers through the ray traces
+    static_assert(EPT <= 16, "the totals use the 16-value transposed wave sum");
+    __shared__ double ftw[NT / kWave][16];
+    if (t < NT / kWave * 16) (&ftw[0][0])[t] = 0.0;
     const int nv = a.nvox[0];
     for (int idx = t; idx < a.n_basis * E; idx += NT) bet[idx] = a.beta[idx];
     for (int k0 = 0; k0 < nv; k0 += KC) {
@@ -628,6 +627,9 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         // those only, in ascending voxel order
         {
             const int ln = t & (kWave - 1);
+            float pf[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) pf[i] = 0.f;
 #pragma unroll
             for (int i = 0; i < EPT; i++) {
                 const int e = h + G * i;
@@ -659,20 +661,13 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
                     pf[i] = fmaf(u.x, u.y, pf[i]);
                 }
             }
-        }
-#pragma unroll
-        for (int i = 0; i < EPT; i++) {
-            ft[i] += (double)pf[i];
-            pf[i] = 0.f;
+            const float sf = transpose_sum16(pf, ln);
+            if ((ln & 3) == 0) ftw[t >> 6][(ln >> 2) & 15] += (double)sf;
         }
         __syncthreads();
     }
-    const int lane = t & 63, wid = t >> 6;
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-        const double sv = wave_sum(ft[i]);
-        if (lane == 0) red[wid * EPT + i] = sv;
-    }
+    const int wid = t >> 6;
+    if (t < NT / kWave * EPT) red[(t / EPT) * EPT + t % EPT] = ftw[t / EPT][t % EPT];
     __syncthreads();
     double *hf = red + (NT / kWave) * EPT;
     if (t < E) {
@@ -1011,9 +1006,12 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 vox[t] = vs;
             }
             if (t < E) {
-                // transpose of the detector-response window
+                // transpose of the detector-response window: the detector bins whose
+                // window holds source bin t lie within 2*DNRG of it (the windows are
+                // 2*DNRG+1 bins, shifted but never truncated), visited in ascending order
                 double sw = 0.0;
-                for (int ed = 0; ed < E; ed++) {
+                const int ed0 = max(0, t - 2 * f.dn), ed1 = min(E - 1, t + 2 * f.dn);
+                for (int ed = ed0; ed <= ed1; ed++) {
                     int wlo, whi;
                     src_window(ed, E, f.dn, wlo, whi);
                     if (t >= wlo && t <= whi)

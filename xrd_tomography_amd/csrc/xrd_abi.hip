@@ -1381,7 +1381,7 @@ extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
 
 // ---- batched MLEM -----------------------------------------------------------------
 
-// Launch shape of one row-span group (index log2(span / 256)).
+// Launch shape of one row-span group (index span_index(span)).
 struct BatchShape {
     int blocks_per_cu = 2;
     int rows_per_wave = 2;
