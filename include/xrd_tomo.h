@@ -256,7 +256,7 @@ int xrd_trace_rays(xrd_context *ctx, int32_t nx, int32_t ny, double ray_step, in
  *     (the reference is B = 2, maps = (a1,a2), beta = (1/e^3, 1/f2(e)) of
  *     integrate_diffraction.cu:291-296; a per-energy map is B = NRG, beta = I).
  * out_counts has the Nphotons layout (Compton excluded).
- * Limits: K <= 32, B <= 64, NRG <= 64, NWT <= 128. */
+ * Limits: K <= 32, B <= 64, NRG <= 64, NWT <= 128, (nx+2)(ny+2) < 2^24. */
 int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
                        const float *spectrum, const int32_t *segmentation, const float *coef,
                        int32_t n_coef, const float *basis, int32_t n_basis, const float *maps,
@@ -289,7 +289,8 @@ int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g, const xrd_discretiza
  * source-side entry, and
  *   out_t[v + V*(c + COL*ed)] = sum_{e in win(ed)} spectrum[e + NRG*ed]
  *                               * exp(-sum_b beta[b,e] L_b),
- * L_b the line integral of maps[b] [nx*ny].  Limits: B <= 64, NRG <= 64. */
+ * L_b the line integral of maps[b] [nx*ny].  Limits: B <= 64, NRG <= 64,
+ * (nx+2)(ny+2) < 2^24. */
 int xrd_forward_ct(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *d,
                    const float *spectrum, int32_t n_basis, const float *maps, const float *beta,
                    float *out_t, void *stream);

@@ -22,9 +22,14 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("full_mu", [True, False])
-def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu):
+@pytest.mark.parametrize("full_mu,trace", [(True, "slabs"), (True, "inplace"), (False, "slabs")])
+def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu, trace, monkeypatch):
+    """trace: with texel-major maps the detector legs are traced by their own launch
+    per slab of views (the default) or inside the forward launch
+    (XRD_VJP_SCATTER=inplace, the path of problems whose slab buffer would not fit)."""
     import torch
+    if trace == "inplace":
+        monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
     cp = _case(full_mu=full_mu)
     args = xrd.coeff_problem_tensors(cp)
     y = xrd.forward_coeffs(*args, geometry=cp.base.geometry, disc=cp.base.disc)
@@ -33,16 +38,17 @@ def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu):
     yg = y.cpu().numpy().reshape(-1).astype(np.float64)
     live = yo > 1e-9 * yo.max()
     rel = np.max(np.abs(yg[live] - yo[live]) / yo[live])
-    print(f"\n[coeff forward B={cp.B}] max rel err vs FP64 {rel:.2e}")
+    print(f"\n[coeff forward B={cp.B}, {trace}] max rel err vs FP64 {rel:.2e}")
     assert rel <= 1e-5
 
 
 @pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "inplace"),
                                              (False, "bundled")])
 def test_vjp_vs_oracle(xrd, oracle, full_mu, scatter, monkeypatch):
-    """scatter: the texel-major map gradient's detector legs in pixel bundles (the
-    default), or inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of problems
-    whose gamma buffer would not fit)."""
+    """scatter: with texel-major maps the detector legs are traced by their own launch
+    per slab of views and the map gradient's legs scattered in pixel bundles (the
+    default), or both inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of
+    problems whose slab buffers would not fit)."""
     import torch
     if scatter == "inplace":
         monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")

@@ -170,7 +170,10 @@ union WaveList {
         int ci[kWave], cj[kWave];
     } runs;
     struct {
-        double w[4 * kWave];
+        union {
+            double w[4 * kWave];   // the scatter walks' weights
+            float wf[4 * kWave];   // the trace walk's weights
+        };
         int off[4 * kWave];
     } tex;
 };
@@ -192,61 +195,99 @@ struct SettledRun {
     double w[4];
 };
 
-// The runs of samples q0 .. q0+63 of a ray (one wave, one sample per lane) that
-// share a bilinear cell, in ray order; returns their number nr, run r settled in lane
-// r's registers.  Each run-end lane forms its run's FP64 bilinear weight sums in
-// closed form and writes its cell to the wave's run list.  A stencil corner is
-// shared by at most three consecutive cells of a straight ray (the cell indices are
-// monotone along it), so each run then folds in the weights of the two runs before it
-// for the corners it shares with them and drops the corners that one of the next two
-// runs shares: every texel of the window appears once, with its total weight.  The
-// run list is free again on return.
-__device__ __forceinline__ int cell_runs_settled(const Ray &ray, double step, int n, int q0, int nx,
-                                                 int ny, int lane, WaveList &L, SettledRun &sr) {
-    const int cnt = min(kWave, n - q0);
-    int i0, j0;
-    double ax, ay;
-    lane_sample(ray, step, min(q0 + lane, n - 1), i0, j0, ax, ay);
-    const bool act = lane < cnt;
-    const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
-    const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
-    const bool first = lane == 0 || pi != i0 || pj != j0;
-    const bool last = act && (lane == cnt - 1 || ni != i0 || nj != j0);
-    const unsigned long long starts = __ballot(first);
-    // first lane of this lane's run: highest start at or below it
-    const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-    const int rs = 63 - __clzll(below);
-    // the run's weight sums in closed form: along the run the fractions fall back
-    // from this lane's (ax, ay) by (sx, sy) per sample (positions are linear in the
-    // sample index), so with k = 0 .. nn-1 samples back,
-    //   sum ax = nn ax - sx S1,  sum ay = nn ay - sy S1,
-    //   sum ax ay = nn ax ay - (ax sy + ay sx) S1 + sx sy S2,
-    // S1 = sum k, S2 = sum k^2; only the run-end lanes' values are used
+// Passes of 64 samples that one window can take: a pass adds at most one run per
+// cell it enters, and its 63 steps of length step cross at most 63 step |ux| + 1
+// columns and 63 step |uy| + 1 rows, so it adds at most 3 + 63 step (|ux| + |uy|)
+// runs; windows hold up to 64 runs.
+constexpr int kMaxPasses = 4;
+__device__ __forceinline__ int window_passes(const Ray &ray, double step) {
+    const double c = 63.0 * step * (fabs(ray.ux) + fabs(ray.uy));
+    const int b = 4 + (int)fmin(c, 64.0);
+    return max(1, min(kMaxPasses, kWave / b));
+}
+
+// The runs of samples q0 .. q0 + 64 passes - 1 of a ray (clamped to its n samples)
+// that share a bilinear cell, in ray order; returns their number nr, run r settled in
+// lane r's registers.  Each pass takes one sample per lane; each run-end lane forms
+// its run's FP64 bilinear weight sums in closed form over the run's samples in the
+// pass and writes its cell to the wave's run list, and a run that continues from the
+// previous pass adds its sums to that pass's last entry.  A stencil corner is shared
+// by at most three consecutive cells of a straight ray (the cell indices are monotone
+// along it), so each run then folds in the weights of the two runs before it for the
+// corners it shares with them and drops the corners that one of the next two runs
+// shares: every texel of the window appears once, with its total weight.  The run
+// list is free again on return.  passes <= window_passes(ray) keeps nr <= 64.  FLAT
+// issues every LDS read of the settle before the adjacency tests select among them
+// (no dependent read behind a branch); the other form reads under the tests and
+// holds fewer registers.  Both give the same bits.  MAXP = 1 compiles the one-pass
+// window only.
+template <bool FLAT, int MAXP>
+__device__ __forceinline__ int cell_runs_settled(const Ray &ray, double step, int n, int q0,
+                                                 int passes, int nx, int ny, int lane, WaveList &L,
+                                                 SettledRun &sr) {
     const double sx = ray.ux * step, sy = ray.uy * step;
-    const double nn = (double)(lane - rs + 1);
-    const double s1 = nn * (nn - 1.0) * 0.5, s2 = (nn - 1.0) * nn * (2.0 * nn - 1.0) / 6.0;
-    const double sax = nn * ax - sx * s1, say = nn * ay - sy * s1;
-    const double saxy = nn * ax * ay - (ax * sy + ay * sx) * s1 + sx * sy * s2;
-    const double w[4] = {nn - sax - say + saxy, sax - saxy, say - saxy, saxy};
-    const unsigned long long ends = __ballot(last);
-    const int nr = __popcll(ends);
-    if (last) {
-        const int r = __popcll(ends & ((1ull << lane) - 1ull));
+    const unsigned long long lower = (1ull << lane) - 1ull;
+    int nr = 0, ci_prev = 0, cj_prev = 0;
+    for (int p = 0; p < (MAXP == 1 ? 1 : passes); p++) {
+        const int qb = q0 + p * kWave;
+        if (qb >= n) break;  // uniform
+        const int cnt = min(kWave, n - qb);
+        int i0, j0;
+        double ax, ay;
+        lane_sample(ray, step, min(qb + lane, n - 1), i0, j0, ax, ay);
+        const bool act = lane < cnt;
+        const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
+        const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
+        const bool first = lane == 0 ? (p == 0 || ci_prev != i0 || cj_prev != j0)
+                                     : (pi != i0 || pj != j0);
+        const bool last = act && (lane == cnt - 1 || ni != i0 || nj != j0);
+        const unsigned long long starts = __ballot(first);
+        // the first run of a later pass may continue the previous pass's last run
+        const bool cont = (starts & 1ull) == 0ull;
+        // first lane of this lane's run in the pass: highest start at or below it
+        // (none: the continued run, from lane 0)
+        const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int rs = below ? 63 - __clzll(below) : 0;
+        // the run's weight sums in closed form: along the run the fractions fall back
+        // from this lane's (ax, ay) by (sx, sy) per sample (positions are linear in the
+        // sample index), so with k = 0 .. nn-1 samples back,
+        //   sum ax = nn ax - sx S1,  sum ay = nn ay - sy S1,
+        //   sum ax ay = nn ax ay - (ax sy + ay sx) S1 + sx sy S2,
+        // S1 = sum k, S2 = sum k^2; only the run-end lanes' values are used
+        // (S1 and S2 are integers below 2^19, formed exactly in integer arithmetic)
+        const int m = lane - rs + 1;
+        const double nn = (double)m;
+        const double s1 = (double)((m * (m - 1)) >> 1), s2 = (double)((m - 1) * m * (2 * m - 1) / 6);
+        const double sax = nn * ax - sx * s1, say = nn * ay - sy * s1;
+        const double saxy = nn * ax * ay - (ax * sy + ay * sx) * s1 + sx * sy * s2;
+        const double w[4] = {nn - sax - say + saxy, sax - saxy, say - saxy, saxy};
+        const unsigned long long ends = __ballot(last);
+        if (last) {
+            const int r = nr + __popcll(ends & lower) - (cont ? 1 : 0);
+            if (cont && r == nr - 1) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
-            const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
-            L.runs.off[c][r] = in ? (ti + nx * tj) * kTexelMaps : -1;
-            L.runs.w[c][r] = w[c];
+                for (int c = 0; c < 4; c++) L.runs.w[c][r] += w[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
+                    const bool in = ti >= 0 && tj >= 0 && ti < nx && tj < ny;
+                    L.runs.off[c][r] = in ? (ti + nx * tj) * kTexelMaps : -1;
+                    L.runs.w[c][r] = w[c];
+                }
+                L.runs.ci[r] = i0;
+                L.runs.cj[r] = j0;
+            }
         }
-        L.runs.ci[r] = i0;
-        L.runs.cj[r] = j0;
+        nr += __popcll(ends) - (cont ? 1 : 0);
+        ci_prev = __builtin_amdgcn_readlane(i0, kWave - 1);
+        cj_prev = __builtin_amdgcn_readlane(j0, kWave - 1);
+        // a wave's LDS accesses complete in issue order, so only the compiler must
+        // keep the list reads below the writes (a memory fence would also wait for
+        // every global access in flight)
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
     }
-    // a wave's LDS accesses complete in issue order, so only the compiler must
-    // keep the list reads below the writes (a memory fence would also wait for
-    // every global access in flight)
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
     // lane r < nr settles run r's corners against runs r-2 .. r+2
     sr.ci = sr.cj = 0;
 #pragma unroll
@@ -254,7 +295,56 @@ __device__ __forceinline__ int cell_runs_settled(const Ray &ray, double step, in
         sr.off[c] = -1;
         sr.w[c] = 0.0;
     }
-    if (lane < nr) {
+    if (FLAT && lane < nr) {
+        // every read first (indices clamped into the list, so each is in range),
+        // then the adjacency selects: no branch waits on an LDS read
+        const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
+        sr.ci = ci;
+        sr.cj = cj;
+        int nci[4], ncj[4];  // runs lane-2, lane-1, lane+1, lane+2
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int k = s < 2 ? s - 2 : s - 1;
+            const int r = min(max(lane + k, 0), kWave - 1);
+            const bool ok = lane + k >= 0 && lane + k < nr;
+            const int vi = L.runs.ci[r], vj = L.runs.cj[r];
+            nci[s] = ok ? vi : INT_MIN / 2;
+            ncj[s] = ok ? vj : INT_MIN / 2;
+        }
+        double own[4], prev[2][4];
+        int offs[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            own[c] = L.runs.w[c][lane];
+            offs[c] = L.runs.off[c][lane];
+            const int ti = ci + (c & 1), tj = cj + (c >> 1);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                // the corner of run lane-2+s that is texel (ti, tj), if it is adjacent
+                const int di = ti - nci[s], dj = tj - ncj[s];
+                prev[s][c] = L.runs.w[(di & 1) + 2 * (dj & 1)][max(lane + s - 2, 0)];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int ti = ci + (c & 1), tj = cj + (c >> 1);
+            bool later = false;
+            double wsum = own[c];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int di = ti - nci[s], dj = tj - ncj[s];
+                const bool adj = di >= 0 && di <= 1 && dj >= 0 && dj <= 1;
+                if (s >= 2)
+                    later = later || adj;
+                else
+                    wsum = adj ? wsum + prev[s][c] : wsum;
+            }
+            sr.off[c] = later ? -1 : offs[c];
+            sr.w[c] = wsum;
+        }
+    }
+    if (!FLAT && lane < nr) {
+        // the same settle with the reads under the adjacency tests: fewer registers
         const int ci = L.runs.ci[lane], cj = L.runs.cj[lane];
         sr.ci = ci;
         sr.cj = cj;
@@ -289,17 +379,22 @@ __device__ __forceinline__ int cell_runs_settled(const Ray &ray, double step, in
 }
 
 // Compacts the kept corners of the settled runs into the wave's texel list, in lane
-// order and corner order within a lane: the element offset of the texel in the
-// padded texel-major maps (padded) or in the unpadded grid, and its weight.  The
-// list is padded with (0, 0.0) entries to a whole number of blocks of kTexBlock, so
+// order and corner order within a lane, with each texel's weight:
+//   kListScatter: the element offset in the unpadded grid, FP64 weight;
+//   kListTrace64: the element offset in the padded texel-major maps, FP64 weight;
+//   kListTrace32: the byte offset in the padded texel-major maps (below 2^32: the host
+//                 keeps those maps under 4 GiB), FP32 weight.
+// The list is padded with (0, 0) entries to a whole number of blocks of kTexBlock, so
 // a walk reads aligned 128-bit broadcasts; returns the entry count, nblk the padded
 // length.  Offset 0 is the zero border texel of the padded maps and a real texel of
 // the unpadded grid, so walks that scatter skip the padding.
 constexpr int kTexBlock = 16;
 static_assert((4 * kWave) % kTexBlock == 0, "the texel list holds whole blocks");
+enum { kListScatter, kListTrace64, kListTrace32 };
 
+template <int MODE>
 __device__ __forceinline__ int compact_texels(WaveList &L, int lane, const SettledRun &sr, int nx,
-                                              bool padded, int &nblk) {
+                                              int &nblk) {
     const int pw = nx + 2;
     bool kept[4];
     int off[4];
@@ -307,8 +402,10 @@ __device__ __forceinline__ int compact_texels(WaveList &L, int lane, const Settl
     for (int c = 0; c < 4; c++) {
         // kept corners lie inside the grid, so the padded index is in range
         kept[c] = sr.off[c] >= 0;
-        off[c] = padded ? ((sr.ci + (c & 1) + 1) + pw * (sr.cj + (c >> 1) + 1)) * kTexelMaps
-                        : sr.off[c];
+        const unsigned pt = (unsigned)((sr.ci + (c & 1) + 1) + pw * (sr.cj + (c >> 1) + 1));
+        off[c] = MODE == kListTrace32   ? (int)(pt * (unsigned)(kTexelMaps * sizeof(float)))
+                 : MODE == kListTrace64 ? (int)(pt * (unsigned)kTexelMaps)
+                                        : sr.off[c];
     }
     const unsigned long long lower = (1ull << lane) - 1ull;
     int pos = 0, nent = 0;
@@ -322,14 +419,20 @@ __device__ __forceinline__ int compact_texels(WaveList &L, int lane, const Settl
     for (int c = 0; c < 4; c++) {
         if (kept[c]) {
             L.tex.off[pos] = off[c];
-            L.tex.w[pos] = sr.w[c];
+            if (MODE == kListTrace32)
+                L.tex.wf[pos] = (float)sr.w[c];
+            else
+                L.tex.w[pos] = sr.w[c];
             pos++;
         }
     }
     nblk = (nent + kTexBlock - 1) / kTexBlock * kTexBlock;
     if (nent + lane < nblk) {
         L.tex.off[nent + lane] = 0;
-        L.tex.w[nent + lane] = 0.0;
+        if (MODE == kListTrace32)
+            L.tex.wf[nent + lane] = 0.f;
+        else
+            L.tex.w[nent + lane] = 0.0;
     }
     __builtin_amdgcn_wave_barrier();
     asm volatile("" ::: "memory");
@@ -338,44 +441,72 @@ __device__ __forceinline__ int compact_texels(WaveList &L, int lane, const Settl
 
 // Line integral (before the factor step), for lane b's map, of one ray through the
 // texel-major maps: sum over texels of m[texel][b] times the texel's summed bilinear
-// weight.  Each window of 64 samples lists its texels once (cell_runs_settled,
-// compact_texels) and the wave walks the list with sixteen coalesced 256-byte loads
-// in flight, accumulating in FP64 (four partial sums, added in a fixed order).  A
+// weight.  Each window lists its texels once (cell_runs_settled, compact_texels)
+// and the wave walks the list with sixteen coalesced 256-byte loads in flight.  A
 // texel costs one load per window instead of one per sample and corner.
+//   F64: FP64 weights and FP64 sums of the products (four partial sums, added in a
+//        fixed order);
+//   otherwise: FP32 weights, 32-bit byte offsets from the maps' base (one add per
+//        load), and each block's sixteen products summed in FP32 (four partial sums
+//        of four) before the block is added in FP64, so a block adds at most
+//        4 * 2^-24 of the sum of its terms' magnitudes to the error of the sum, and
+//        the FP32 products and weights 2^-23 of it.
+template <bool FLAT, int MAXP, bool F64>
 __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int nx, int ny,
                                                 double step, const Ray &ray, int lane,
                                                 WaveList &L) {
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int q0 = 0; q0 < n; q0 += kWave) {
+    const char *__restrict__ mb = reinterpret_cast<const char *>(m);
+    const unsigned lane4 = (unsigned)lane * (unsigned)sizeof(float);
+    const int passes = MAXP == 1 ? 1 : __builtin_amdgcn_readfirstlane(min(MAXP, window_passes(ray, step)));
+    double total = 0.0, acc64[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < n; q0 += kWave * passes) {
         SettledRun sr;
-        cell_runs_settled(ray, step, n, q0, nx, ny, lane, L, sr);
+        cell_runs_settled<FLAT, MAXP>(ray, step, n, q0, passes, nx, ny, lane, L, sr);
         int nblk;
-        compact_texels(L, lane, sr, nx, true, nblk);
+        compact_texels<F64 ? kListTrace64 : kListTrace32>(L, lane, sr, nx, nblk);
         for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
             float v[kTexBlock];
 #pragma unroll
-            for (int k = 0; k < kTexBlock; k++) v[k] = m[(unsigned)(L.tex.off[e0 + k] + lane)];
-            __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
-#pragma unroll
             for (int k = 0; k < kTexBlock; k++)
-                acc[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc[k & 3]);
+                v[k] = F64 ? m[(unsigned)(L.tex.off[e0 + k] + lane)]
+                           : *reinterpret_cast<const float *>(mb + ((unsigned)L.tex.off[e0 + k] + lane4));
+            __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+            if (F64) {
+#pragma unroll
+                for (int k = 0; k < kTexBlock; k++)
+                    acc64[k & 3] = fma(L.tex.w[e0 + k], (double)v[k], acc64[k & 3]);
+            } else {
+                float acc[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[k] = L.tex.wf[e0 + k] * v[k];
+#pragma unroll
+                for (int k = 4; k < kTexBlock; k++) acc[k & 3] = fmaf(L.tex.wf[e0 + k], v[k], acc[k & 3]);
+                total += (double)((acc[0] + acc[1]) + (acc[2] + acc[3]));
+            }
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
     }
-    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    return F64 ? (acc64[0] + acc64[1]) + (acc64[2] + acc64[3]) : total;
 }
 
+// Settle form and passes per window per kernel (cell_runs_settled): the flat form and
+// multi-pass windows where registers are to spare; the branch form and one-pass
+// windows in the VJP kernel, whose FP64 gradient totals stay live through every trace.
+constexpr bool kFlatForward = true, kFlatVjp = false, kFlatBundle = true;
+constexpr int kPassesForward = kMaxPasses, kPassesVjp = 1;
+
 // Detector legs of a voxel chunk, one wave per ray, one lane per map.
+template <bool FLAT, int MAXP>
 __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &pg, const VoxC *vox,
                                                int nk, float *Ld, WaveList *lists) {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     for (int j = wid; j < nk; j += NT / kWave) {
         const Ray ray = make_ray(pg.dx, pg.dy, (double)vox[j].ix + 0.5, (double)vox[j].iy + 0.5,
                                  a.f.nx, a.f.ny, a.f.ray_step);
-        const double sum = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane,
-                                         lists[wid]);
+        const double sum = wave_trace_tm<FLAT, MAXP, false>(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step,
+                                                            ray, lane, lists[wid]);
         if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)(sum * a.f.ray_step);
     }
 }
@@ -520,9 +651,34 @@ __global__ __launch_bounds__(256) void coeff_src_tm_kernel(CoeffArgs a) {
     const ViewConst vc = a.f.views[v];
     const Ray r = make_ray(vc.src_x, vc.src_y, (double)(pix % nx) + 0.5, (double)(pix / nx) + 0.5,
                            nx, ny, a.f.ray_step);
-    const double sum = wave_trace_tm(a.maps_tm, nx, ny, a.f.ray_step, r, lane, lists[wid]);
+    const double sum =
+        wave_trace_tm<kFlatForward, kPassesForward, false>(a.maps_tm, nx, ny, a.f.ray_step, r, lane,
+                                                            lists[wid]);
     if (lane < a.n_basis)
         a.src_ints[((size_t)v * nx * ny + k) * a.n_basis + lane] = (float)(sum * a.f.ray_step);
+}
+
+// Detector legs of a slab of views for the VJP: one wave per (view of the slab,
+// voxel, pixel) ray, the line integrals of all maps (lane b: map b) stored at
+// ld_buf[((view - v_base) nv + voxel) n_det + pixel][b].  The same trace as the
+// forward kernel's (trace_chunk_tm), so the VJP sees the forward's bits.
+constexpr int kTraceWaves = 4;
+__global__ __launch_bounds__(kTraceWaves *kWave) void coeff_trace_kernel(CoeffArgs a) {
+    __shared__ WaveList lists[kTraceWaves];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int P = a.f.n_det, nv = a.n_vox, nx = a.f.nx;
+    const int64_t item = (int64_t)blockIdx.x * kTraceWaves + wid;
+    if (item >= (int64_t)a.v_count * nv * P) return;  // uniform per wave
+    const int p = (int)(item % P);
+    const int k = (int)((item / P) % nv);
+    const int vs = (int)(item / ((int64_t)P * nv));
+    const PixG pg = pixel_setup(a, a.v_base + vs, p);
+    const int pix = a.vlist[k];
+    const Ray ray = make_ray(pg.dx, pg.dy, (double)(pix % nx) + 0.5, (double)(pix / nx) + 0.5, nx,
+                             a.f.ny, a.f.ray_step);
+    const double sum = wave_trace_tm<kFlatForward, kPassesForward, false>(
+        a.maps_tm, nx, a.f.ny, a.f.ray_step, ray, lane, lists[wid]);
+    a.ld_buf[(size_t)item * kTexelMaps + lane] = (float)(sum * a.f.ray_step);
 }
 
 // Forward counts: one workgroup per (view, pixel).
@@ -530,6 +686,7 @@ __global__ __launch_bounds__(256) void coeff_src_tm_kernel(CoeffArgs a) {
 // are cheaper than the latency that two waves per SIMD leave exposed (fan256, 12 views:
 // 0.98 -> 0.69 s).  Two workgroups need the LDS below 80 KB, so the trace's run
 // lists share their space with the chunk's table, which is never live at the same time.
+template <bool LDBUF>
 __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ VoxC vox[KC];
     __shared__ float2 part[NT];
@@ -557,7 +714,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
     // neighbouring pixels of one view run together on an XCD: their ray fans from a
     // voxel chunk cross nearly the same texels, which stay in that XCD's L2
     const int l = xcd_contiguous(blockIdx.x, gridDim.x);
-    const int v = l / f.n_det, p = l % f.n_det;
+    const int v = a.v_base + l / f.n_det, p = l % f.n_det;
     const size_t npix = (size_t)f.nx * f.ny;
     const PixG pg = pixel_setup(a, v, p);
     float qch = 1e30f, qcl = 0.f, dqc = 0.f;
@@ -597,10 +754,17 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
             Lsrc[j * MAXB + b] = a.src_ints[((size_t)v * npix + k0 + j) * a.n_basis + b];
         }
         __syncthreads();
-        if (a.maps_tm)
-            trace_chunk_tm(a, pg, vox, nk, Ld, cruns);
-        else
+        if (LDBUF) {
+            for (int idx = t; idx < nk * a.n_basis; idx += NT) {
+                const int j = idx / a.n_basis, b = idx % a.n_basis;
+                Ld[j * MAXB + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * f.n_det + p) *
+                                                kTexelMaps + b];
+            }
+        } else if (a.maps_tm) {
+            trace_chunk_tm<kFlatForward, kPassesForward>(a, pg, vox, nk, Ld, cruns);
+        } else {
             trace_chunk(a, pg, vox, nk, part, Ld);
+        }
         // F_vox(q_n) for the chunk
         for (int idx = t; idx < nk * TQ; idx += NT) {
             const int j = idx / TQ, n = idx % TQ;
@@ -731,7 +895,7 @@ __device__ __forceinline__ void scatter_ray(const CoeffArgs &a, const Ray &ray, 
     flush();
 }
 
-// Texel-major form of scatter_ray for one wave: for each window of 64 samples,
+// Texel-major form of scatter_ray for one wave: for each window of samples,
 // cell_runs_settled and compact_texels list the window's texels once each with their
 // summed weights, and the wave walks the list: lane b adds gamma[b] times the weight
 // to grad_tm[texel][b], one coalesced atomic instruction per texel.
@@ -739,11 +903,13 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
                                                int lane, WaveList &L) {
     const double gs = (lane < a.n_basis ? gam[lane] : 0.0) * a.f.ray_step;
     const int n = __builtin_amdgcn_readfirstlane(ray.n);
-    for (int q0 = 0; q0 < n; q0 += kWave) {
+    const int passes = 1;
+    for (int q0 = 0; q0 < n; q0 += kWave * passes) {
         SettledRun sr;
-        cell_runs_settled(ray, a.f.ray_step, n, q0, a.f.nx, a.f.ny, lane, L, sr);
+        cell_runs_settled<kFlatVjp, kPassesVjp>(ray, a.f.ray_step, n, q0, passes, a.f.nx, a.f.ny,
+                                                lane, L, sr);
         int nblk;
-        const int nent = compact_texels(L, lane, sr, a.f.nx, false, nblk);
+        const int nent = compact_texels<kListScatter>(L, lane, sr, a.f.nx, nblk);
         if (gs != 0.0) {
             for (int e0 = 0; e0 < nblk; e0 += kTexBlock) {
                 int off[kTexBlock];
@@ -851,7 +1017,7 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
         for (int i = 0; i < kBundle; i++) {
             if (q0 >= n[i]) continue;
             SettledRun sr;
-            cell_runs_settled(ray[i], step, n[i], q0, nx, ny, lane, L, sr);
+            cell_runs_settled<kFlatBundle, 1>(ray[i], step, n[i], q0, 1, nx, ny, lane, L, sr);
             if (boxed) {
                 // each kept corner is a texel of its own within this ray's window, so the
                 // run lanes write their corners straight into the ray's box column
@@ -864,7 +1030,7 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
                 }
             } else {
                 int nblk;
-                const int nent = compact_texels(L, lane, sr, nx, false, nblk);
+                const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
                 if (g[i] != 0.0)
                     for (int e = 0; e < nent; e++)
                         unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g[i] * L.tex.w[e]);
@@ -921,6 +1087,7 @@ __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1)
 // coefficient VJP 1.11 -> 0.85 s): the trace's lists and the table/Gaussian arrays
 // share their LDS space (live in different phases of each pixel), and beta is read
 // from global memory, where its 16 KB stay in L1.
+template <bool LDBUF>
 __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
     __shared__ VoxC vox[KC];
     __shared__ float Ld[KC * MAXB];
@@ -1021,10 +1188,17 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 wt[t] = sw;
             }
             __syncthreads();
-            if (a.maps_tm)
-                trace_chunk_tm(a, pg, vox, nk, Ld, cruns);
-            else
+            if (LDBUF) {
+                for (int idx = t; idx < nk * B; idx += NT) {
+                    const int j = idx / B, b = idx % B;
+                    Ld[j * MAXB + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * P + p) *
+                                                    kTexelMaps + b];
+                }
+            } else if (a.maps_tm) {
+                trace_chunk_tm<kFlatVjp, kPassesVjp>(a, pg, vox, nk, Ld, cruns);
+            } else {
                 trace_chunk(a, pg, vox, nk, part, Ld);
+            }
             __syncthreads();
             for (int idx = t; idx < nk * E; idx += NT) {
                 const int j = idx / E, e = idx % E;
@@ -1235,7 +1409,7 @@ __device__ __forceinline__ double ct_ray_p(const CoeffArgs &a, int v, int c, int
     const PixG pg = pixel_setup(a, v, c);
     hit = ct_chord(pg.sx, pg.sy, pg.dx, pg.dy, a.f.nx, a.f.ny, a.f.ray_step, ray);
     double Lb = 0.0;
-    if (hit) Lb = wave_trace_tm(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane, runs) *
+    if (hit) Lb = wave_trace_tm<kFlatForward, kPassesForward, true>(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step, ray, lane, runs) *
                   a.f.ray_step;
     const int e = min(lane, E - 1);
     double mu = 0.0;
@@ -1357,7 +1531,24 @@ hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(coeff_forward_kernel, dim3(a.f.n_views * a.f.n_det), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(coeff_forward_kernel<false>, dim3(a.f.n_views * a.f.n_det), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_coeff_forward_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
+    const int V = a.f.n_views, P = a.f.n_det;
+    const int most = std::max(1, std::min(V, slab_views));
+    const int n_slabs = (V + most - 1) / most;
+    const int slab = (V + n_slabs - 1) / n_slabs;
+    for (int v0 = 0; v0 < V; v0 += slab) {
+        a.v_base = v0;
+        a.v_count = std::min(slab, V - v0);
+        const int64_t rays = (int64_t)a.v_count * a.n_vox * P;
+        if (rays > 0)
+            hipLaunchKernelGGL(coeff_trace_kernel, dim3((unsigned)((rays + kTraceWaves - 1) / kTraceWaves)),
+                               dim3(kTraceWaves * kWave), 0, s, a);
+        hipLaunchKernelGGL(coeff_forward_kernel<true>, dim3(a.v_count * P), dim3(NT), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -1384,7 +1575,7 @@ hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
     const int vpg = (a.f.n_views + ng - 1) / ng;
     const int nb = (a.nvox_cap + KC - 1) / KC;
     if (nb > 0) {
-        hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb, ng), dim3(NT), 0, s, a, vpg);
+        hipLaunchKernelGGL(coeff_vjp_kernel<false>, dim3(nb, ng), dim3(NT), 0, s, a, vpg);
         if (a.grad_coef)
             hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
         if (a.grad_maps && a.grad_tm)
@@ -1397,33 +1588,37 @@ size_t coeff_gam_floats_per_view(int n_vox, int n_det) {
     return (size_t)n_vox * (size_t)n_det * kTexelMaps;
 }
 
-hipError_t launch_coeff_vjp_bundled(CoeffArgs a, size_t gam_floats, hipStream_t s) {
+hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
     const int V = a.f.n_views, P = a.f.n_det;
-    const size_t per_view = coeff_gam_floats_per_view(a.n_vox, P);
-    size_t fit = per_view > 0 ? gam_floats / per_view : (size_t)V;
-    const int most = (int)std::max<size_t>(1, std::min<size_t>((size_t)V, fit));
-    // equal slabs (the last one no thinner than the rest), each within the buffer
+    const int most = std::max(1, std::min(V, slab_views));
+    // equal slabs (the last one no thinner than the rest), each within the buffers
     const int n_slabs = (V + most - 1) / most;
     const int slab = (V + n_slabs - 1) / n_slabs;
     const int ng = coeff_vjp_groups(slab, a.nvox_cap);
     const int nb = (a.nvox_cap + KC - 1) / KC;
     const int nbp = (P + kBundle - 1) / kBundle;
-    if (nb <= 0) return hipSuccess;
+    if (nb <= 0 || a.n_vox <= 0) return hipSuccess;
     for (int v0 = 0; v0 < V; v0 += slab) {
         a.v_base = v0;
         a.v_count = std::min(slab, V - v0);
         a.hpart_add = v0 > 0;
+        const int64_t rays = (int64_t)a.v_count * a.n_vox * P;
+        hipLaunchKernelGGL(coeff_trace_kernel, dim3((unsigned)((rays + kTraceWaves - 1) / kTraceWaves)),
+                           dim3(kTraceWaves * kWave), 0, s, a);
         const int ngs = std::min(ng, a.v_count);
         const int vpg = (a.v_count + ngs - 1) / ngs;
-        hipLaunchKernelGGL(coeff_vjp_kernel, dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
-        const int64_t items = (int64_t)a.v_count * a.n_vox * nbp;
-        const int64_t blocks = (items + kBundleWaves - 1) / kBundleWaves;
-        if (blocks > 0)
-            hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3((unsigned)blocks),
-                               dim3(kBundleWaves * kWave), 0, s, a, nbp);
+        hipLaunchKernelGGL(coeff_vjp_kernel<true>, dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
+        if (a.gam_buf) {
+            const int64_t items = (int64_t)a.v_count * a.n_vox * nbp;
+            const int64_t blocks = (items + kBundleWaves - 1) / kBundleWaves;
+            if (blocks > 0)
+                hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3((unsigned)blocks),
+                                   dim3(kBundleWaves * kWave), 0, s, a, nbp);
+        }
     }
     if (a.grad_coef) hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
-    hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
+    if (a.grad_maps && a.grad_tm)
+        hipLaunchKernelGGL(coeff_untranspose_kernel, dim3(4096), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

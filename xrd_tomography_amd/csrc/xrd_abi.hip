@@ -110,7 +110,7 @@ struct xrd_context {
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
-    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm, cm_gam;
+    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm, cm_gam, cm_ld;
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -236,7 +236,8 @@ int xrd_context_destroy(xrd_context *c) {
     if (c->staged_pending) hipEventSynchronize(c->staged);
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
                       &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial, &c->small, &c->cm_maps,
-                      &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm, &c->cm_gam})
+                      &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm, &c->cm_gam,
+                      &c->cm_ld})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
@@ -251,7 +252,7 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
                             &c->offsets, &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial,
                             &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h,
-                            &c->cm_tm, &c->cm_gtm, &c->cm_gam})
+                            &c->cm_tm, &c->cm_gtm, &c->cm_gam, &c->cm_ld})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -1149,8 +1150,9 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
         return fail(XRD_ERR_UNSUPPORTED, "%s: need 1 <= n_coef <= 32 and 1 <= n_basis <= 64", fn);
     if (d->n_energy > 64 || d->n_q > 128)
         return fail(XRD_ERR_UNSUPPORTED, "%s: need n_energy <= 64 and n_q <= 128", fn);
-    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 >= ((size_t)1 << 31))
-        return fail(XRD_ERR_UNSUPPORTED, "%s: (nx+2)(ny+2)*64 must stay below 2^31", fn);
+    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 * sizeof(float) >= ((size_t)1 << 32))
+        return fail(XRD_ERR_UNSUPPORTED,
+                    "%s: the texel-major maps, (nx+2)(ny+2)*64 floats, must stay below 4 GiB", fn);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const size_t npix = (size_t)g->nx * g->ny;
     const size_t npad = (size_t)(g->nx + 2) * (g->ny + 2);
@@ -1181,6 +1183,47 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     return XRD_OK;
 }
 
+// Buffers of the view slabs of the coefficient model with texel-major maps: per view
+// of a slab, the detector legs' line integrals and, for the map VJP, their gamma
+// (FP32, voxels x pixels x 64 each), within kSlabBudget bytes (16 GiB: 20 views of a
+// 256x256 phantom with 256 pixels, 10 with gamma) and within half of the free device
+// memory (counting the buffers the context already holds), so a slab leaves room for
+// the caller.  Sets ca.n_vox (the voxel count, read back: this synchronises the
+// stream), ca.ld_buf and ca.gam_buf, and views to the slab length; views = 0 when
+// the maps are planar, no voxel scatters, one view does not fit or an allocation
+// fails (the launches then trace the legs themselves and the map VJP scatters in
+// place: the same results up to the atomic order and the last bits of the traces).
+int coeff_slab_buffers(xrd_context *ctx, const xrd_geometry *g, CoeffArgs &ca, bool gamma,
+                       hipStream_t s, size_t &views) {
+    constexpr size_t kSlabBudget = (size_t)16 << 30;
+    views = 0;
+    // XRD_VJP_SCATTER=inplace: the launches trace the legs themselves and the map VJP
+    // scatters in place (the path of problems whose buffers do not fit; for testing)
+    const char *mode = std::getenv("XRD_VJP_SCATTER");
+    if (!ca.maps_tm || (mode && std::strcmp(mode, "inplace") == 0)) return XRD_OK;
+    int nv = 0;
+    HIP_TRY(hipMemcpyAsync(&nv, ca.nvox, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    ca.n_vox = nv;
+    const size_t floats_per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
+    const size_t per_view = floats_per_view * sizeof(float) * (gamma ? 2 : 1);
+    size_t budget = kSlabBudget;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        budget = std::min(budget, (free_b + ctx->cm_gam.bytes + ctx->cm_ld.bytes) / 2);
+    if (nv <= 0 || per_view > budget) return XRD_OK;
+    const size_t n = std::max<size_t>(1, std::min<size_t>((size_t)g->n_views, budget / per_view));
+    if (ctx->cm_ld.ensure(n * floats_per_view * sizeof(float)) != hipSuccess ||
+        (gamma && ctx->cm_gam.ensure(n * floats_per_view * sizeof(float)) != hipSuccess)) {
+        (void)hipGetLastError();  // the failed allocation is not the caller's error
+        return XRD_OK;
+    }
+    ca.ld_buf = (float *)ctx->cm_ld.ptr;
+    ca.gam_buf = gamma ? (float *)ctx->cm_gam.ptr : nullptr;
+    views = n;
+    return XRD_OK;
+}
+
 }  // namespace
 
 extern "C" int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g,
@@ -1198,7 +1241,13 @@ extern "C" int xrd_forward_coeffs(xrd_context *ctx, const xrd_geometry *g,
                          beta, ca, s, fn);
     if (rc) return rc;
     ca.out_counts = out_counts;
-    HIP_TRY(launch_coeff_forward(ca, s), "coefficient forward kernel");
+    size_t views = 0;
+    rc = coeff_slab_buffers(ctx, g, ca, false, s, views);
+    if (rc) return rc;
+    if (views > 0)
+        HIP_TRY(launch_coeff_forward_slabs(ca, (int)views, s), "coefficient forward in view slabs");
+    else
+        HIP_TRY(launch_coeff_forward(ca, s), "coefficient forward kernel");
     return XRD_OK;
 }
 
@@ -1243,39 +1292,12 @@ extern "C" int xrd_vjp_coeffs(xrd_context *ctx, const xrd_geometry *g,
             HIP_TRY(launch_zero_f64(ca.grad_maps, npix * n_basis, s), "zero kernel");
         }
     }
-    // XRD_VJP_SCATTER=inplace keeps the detector legs' scatter inside the VJP launch
-    // (the path of problems whose gamma buffer would not fit; also for testing)
-    const char *mode = std::getenv("XRD_VJP_SCATTER");
-    const bool inplace = mode && std::strcmp(mode, "inplace") == 0;
-    if (grad_maps && ca.maps_tm && !inplace) {
-        // detector legs scattered in bundles of adjacent pixels, views in slabs whose
-        // gamma buffer (FP32, voxels x pixels x 64 per view) fits in kGamBudget bytes
-        // (16 GiB: 20 views of a 256x256 phantom with 256 pixels)
-        constexpr size_t kGamBudget = (size_t)16 << 30;
-        int nv = 0;
-        HIP_TRY(hipMemcpyAsync(&nv, ca.nvox, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-        HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
-        ca.n_vox = nv;
-        const size_t per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
-        // The budget is also held to half of the device memory that is free (counting
-        // the buffer the context already holds), so the slab leaves room for the
-        // caller.  One view's gamma must fit it, and the allocation must succeed;
-        // otherwise the detector legs are scattered in place (same results up to the
-        // atomic order).
-        size_t budget = kGamBudget;
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-            budget = std::min(budget, (free_b + ctx->cm_gam.bytes) / 2);
-        if (nv > 0 && per_view * sizeof(float) <= budget) {
-            const size_t views = std::max<size_t>(
-                1, std::min<size_t>((size_t)g->n_views, budget / (per_view * sizeof(float))));
-            if (ctx->cm_gam.ensure(views * per_view * sizeof(float)) == hipSuccess) {
-                ca.gam_buf = (float *)ctx->cm_gam.ptr;
-                HIP_TRY(launch_coeff_vjp_bundled(ca, views * per_view, s), "bundled map VJP");
-                return XRD_OK;
-            }
-            (void)hipGetLastError();  // the failed allocation is not the caller's error
-        }
+    size_t views = 0;
+    rc = coeff_slab_buffers(ctx, g, ca, grad_maps != nullptr, s, views);
+    if (rc) return rc;
+    if (views > 0) {
+        HIP_TRY(launch_coeff_vjp_slabs(ca, (int)views, s), "coefficient VJP in view slabs");
+        return XRD_OK;
     }
     HIP_TRY(launch_coeff_vjp(ca, s), "coefficient VJP kernel");
     if (grad_maps && !ca.maps_tm)
@@ -1297,8 +1319,9 @@ int ct_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretization *
     if (n_basis <= 0 || n_basis > 64)
         return fail(XRD_ERR_UNSUPPORTED, "%s: need 1 <= n_basis <= 64", fn);
     if (d->n_energy > 64) return fail(XRD_ERR_UNSUPPORTED, "%s: need n_energy <= 64", fn);
-    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 >= ((size_t)1 << 31))
-        return fail(XRD_ERR_UNSUPPORTED, "%s: (nx+2)(ny+2)*64 must stay below 2^31", fn);
+    if ((size_t)(g->nx + 2) * (g->ny + 2) * 64 * sizeof(float) >= ((size_t)1 << 32))
+        return fail(XRD_ERR_UNSUPPORTED,
+                    "%s: the texel-major maps, (nx+2)(ny+2)*64 floats, must stay below 4 GiB", fn);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     HIP_TRY(ctx->cm_tm.ensure(coeff_texel_floats(g->nx, g->ny) * sizeof(float)), "workspace alloc");
     ForwardArgs fa;

@@ -169,14 +169,24 @@ struct CoeffArgs {
     // bundled scatter instead of scattering them itself (NULL: scatter in place)
     int v_base, v_count, hpart_add, n_vox;
     float *gam_buf;
+    // detector-leg line integrals of the slab's rays ([v_count x nv x n_det x 64]
+    // FP32, the layout of gam_buf), written by coeff_trace_kernel and read by the VJP
+    // launch instead of tracing the legs itself (NULL: the VJP launch traces)
+    float *ld_buf;
 };
 
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
+// Forward in view slabs of slab_views views: per slab the detector legs are traced
+// into a.ld_buf (coeff_gam_floats_per_view floats per view) and the forward launch
+// reads them; a.n_vox set.
+hipError_t launch_coeff_forward_slabs(CoeffArgs a, int slab_views, hipStream_t s);
 hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s);
-// Map VJP with the detector legs scattered in bundles of adjacent pixels: views in
-// slabs whose gamma buffer fits in gam_floats floats (a.gam_buf); a.n_vox set.
-hipError_t launch_coeff_vjp_bundled(CoeffArgs a, size_t gam_floats, hipStream_t s);
+// VJP in view slabs of slab_views views: per slab the detector legs are traced into
+// a.ld_buf, the VJP launch reads them (and, with a.gam_buf, writes gamma there for the
+// scatter of the detector legs in bundles of adjacent pixels); a.n_vox set, both
+// buffers hold coeff_gam_floats_per_view floats per view of the slab.
+hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s);
 size_t coeff_gam_floats_per_view(int n_vox, int n_det);
 size_t coeff_hpart_doubles(int n_views, int npix);
 bool coeff_texel_major(int n_basis);
