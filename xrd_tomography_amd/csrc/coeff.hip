@@ -511,16 +511,41 @@ __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &p
     }
 }
 
-// Per-(voxel, energy) table: hi/lo wvt, c2 = -log2e/(2 sigma^2), prefactor P, band.
-// Ls, Ld: the voxel's source and detector line integrals; bet: beta [B x NRG]
-// (all three staged in LDS).
-__device__ __forceinline__ void table_entry(const CoeffArgs &a, const VoxC &vs, const float *Ls,
-                                            const float *Ld, const float *bet, int e, float4 &tw,
-                                            float &band, double &P) {
+// Attenuation exponents of a voxel chunk's rays on the matrix cores:
+//   mu[j * 64 + e] = sum_b beta[b, e] (Ls[j][b] + Ld[j][b]),  j < KC voxels, e < E,
+// the basis contraction mu(E) = sum_b beta[b, E] m_b applied to the rays' line
+// integrals (Ls, Ld: [KC x MAXB] FP32; bet: beta [B x E], b-major).  Waves 0..3 of the
+// workgroup each take 16 energies (one 16 x 16 tile of mu) and step over the maps four
+// at a time with v_mfma_f64_16x16x4f64: lane l holds A[l % 16][l / 16] (voxel l % 16,
+// map b0 + l / 16), B[l / 16][l % 16] (map b0 + l / 16, energy e0 + l % 16) and
+// D[l / 16 + 4 r][l % 16], r < 4 (the output layout measured on gfx950).  The sum of
+// the two FP32 integrals and its product with an FP32 beta are exact in FP64, so only
+// the FP64 accumulation rounds, in the matrix core's fixed order.  Voxels j >= nk and
+// maps b >= B enter as zeros.  The other waves return at once; the caller's barrier
+// orders mu before its readers.
+typedef double f64x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void chunk_mu(const float *Ls, const float *Ld, const float *bet, int B,
+                                         int E, int nk, double *mu) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    const int e0 = wid * 16;
+    if (wid >= 4 || e0 >= E) return;  // uniform per wave
+    const int i = lane & 15, kq = lane >> 4, e = e0 + (lane & 15);
+    f64x4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < B; b0 += 4) {
+        const int b = b0 + kq;
+        const double av = (b < B && i < nk) ? (double)Ls[i * MAXB + b] + (double)Ld[i * MAXB + b] : 0.0;
+        const double bv = (b < B && e < E) ? (double)bet[b * E + e] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) mu[(kq + 4 * r) * 64 + e] = acc[r];
+}
+
+// Per-(voxel, energy) table: hi/lo wvt, c2 = -log2e/(2 sigma^2), prefactor P, band;
+// mu: the ray's attenuation exponent at energy e (chunk_mu).
+__device__ __forceinline__ void table_entry(const CoeffArgs &a, const VoxC &vs, double mu, int e,
+                                            float4 &tw, float &band, double &P) {
     const EnergyConst ec = a.f.energies[e];
-    double mu = 0.0;
-    for (int b = 0; b < a.n_basis; b++)
-        mu += (double)bet[b * a.f.n_energy + e] * ((double)Ls[b] + (double)Ld[b]);
     const double sig2 = vs.As + vs.Bs * ec.k2sq;
     const double w = ec.wcoef * vs.s;
     const float wh = (float)w;
@@ -703,11 +728,13 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         struct {
             float4 tw[KC * kTS];
             float tb[KC * kTS];
+            double mu[KC * 64];
         } tab;
     } ov;
     WaveList *cruns = ov.cruns;
     float4 *tw = ov.tab.tw;
     float *tb = ov.tab.tb;
+    double *mu = ov.tab.mu;
     const ForwardArgs &f = a.f;
     const int t = threadIdx.x, tq = t % TQ, h = t / TQ;
     const int V = f.n_views, E = f.n_energy, Q = f.n_q;
@@ -776,12 +803,14 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
             Fq[j * TQ + n] = s;
         }
         __syncthreads();
+        chunk_mu(Lsrc, Ld, bet, a.n_basis, E, nk, mu);
+        __syncthreads();
         for (int idx = t; idx < nk * E; idx += NT) {
             const int j = idx / E, e = idx % E;
             float4 tv;
             float band;
             double P;
-            table_entry(a, vox[j], Lsrc + j * MAXB, Ld + j * MAXB, bet, e, tv, band, P);
+            table_entry(a, vox[j], mu[j * 64 + e], e, tv, band, P);
             tw[j * kTS + e] = tv;
             tb[j * kTS + e] = band;
         }
@@ -1067,6 +1096,22 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
     }
 }
 
+// Source legs of a slab's map VJP: one wave per (view of the slab, voxel) ray from the
+// source, lane b adding gsrc_buf[(view - v_base) nv + voxel][b] (the sum of gamma over
+// the pixels, written by the VJP launch) times the leg's weights to grad_tm.
+__global__ __launch_bounds__(256) void coeff_src_scatter_kernel(CoeffArgs a) {
+    __shared__ WaveList lists[4];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    const int64_t item = (int64_t)blockIdx.x * 4 + wid;
+    if (item >= (int64_t)a.v_count * a.n_vox) return;  // uniform per wave
+    const int k = (int)(item % a.n_vox), vs = (int)(item / a.n_vox);
+    const int pix = a.vlist[k];
+    const ViewConst vc = a.f.views[a.v_base + vs];
+    const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % a.f.nx) + 0.5,
+                             (double)(pix / a.f.nx) + 0.5, a.f.nx, a.f.ny, a.f.ray_step);
+    scatter_ray_tm(a, ray, a.gsrc_buf + item * kTexelMaps, lane, lists[wid]);
+}
+
 // Lane r of sp lanes that share a ray takes a contiguous run of its samples.
 __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1) {
     const int per = (n + sp - 1) / sp;
@@ -1200,12 +1245,16 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 trace_chunk(a, pg, vox, nk, part, Ld);
             }
             __syncthreads();
+            // the exponents go to pw, which each (voxel, energy) entry then overwrites
+            // with its own W(e) P(e)
+            chunk_mu(Lsrc, Ld, bet, B, E, nk, pw);
+            __syncthreads();
             for (int idx = t; idx < nk * E; idx += NT) {
                 const int j = idx / E, e = idx % E;
                 float4 tv;
                 float band;
                 double Pe;
-                table_entry(a, vox[j], Lsrc + j * MAXB, Ld + j * MAXB, bet, e, tv, band, Pe);
+                table_entry(a, vox[j], pw[j * 64 + e], e, tv, band, Pe);
                 pw[j * 64 + e] = wt[e] * Pe;
                 tv.w = (float)(wt[e] * Pe);
                 tw[j * 64 + e] = tv;
@@ -1259,7 +1308,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 __syncthreads();
                 // scatter gamma along the detector legs, or hand it to the bundled
                 // scatter: gam_buf[view in slab][voxel][pixel][b]
-                if (a.gam_buf) {
+                if (LDBUF) {
                     for (int idx = t; idx < nk * kTexelMaps; idx += NT) {
                         const int j = idx / kTexelMaps, b = idx % kTexelMaps;
                         a.gam_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * P + p) *
@@ -1284,8 +1333,16 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
             }
             __syncthreads();
         }
-        // source legs of view v carry the sum of gamma over the pixels
-        if (maps) {
+        // source legs of view v carry the sum of gamma over the pixels; the slab form
+        // hands the sums to coeff_src_scatter_kernel, so this launch holds no ray walk
+        // (and none of its registers)
+        if (maps && LDBUF) {
+            for (int idx = t; idx < nk * kTexelMaps; idx += NT) {
+                const int j = idx / kTexelMaps, b = idx % kTexelMaps;
+                a.gsrc_buf[((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * kTexelMaps + b] =
+                    b < B ? gsrc[j * MAXB + b] : 0.0;
+            }
+        } else if (maps) {
             const ViewConst vc = f.views[v];
             if (a.grad_tm) {
                 for (int j = wid; j < nk; j += NT / kWave) {
@@ -1598,6 +1655,8 @@ hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
     const int nb = (a.nvox_cap + KC - 1) / KC;
     const int nbp = (P + kBundle - 1) / kBundle;
     if (nb <= 0 || a.n_vox <= 0) return hipSuccess;
+    // the slab launch hands every leg's gamma on: map gradients need both buffers
+    if (a.grad_maps && (!a.gam_buf || !a.gsrc_buf || !a.grad_tm)) return hipErrorInvalidValue;
     for (int v0 = 0; v0 < V; v0 += slab) {
         a.v_base = v0;
         a.v_count = std::min(slab, V - v0);
@@ -1614,6 +1673,9 @@ hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
             if (blocks > 0)
                 hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3((unsigned)blocks),
                                    dim3(kBundleWaves * kWave), 0, s, a, nbp);
+            const int64_t src_rays = (int64_t)a.v_count * a.n_vox;
+            hipLaunchKernelGGL(coeff_src_scatter_kernel, dim3((unsigned)((src_rays + 3) / 4)),
+                               dim3(256), 0, s, a);
         }
     }
     if (a.grad_coef) hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);

@@ -1206,7 +1206,10 @@ int coeff_slab_buffers(xrd_context *ctx, const xrd_geometry *g, CoeffArgs &ca, b
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     ca.n_vox = nv;
     const size_t floats_per_view = coeff_gam_floats_per_view(nv, g->n_cols * g->n_rows);
-    const size_t per_view = floats_per_view * sizeof(float) * (gamma ? 2 : 1);
+    // the gamma buffer also holds the source legs' gamma: nv x 64 doubles per view
+    const size_t gsrc_per_view = (size_t)nv * 64 * sizeof(double);
+    const size_t per_view =
+        floats_per_view * sizeof(float) * (gamma ? 2 : 1) + (gamma ? gsrc_per_view : 0);
     size_t budget = kSlabBudget;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
@@ -1214,12 +1217,16 @@ int coeff_slab_buffers(xrd_context *ctx, const xrd_geometry *g, CoeffArgs &ca, b
     if (nv <= 0 || per_view > budget) return XRD_OK;
     const size_t n = std::max<size_t>(1, std::min<size_t>((size_t)g->n_views, budget / per_view));
     if (ctx->cm_ld.ensure(n * floats_per_view * sizeof(float)) != hipSuccess ||
-        (gamma && ctx->cm_gam.ensure(n * floats_per_view * sizeof(float)) != hipSuccess)) {
+        (gamma && ctx->cm_gam.ensure(n * (floats_per_view * sizeof(float) + gsrc_per_view)) !=
+                      hipSuccess)) {
         (void)hipGetLastError();  // the failed allocation is not the caller's error
         return XRD_OK;
     }
     ca.ld_buf = (float *)ctx->cm_ld.ptr;
     ca.gam_buf = gamma ? (float *)ctx->cm_gam.ptr : nullptr;
+    // after the slab's detector-leg gamma (n * floats_per_view floats, a multiple of
+    // 64 floats, so the doubles stay aligned)
+    ca.gsrc_buf = gamma ? reinterpret_cast<double *>(ca.gam_buf + n * floats_per_view) : nullptr;
     views = n;
     return XRD_OK;
 }

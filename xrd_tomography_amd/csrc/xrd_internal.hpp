@@ -169,6 +169,10 @@ struct CoeffArgs {
     // bundled scatter instead of scattering them itself (NULL: scatter in place)
     int v_base, v_count, hpart_add, n_vox;
     float *gam_buf;
+    // with gam_buf: the source legs' gamma (the sum over the pixels) of the slab's
+    // (view, voxel) rays, [v_count x nv x 64] FP64, written by the VJP launch and
+    // scattered by coeff_src_scatter_kernel
+    double *gsrc_buf;
     // detector-leg line integrals of the slab's rays ([v_count x nv x n_det x 64]
     // FP32, the layout of gam_buf), written by coeff_trace_kernel and read by the VJP
     // launch instead of tracing the legs itself (NULL: the VJP launch traces)
