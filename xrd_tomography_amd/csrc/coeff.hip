@@ -964,12 +964,12 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
 // 64-sample window their texel lists nearly coincide.  The wave lists each ray's
 // texels (cell_runs_settled) and its run lanes add their kept corners' weights into an
 // LDS box over the window's texels, one weight column per ray; then every occupied
-// texel is emitted
-// once, in box order: lane b adds sum_i w_i gamma_i[b] (FP64, rays in order) to
-// grad_tm[texel][b].  That is one atomic instruction per texel and bundle instead of
-// one per texel and ray.  A window whose box would exceed kBoxN texels scatters each
-// ray's list directly.
+// texel is emitted once, in box order: sum_i w_i gamma_i[b] (FP64, formed for sixteen
+// texels at a time by FP64 matrix products) is added to grad_tm[texel][b].  That is one
+// atomic per texel, map and bundle instead of one per texel, map and ray.  A window
+// whose box would exceed kBoxN texels scatters each ray's list directly.
 constexpr int kBundle = 8, kBoxSide = 20, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
+static_assert(kBundle % 4 == 0, "the emission products take the rays four at a time");
 
 struct BundleBox {
     float w[kBoxN][kBundle];  // per texel of the box, one weight per ray of the bundle
@@ -1014,6 +1014,19 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
                            step
                      : 0.0;
         nmax = max(nmax, n[i]);
+    }
+    // the B operands of the emission products (bundle_emit): gb[s][t] = gamma * step of
+    // ray 4 s + lane / 16 for map 16 t + lane % 16
+    double gb[kBundle / 4][4];
+#pragma unroll
+    for (int s = 0; s < kBundle / 4; s++) {
+        const int p = bnd * kBundle + 4 * s + (lane >> 4);
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            gb[s][t] = p < P ? (double)a.gam_buf[(((size_t)vs * nv + (size_t)k) * P + p) * kTexelMaps +
+                                                  16 * t + (lane & 15)] *
+                                   step
+                             : 0.0;
     }
     const unsigned long long lower = (1ull << lane) - 1ull;
     for (int q0 = 0; q0 < nmax; q0 += kWave) {
@@ -1077,13 +1090,35 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
             cnt += __popcll(bal);
         }
         wave_lds_order();
-        for (int f = 0; f < cnt; f++) {
-            const int sl = bx.list[f];
-            double val = 0.0;
+        // sixteen texels at a time on the matrix cores: the (texel x ray) weights times
+        // the (ray x map) gamma, v_mfma_f64_16x16x4f64 over four rays per step (lane l:
+        // A[texel l % 16][ray 4 s + l / 16], B = gb[s][t], D[texel l / 16 + 4 r][map
+        // 16 t + l % 16]); each lane then adds its four texels' values for its map
+        for (int f0 = 0; f0 < cnt; f0 += 16) {
+            double wa[kBundle / 4];
+            const int fa = f0 + (lane & 15);
 #pragma unroll
-            for (int i = 0; i < kBundle; i++) val = fma((double)bx.w[sl][i], g[i], val);
-            const int ti = x0 + sl % bw, tj = y0 + sl / bw;
-            if (val != 0.0) unsafeAtomicAdd(a.grad_tm + (size_t)(ti + nx * tj) * kTexelMaps + lane, val);
+            for (int s = 0; s < kBundle / 4; s++)
+                wa[s] = fa < cnt ? (double)bx.w[bx.list[fa]][4 * s + (lane >> 4)] : 0.0;
+            size_t base[4];
+            bool live[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int fd = f0 + (lane >> 4) + 4 * r;
+                live[r] = fd < cnt;
+                const int sl = live[r] ? bx.list[fd] : 0;
+                base[r] = (size_t)((x0 + sl % bw) + nx * (y0 + sl / bw)) * kTexelMaps + (lane & 15);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                f64x4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < kBundle / 4; s++)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[s], gb[s][t], acc, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (live[r] && acc[r] != 0.0) unsafeAtomicAdd(a.grad_tm + base[r] + 16 * t, acc[r]);
+            }
         }
         wave_lds_order();
         for (int f = lane; f < cnt; f += kWave) {
