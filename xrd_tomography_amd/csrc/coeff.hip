@@ -541,6 +541,37 @@ __device__ __forceinline__ void chunk_mu(const float *Ls, const float *Ld, const
     for (int r = 0; r < 4; r++) mu[(kq + 4 * r) * 64 + e] = acc[r];
 }
 
+// Ray weights of a voxel chunk's map gradient on the matrix cores:
+//   gamma[j][b] = -sum_e beta[b, e] X[j * 64 + e],  X = W(e) P(e) S_e,
+// and gsrc[j][b] += gamma[j][b] (the source legs' sum over the pixels), for j < nk,
+// b < B (both [KC x MAXB]).  Waves 0..3 each take 16 maps and step over the energies
+// four at a time with v_mfma_f64_16x16x4f64 (the lane layout of chunk_mu, with voxels
+// as rows, energies as the inner dimension and maps as columns).  Voxels j >= nk and
+// energies e >= E enter as zeros.  The other waves return at once; the caller's barrier
+// orders the results before their readers.
+__device__ __forceinline__ void chunk_gamma(const double *X, const float *bet, int B, int E, int nk,
+                                            double *gam, double *gsrc) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    const int b0 = wid * 16;
+    if (wid >= 4 || b0 >= B) return;  // uniform per wave
+    const int j = lane & 15, kq = lane >> 4, b = b0 + (lane & 15);
+    f64x4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int e0 = 0; e0 < E; e0 += 4) {
+        const int e = e0 + kq;
+        const double av = (e < E && j < nk) ? X[j * 64 + e] : 0.0;
+        const double bv = (e < E && b < B) ? (double)bet[b * E + e] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int jr = kq + 4 * r;
+        if (jr < nk && b < B) {
+            gam[jr * MAXB + b] = -acc[r];
+            gsrc[jr * MAXB + b] -= acc[r];
+        }
+    }
+}
+
 // Per-(voxel, energy) table: hi/lo wvt, c2 = -log2e/(2 sigma^2), prefactor P, band;
 // mu: the ray's attenuation exponent at energy e (chunk_mu).
 __device__ __forceinline__ void table_entry(const CoeffArgs &a, const VoxC &vs, double mu, int e,
@@ -1325,21 +1356,20 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
             }
             __syncthreads();
             if (maps) {
-                // gamma_b = -sum_e beta[b,e] W(e) P(e) S_e; group h's waves are
-                // h*(TQ/64) .. h*(TQ/64) + TQ/64 - 1
-                for (int idx = t; idx < nk * B; idx += NT) {
-                    const int j = idx / B, b = idx % B;
-                    double g = 0.0;
-                    for (int e = 0; e < E; e++) {
-                        const int hh = e % G, i = e / G;
-                        double se = 0.0;
-                        for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++)
-                            se += (double)sp_[w][j][i];
-                        g -= (double)bet[b * E + e] * pw[j * 64 + e] * se;
-                    }
-                    gam[j * MAXB + b] = g;
-                    gsrc[j * MAXB + b] += g;
+                // gamma_b = -sum_e beta[b,e] W(e) P(e) S_e: first W(e) P(e) S_e into pw
+                // (S_e: the partials of the waves of energy e's group h, waves
+                // h*(TQ/64) .. h*(TQ/64) + TQ/64 - 1, added in wave order), then the
+                // contraction over the energies on the matrix cores
+                for (int idx = t; idx < nk * E; idx += NT) {
+                    const int j = idx / E, e = idx % E;
+                    const int hh = e % G, i = e / G;
+                    double se = 0.0;
+                    for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++)
+                        se += (double)sp_[w][j][i];
+                    pw[j * 64 + e] *= se;
                 }
+                __syncthreads();
+                chunk_gamma(pw, bet, B, E, nk, gam, gsrc);
                 __syncthreads();
                 // scatter gamma along the detector legs, or hand it to the bundled
                 // scatter: gam_buf[view in slab][voxel][pixel][b]
