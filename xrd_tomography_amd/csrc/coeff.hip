@@ -1008,6 +1008,50 @@ struct BundleBox {
     int list[kBoxN];          // the touched texels, in box order
 };
 
+// One 64-sample window of ray `col` of a bundle, added into the box: the runs of
+// samples q0 .. q0 + 63 (clamped to the ray's n samples) that share a bilinear cell,
+// each run's four corner weight sums formed in closed form by its run-end lane (as in
+// cell_runs_settled), then added to box slot (ti - x0) + bw (tj - y0) of every corner
+// (ti, tj) inside the grid, in four phases, one per stencil corner.  The cells along a
+// straight ray are all distinct, so within a phase no two lanes touch the same slot and
+// no settle against the neighbouring runs is needed: a texel that is a corner of up to
+// three runs receives their weights in corner order.  The window's box must hold every
+// in-grid corner of its runs.
+__device__ __forceinline__ void window_runs_to_box(const Ray &ray, double step, int n, int q0,
+                                                   int nx, int ny, int lane, int x0, int y0,
+                                                   int bw, BundleBox &bx, int col) {
+    const double sx = ray.ux * step, sy = ray.uy * step;
+    const int cnt = min(kWave, n - q0);
+    int i0, j0;
+    double ax, ay;
+    lane_sample(ray, step, min(q0 + lane, n - 1), i0, j0, ax, ay);
+    const int pi = __shfl_up(i0, 1, kWave), pj = __shfl_up(j0, 1, kWave);
+    const int ni = __shfl_down(i0, 1, kWave), nj = __shfl_down(j0, 1, kWave);
+    const bool first = lane == 0 || pi != i0 || pj != j0;
+    const bool last = lane < cnt && (lane == cnt - 1 || ni != i0 || nj != j0);
+    const unsigned long long starts = __ballot(first);
+    // first lane of this lane's run: the highest run start at or below it
+    const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int rs = 63 - __clzll(below);  // lane 0 starts a run, so below != 0
+    // closed-form weight sums over the run's m samples (cell_runs_settled)
+    const int m = lane - rs + 1;
+    const double nn = (double)m;
+    const double s1 = (double)((m * (m - 1)) >> 1), s2 = (double)((m - 1) * m * (2 * m - 1) / 6);
+    const double sax = nn * ax - sx * s1, say = nn * ay - sy * s1;
+    const double saxy = nn * ax * ay - (ax * sy + ay * sx) * s1 + sx * sy * s2;
+    const double w[4] = {nn - sax - say + saxy, sax - saxy, say - saxy, saxy};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int ti = i0 + (c & 1), tj = j0 + (c >> 1);
+        if (last && ti >= 0 && tj >= 0 && ti < nx && tj < ny) {
+            const int sl = (ti - x0) + bw * (tj - y0);
+            bx.w[sl][col] += (float)w[c];
+            bx.occ[sl] = 1;
+        }
+        wave_lds_order();  // this phase's writes before the next phase's reads
+    }
+}
+
 __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kernel(CoeffArgs a,
                                                                                   int nb_pix) {
     __shared__ WaveList lists[kBundleWaves];
@@ -1089,19 +1133,11 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
 #pragma unroll
         for (int i = 0; i < kBundle; i++) {
             if (q0 >= n[i]) continue;
-            SettledRun sr;
-            cell_runs_settled<kFlatBundle, 1>(ray[i], step, n[i], q0, 1, nx, ny, lane, L, sr);
             if (boxed) {
-                // each kept corner is a texel of its own within this ray's window, so the
-                // run lanes write their corners straight into the ray's box column
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    if (sr.off[c] < 0) continue;
-                    const int sl = (sr.ci + (c & 1) - x0) + bw * (sr.cj + (c >> 1) - y0);
-                    bx.w[sl][i] = (float)sr.w[c];
-                    bx.occ[sl] = 1;
-                }
+                window_runs_to_box(ray[i], step, n[i], q0, nx, ny, lane, x0, y0, bw, bx, i);
             } else {
+                SettledRun sr;
+                cell_runs_settled<kFlatBundle, 1>(ray[i], step, n[i], q0, 1, nx, ny, lane, L, sr);
                 int nblk;
                 const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
                 if (g[i] != 0.0)
