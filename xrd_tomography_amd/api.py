@@ -265,6 +265,26 @@ def forward_problem_bf16(prob, n_slices: int = 1, slice_: int = 0, device="cuda"
                             slice_, geometry=prob.geometry, disc=prob.disc, with_A=with_A, ctx=ctx)
 
 
+STREAM_SOURCES = ("csrc/matrix_stream.hpp", "csrc/mlem.hip")
+
+
+def stream_kernels_tag() -> str:
+    """First 16 hex digits of the SHA-256 of the streaming kernels' sources shipped with
+    the package ("nosrc" without them): tags launch shapes tuned for, and traffic
+    measured on, those kernels."""
+    import hashlib
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    h = hashlib.sha256()
+    for rel in STREAM_SOURCES:
+        path = os.path.join(here, rel)
+        if not os.path.exists(path):
+            return "nosrc"
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _cache_table(cache: Optional[str]) -> dict:
     import json
     import os
@@ -481,10 +501,10 @@ class MlemPlan:
 
     def autotune(self, reps: int = 3, cache: Optional[str] = None, stream=None) -> dict:
         """Pick the launch shape of the fused pass (xrd_mlem_plan_autotune).  With
-        `cache` (a JSON file path) an earlier result for the same matrix shape, dtype and
-        device is reused instead of re-timing."""
+        `cache` (a JSON file path) an earlier result for the same matrix shape, dtype,
+        device and streaming-kernel sources is reused instead of re-timing."""
         key = f"mlem|{torch.cuda.get_device_name(self.x.device)}|{self._keep[0].shape[0]}x" \
-              f"{self.n_cols}|{self._keep[0].dtype}"
+              f"{self.n_cols}|{self._keep[0].dtype}|kernels {stream_kernels_tag()}"
         hit = _cache_table(cache).get(key)
         if hit is not None:
             b, r = hit["blocks_per_cu"], hit["rows_per_wave"]

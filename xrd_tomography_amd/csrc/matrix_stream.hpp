@@ -27,6 +27,14 @@ typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2v __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
 
+// The pointers of a batched problem are read from its descriptor in memory, so the
+// compiler cannot see that they address global memory and would emit flat loads;
+// glob() states it, which gives global loads with a scalar base.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *glob(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 // Storage tags of the matrix element vector one lane loads.
 struct bf16x4 {};
 struct f16x4 {};
@@ -55,7 +63,7 @@ template <> struct Vec<float> {
     typedef f32x4v raw;
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
-            reinterpret_cast<const f32x4v *>(reinterpret_cast<const float *>(A) + e));
+            glob(reinterpret_cast<const f32x4v *>(reinterpret_cast<const float *>(A) + e)));
     }
     static __device__ __forceinline__ void unpack(const raw &v, float *a) {
         a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
@@ -66,7 +74,7 @@ template <> struct Vec<bf16x4> {
     typedef u32x2v raw;
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
-            reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e));
+            glob(reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e)));
     }
     static __device__ __forceinline__ void unpack(const raw &u, float *a) {
         unpack_bf16(u.x, a[0], a[1]);
@@ -78,7 +86,7 @@ template <> struct Vec<f16x4> {
     typedef u32x2v raw;
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
-            reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e));
+            glob(reinterpret_cast<const u32x2v *>(reinterpret_cast<const uint16_t *>(A) + e)));
     }
     static __device__ __forceinline__ void unpack(const raw &u, float *a) {
         unpack_f16(u.x, a[0], a[1]);
@@ -90,7 +98,7 @@ template <> struct Vec<bf16x8> {
     typedef u32x4v raw;
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
-            reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e));
+            glob(reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e)));
     }
     static __device__ __forceinline__ void unpack(const raw &u, float *a) {
         unpack_bf16(u.x, a[0], a[1]);
@@ -104,7 +112,7 @@ template <> struct Vec<f16x8> {
     typedef u32x4v raw;
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t) {
         return __builtin_nontemporal_load(
-            reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e));
+            glob(reinterpret_cast<const u32x4v *>(reinterpret_cast<const uint16_t *>(A) + e)));
     }
     static __device__ __forceinline__ void unpack(const raw &u, float *a) {
         unpack_f16(u.x, a[0], a[1]);
@@ -125,8 +133,8 @@ template <> struct Vec<bf24x8> {
     static __device__ __forceinline__ raw load(const void *A, int64_t e, int64_t n) {
         const uint8_t *b = reinterpret_cast<const uint8_t *>(A);
         raw v;
-        v.h = __builtin_nontemporal_load(reinterpret_cast<const u32x4v *>(b + 2 * e));
-        v.l = __builtin_nontemporal_load(reinterpret_cast<const u32x2v *>(b + 2 * n + e));
+        v.h = __builtin_nontemporal_load(glob(reinterpret_cast<const u32x4v *>(b + 2 * e)));
+        v.l = __builtin_nontemporal_load(glob(reinterpret_cast<const u32x2v *>(b + 2 * n + e)));
         return v;
     }
     static __device__ __forceinline__ void unpack(const raw &v, float *a) {
@@ -182,11 +190,16 @@ __device__ __forceinline__ void load_x(const float *x, const float *scale, int n
 }
 
 // The lane's part of rows row0 + k S (k < R), raw (rows >= r1 and groups outside
-// the matrix read as zero).
-template <typename T, int L, int NG, int R>
+// the matrix read as zero), and each row's per-row values: y and c for the MLEM
+// ratio (kColsMlem) or the weight w (kColsWeighted, in yv).  The per-row loads are
+// issued right after the row loads, unconditionally from the row index clamped to
+// r1 - 1 (the callers use them only for rows below r1), so both are in flight
+// together and a step waits for memory once.
+template <typename T, int L, int NG, int R, int MODE>
 __device__ __forceinline__ void load_rows(const void *A, int64_t n_elem, int n_cols, int64_t row0,
-                                          int64_t r1, int li, const bool *gv,
-                                          typename Vec<T>::raw (*rv)[NG]) {
+                                          int64_t r1, int li, const bool *gv, const float *y,
+                                          const float *c, const float *w,
+                                          typename Vec<T>::raw (*rv)[NG], float *yv, float *cv) {
     constexpr int W = Vec<T>::W;
     constexpr int S = kWave / L;
 #pragma unroll
@@ -198,6 +211,19 @@ __device__ __forceinline__ void load_rows(const void *A, int64_t n_elem, int n_c
             if (r < r1 && gv[g])
                 rv[k][g] = Vec<T>::load(A, r * n_cols + W * (li + L * g), n_elem);
         }
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t r = row0 + (int64_t)k * S;
+        const int64_t rc = r < r1 ? r : r1 - 1;
+        yv[k] = 0.f;
+        cv[k] = 0.f;
+        if (MODE == kColsMlem) {
+            yv[k] = *glob(y + rc);
+            cv[k] = *glob(c + rc);
+        } else if (MODE == kColsWeighted) {
+            yv[k] = *glob(w + rc);
+        }
+    }
 }
 
 // FP32 dot product of the lane's part of one raw row with x, reduced over the
@@ -243,7 +269,9 @@ __device__ __forceinline__ void stream_rows(const void *A, int64_t n_elem, int n
     for (int64_t base = r0 + (int64_t)wid * R * S; base < r1;
          base += (int64_t)kMatWaves * R * S) {
         typename Vec<T>::raw rv[R][NG];
-        load_rows<T, L, NG, R>(A, n_elem, n_cols, base + slot, r1, li, gv, rv);
+        float yv[R], cv[R];
+        load_rows<T, L, NG, R, MODE>(A, n_elem, n_cols, base + slot, r1, li, gv, y, c, w, rv, yv,
+                                     cv);
         if constexpr (kWide) {
             double ratio[R];
 #pragma unroll
@@ -253,12 +281,12 @@ __device__ __forceinline__ void stream_rows(const void *A, int64_t n_elem, int n
                     const float d = row_dot<T, L, NG>(rv[k], xv);
                     double rt = 0.0;
                     if (r < r1) {
-                        const double cr = (double)c[r];
-                        rt = ((double)y[r] - 0.0 * cr) / ((double)d + cr + 1e-20);
+                        const double cr = (double)cv[k];
+                        rt = ((double)yv[k] - 0.0 * cr) / ((double)d + cr + 1e-20);
                     }
                     ratio[k] = rt;
                 } else if (MODE == kColsWeighted) {
-                    ratio[k] = (r < r1) ? (double)w[r] : 0.0;
+                    ratio[k] = (r < r1) ? (double)yv[k] : 0.0;
                 } else {
                     ratio[k] = (r < r1) ? 1.0 : 0.0;
                 }
@@ -281,12 +309,12 @@ __device__ __forceinline__ void stream_rows(const void *A, int64_t n_elem, int n
                     const float d = row_dot<T, L, NG>(rv[k], xv);
                     float rt = 0.f;
                     if (r < r1) {
-                        const float cr = c[r];
-                        rt = (y[r] - 0.f * cr) / (d + cr + 1e-20f);
+                        const float cr = cv[k];
+                        rt = (yv[k] - 0.f * cr) / (d + cr + 1e-20f);
                     }
                     ratio[k] = rt;
                 } else if (MODE == kColsWeighted) {
-                    ratio[k] = (r < r1) ? w[r] : 0.f;
+                    ratio[k] = (r < r1) ? yv[k] : 0.f;
                 } else {
                     ratio[k] = (r < r1) ? 1.f : 0.f;
                 }

@@ -54,7 +54,9 @@ __global__ __launch_bounds__(kMatThreads) void project_kernel(const void *A, int
     for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
          base += (int64_t)kMatWaves * R * Sh::S) {
         typename Vec<T>::raw rv[R][Sh::NG];
-        load_rows<T, Sh::L, Sh::NG, R>(A, n_rows * n_cols, n_cols, base + slot, r1, li, gv, rv);
+        float yv[R], cv[R];  // no per-row values in the projection (kColsOnes)
+        load_rows<T, Sh::L, Sh::NG, R, kColsOnes>(A, n_rows * n_cols, n_cols, base + slot, r1, li,
+                                                  gv, nullptr, nullptr, nullptr, rv, yv, cv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const float d = row_dot<T, Sh::L, Sh::NG>(rv[k], xv);
@@ -125,14 +127,16 @@ __global__ __launch_bounds__(kMatThreads) void loglik_kernel(const void *A, int6
     for (int64_t base = r0 + (int64_t)wid * R * Sh::S; base < r1;
          base += (int64_t)kMatWaves * R * Sh::S) {
         typename Vec<T>::raw rv[R][Sh::NG];
-        load_rows<T, Sh::L, Sh::NG, R>(A, n_rows * n_cols, n_cols, base + slot, r1, li, gv, rv);
+        float yv[R], cv[R];
+        load_rows<T, Sh::L, Sh::NG, R, kColsMlem>(A, n_rows * n_cols, n_cols, base + slot, r1, li,
+                                                  gv, y, c, nullptr, rv, yv, cv);
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const int64_t r = base + (int64_t)k * Sh::S + slot;
             const float d = row_dot<T, Sh::L, Sh::NG>(rv[k], xv);
             if (li == 0 && r < r1) {
-                const double f = (double)d + (double)c[r] + 1e-20;
-                const double yr = (double)y[r];
+                const double f = (double)d + (double)cv[k] + 1e-20;
+                const double yr = (double)yv[k];
                 ll += (yr > 0.0 ? yr * log(f) : 0.0) - f;
             }
         }
