@@ -22,14 +22,19 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("full_mu,trace", [(True, "slabs"), (True, "inplace"), (False, "slabs")])
+@pytest.mark.parametrize("full_mu,trace", [(True, "slabs"), (True, "one-view slabs"),
+                                           (True, "inplace"), (False, "slabs")])
 def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu, trace, monkeypatch):
     """trace: with texel-major maps the detector legs are traced by their own launch
-    per slab of views (the default) or inside the forward launch
-    (XRD_VJP_SCATTER=inplace, the path of problems whose slab buffer would not fit)."""
+    per slab of views (the default; "one-view slabs" caps a slab at one view with
+    XRD_VJP_SLAB_VIEWS, so the small case runs in four slabs) or inside the forward
+    launch (XRD_VJP_SCATTER=inplace, the path of problems whose slab buffer would not
+    fit)."""
     import torch
     if trace == "inplace":
         monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
+    if trace == "one-view slabs":
+        monkeypatch.setenv("XRD_VJP_SLAB_VIEWS", "1")
     cp = _case(full_mu=full_mu)
     args = xrd.coeff_problem_tensors(cp)
     y = xrd.forward_coeffs(*args, geometry=cp.base.geometry, disc=cp.base.disc)
@@ -42,16 +47,20 @@ def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu, trace, monkeypatch):
     assert rel <= 1e-5
 
 
-@pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "inplace"),
-                                             (False, "bundled")])
+@pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "one-view slabs"),
+                                             (True, "inplace"), (False, "bundled")])
 def test_vjp_vs_oracle(xrd, oracle, full_mu, scatter, monkeypatch):
     """scatter: with texel-major maps the detector legs are traced by their own launch
     per slab of views and the map gradient's legs scattered in pixel bundles (the
-    default), or both inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of
+    default; "one-view slabs" caps a slab at one view with XRD_VJP_SLAB_VIEWS, so the
+    small case's coefficient partials, gamma and source-leg sums go through four
+    slabs), or both inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of
     problems whose slab buffers would not fit)."""
     import torch
     if scatter == "inplace":
         monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
+    if scatter == "one-view slabs":
+        monkeypatch.setenv("XRD_VJP_SLAB_VIEWS", "1")
     cp = _case(full_mu=full_mu)
     args = xrd.coeff_problem_tensors(cp)
     rng = np.random.default_rng(3)

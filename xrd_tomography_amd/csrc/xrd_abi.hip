@@ -1215,7 +1215,13 @@ int coeff_slab_buffers(xrd_context *ctx, const xrd_geometry *g, CoeffArgs &ca, b
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
         budget = std::min(budget, (free_b + ctx->cm_gam.bytes + ctx->cm_ld.bytes) / 2);
     if (nv <= 0 || per_view > budget) return XRD_OK;
-    const size_t n = std::max<size_t>(1, std::min<size_t>((size_t)g->n_views, budget / per_view));
+    size_t n = std::max<size_t>(1, std::min<size_t>((size_t)g->n_views, budget / per_view));
+    // XRD_VJP_SLAB_VIEWS=k caps the views per slab, so that a small problem runs in
+    // several slabs (for testing)
+    if (const char *cap = std::getenv("XRD_VJP_SLAB_VIEWS")) {
+        const long k = std::strtol(cap, nullptr, 10);
+        if (k > 0) n = std::min<size_t>(n, (size_t)k);
+    }
     if (ctx->cm_ld.ensure(n * floats_per_view * sizeof(float)) != hipSuccess ||
         (gamma && ctx->cm_gam.ensure(n * (floats_per_view * sizeof(float) + gsrc_per_view)) !=
                       hipSuccess)) {
