@@ -999,14 +999,22 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
 // texels at a time by FP64 matrix products) is added to grad_tm[texel][b].  That is one
 // atomic per texel, map and bundle instead of one per texel, map and ray.  A window
 // whose box would exceed kBoxN texels scatters each ray's list directly.
-constexpr int kBundle = 8, kBoxSide = 20, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
+constexpr int kBundle = 8, kBoxSide = 17, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
 static_assert(kBundle % 4 == 0, "the emission products take the rays four at a time");
 
-struct BundleBox {
-    float w[kBoxN][kBundle];  // per texel of the box, one weight per ray of the bundle
-    int occ[kBoxN];           // texel touched by some ray of the bundle in this window
-    int list[kBoxN];          // the touched texels, in box order
+// A wave's LDS: the box (per texel, one weight per ray of the bundle; a texel counts as
+// touched when one of its weights is nonzero, a corner of weight 0 adding nothing), the
+// touched texels of a window in box order, and the bundle's rays.  A window too large
+// for the box uses the start of the weight array as the run list of cell_runs_settled
+// and clears it again, so 10.7 KB per wave leave room for seven two-wave workgroups
+// per CU.
+struct alignas(16) BundleBox {
+    float w[kBoxN][kBundle];
+    int list[kBoxN];
+    Ray rays[kBundle];
 };
+static_assert(sizeof(WaveList) <= sizeof(float) * kBoxN * kBundle,
+              "the fallback run list fits the box's weights");
 
 // One 64-sample window of ray `col` of a bundle, added into the box: the runs of
 // samples q0 .. q0 + 63 (clamped to the ray's n samples) that share a bilinear cell,
@@ -1046,23 +1054,22 @@ __device__ __forceinline__ void window_runs_to_box(const Ray &ray, double step, 
         if (last && ti >= 0 && tj >= 0 && ti < nx && tj < ny) {
             const int sl = (ti - x0) + bw * (tj - y0);
             bx.w[sl][col] += (float)w[c];
-            bx.occ[sl] = 1;
         }
         wave_lds_order();  // this phase's writes before the next phase's reads
     }
 }
 
-__global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kernel(CoeffArgs a,
-                                                                                  int nb_pix) {
-    __shared__ WaveList lists[kBundleWaves];
+// Three waves per SIMD: at most 168 VGPRs.  The rays live in LDS and the per-ray loop
+// is not unrolled, so no ray's parameters are held in registers across the window.
+__global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_kernel(CoeffArgs a,
+                                                                                     int nb_pix) {
     __shared__ BundleBox boxes[kBundleWaves];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int nv = a.n_vox, P = a.f.n_det, nx = a.f.nx, ny = a.f.ny;
     const double step = a.f.ray_step;
     BundleBox &bx = boxes[wid];
-    WaveList &L = lists[wid];
+    WaveList &L = *reinterpret_cast<WaveList *>(&bx.w[0][0]);
     for (int sl = lane; sl < kBoxN; sl += kWave) {
-        bx.occ[sl] = 0;
 #pragma unroll
         for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
     }
@@ -1075,83 +1082,106 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
     const int v = a.v_base + vs;
     const int pix = a.vlist[k];
     const double vx = (double)(pix % nx) + 0.5, vy = (double)(pix / nx) + 0.5;
-    Ray ray[kBundle];
-    double g[kBundle];
-    int n[kBundle];
+    const float *gam = a.gam_buf + ((size_t)vs * nv + (size_t)k) * P * kTexelMaps;
+    // the rays (n = 0 for pixels past the detector), written by lane i
     int nmax = 0;
+    {
+        Ray r{};
+        const int p = bnd * kBundle + lane;  // pixel 0 of a bundle always exists
+        if (lane < kBundle) {
+            const PixG pg = pixel_setup(a, v, p < P ? p : bnd * kBundle);
+            r = make_ray(pg.dx, pg.dy, vx, vy, nx, ny, step);
+            if (p >= P) r.n = 0;
+            bx.rays[lane] = r;
+        }
+        int nm = lane < kBundle ? r.n : 0;
 #pragma unroll
-    for (int i = 0; i < kBundle; i++) {
-        const int p = bnd * kBundle + i;  // pixel 0 of a bundle always exists
-        const PixG pg = pixel_setup(a, v, p < P ? p : bnd * kBundle);
-        ray[i] = make_ray(pg.dx, pg.dy, vx, vy, nx, ny, step);
-        n[i] = p < P ? __builtin_amdgcn_readfirstlane(ray[i].n) : 0;
-        g[i] = p < P ? (double)a.gam_buf[(((size_t)vs * nv + (size_t)k) * P + p) * kTexelMaps + lane] *
-                           step
-                     : 0.0;
-        nmax = max(nmax, n[i]);
+        for (int o = 1; o < kBundle; o <<= 1) nm = max(nm, __shfl_xor(nm, o, kWave));
+        nmax = __builtin_amdgcn_readfirstlane(nm);
     }
-    // the B operands of the emission products (bundle_emit): gb[s][t] = gamma * step of
-    // ray 4 s + lane / 16 for map 16 t + lane % 16
+    wave_lds_order();
+    // the B operands of the emission products: gb[s][t] = gamma * step of ray
+    // 4 s + lane / 16 for map 16 t + lane % 16
     double gb[kBundle / 4][4];
 #pragma unroll
     for (int s = 0; s < kBundle / 4; s++) {
         const int p = bnd * kBundle + 4 * s + (lane >> 4);
 #pragma unroll
         for (int t = 0; t < 4; t++)
-            gb[s][t] = p < P ? (double)a.gam_buf[(((size_t)vs * nv + (size_t)k) * P + p) * kTexelMaps +
-                                                  16 * t + (lane & 15)] *
-                                   step
-                             : 0.0;
+            gb[s][t] = p < P ? (double)gam[(size_t)p * kTexelMaps + 16 * t + (lane & 15)] * step : 0.0;
     }
     const unsigned long long lower = (1ull << lane) - 1ull;
     for (int q0 = 0; q0 < nmax; q0 += kWave) {
-        // the window's cells are bounded by those of each ray's first and last sample
-        // in it (cell indices are monotone along a ray); corners add one, and only
-        // texels inside the grid are kept
+        // the window's cells are bounded by those of each ray's first and last sample in
+        // it (cell indices are monotone along a ray); corners add one, and only texels
+        // inside the grid are kept.  Lane 2 i takes ray i's first sample in the window
+        // and lane 2 i + 1 its last; the 16 lanes' cells are reduced by shuffles.
         int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
-#pragma unroll
-        for (int i = 0; i < kBundle; i++) {
-            if (q0 >= n[i]) continue;
-            const int qe[2] = {q0, min(q0 + kWave - 1, n[i] - 1)};
-#pragma unroll
-            for (int e = 0; e < 2; e++) {
+        if (lane < 2 * kBundle) {
+            const Ray rr = bx.rays[lane >> 1];
+            if (q0 < rr.n) {
                 int ci, cj;
                 double ax, ay;
-                lane_sample(ray[i], step, qe[e], ci, cj, ax, ay);
-                x0 = min(x0, ci);
-                x1 = max(x1, ci + 1);
-                y0 = min(y0, cj);
-                y1 = max(y1, cj + 1);
+                lane_sample(rr, step, (lane & 1) ? min(q0 + kWave - 1, rr.n - 1) : q0, ci, cj, ax, ay);
+                x0 = ci;
+                x1 = ci + 1;
+                y0 = cj;
+                y1 = cj + 1;
             }
         }
-        x0 = max(x0, 0);
-        y0 = max(y0, 0);
-        x1 = min(x1, nx - 1);
-        y1 = min(y1, ny - 1);
+#pragma unroll
+        for (int o = 1; o < 2 * kBundle; o <<= 1) {
+            x0 = min(x0, __shfl_xor(x0, o, kWave));
+            x1 = max(x1, __shfl_xor(x1, o, kWave));
+            y0 = min(y0, __shfl_xor(y0, o, kWave));
+            y1 = max(y1, __shfl_xor(y1, o, kWave));
+        }
+        x0 = max(__builtin_amdgcn_readfirstlane(x0), 0);
+        y0 = max(__builtin_amdgcn_readfirstlane(y0), 0);
+        x1 = min(__builtin_amdgcn_readfirstlane(x1), nx - 1);
+        y1 = min(__builtin_amdgcn_readfirstlane(y1), ny - 1);
         const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
         const bool boxed = bw > 0 && bh > 0 && bw * bh <= kBoxN;
-#pragma unroll
+#pragma unroll 1
         for (int i = 0; i < kBundle; i++) {
-            if (q0 >= n[i]) continue;
+            const Ray ray = bx.rays[i];
+            const int n = __builtin_amdgcn_readfirstlane(ray.n);
+            if (q0 >= n) continue;  // uniform
             if (boxed) {
-                window_runs_to_box(ray[i], step, n[i], q0, nx, ny, lane, x0, y0, bw, bx, i);
+                window_runs_to_box(ray, step, n, q0, nx, ny, lane, x0, y0, bw, bx, i);
             } else {
+                // a window too large for the box: this ray's texel list, one atomic per
+                // texel and map
                 SettledRun sr;
-                cell_runs_settled<kFlatBundle, 1>(ray[i], step, n[i], q0, 1, nx, ny, lane, L, sr);
+                cell_runs_settled<false, 1>(ray, step, n, q0, 1, nx, ny, lane, L, sr);
                 int nblk;
                 const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
-                if (g[i] != 0.0)
+                const int p = bnd * kBundle + i;
+                const double g = (double)gam[(size_t)p * kTexelMaps + lane] * step;
+                if (g != 0.0)
                     for (int e = 0; e < nent; e++)
-                        unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g[i] * L.tex.w[e]);
+                        unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g * L.tex.w[e]);
             }
             wave_lds_order();
         }
-        if (!boxed) continue;
-        // the touched texels in box order, then one atomic instruction each
+        if (!boxed) {
+            // the run list used the box's weights: clear them for the next window
+            float *wz = &bx.w[0][0];
+            for (int q = lane; q < (int)(sizeof(WaveList) / sizeof(float)); q += kWave) wz[q] = 0.f;
+            wave_lds_order();
+            continue;
+        }
+        // the touched texels in box order
         int cnt = 0;
         for (int s0 = 0; s0 < bw * bh; s0 += kWave) {
             const int sl = s0 + lane;
-            const bool o = sl < bw * bh && bx.occ[sl] != 0;
+            bool o = false;
+            if (sl < bw * bh) {
+                const float4 *wp = reinterpret_cast<const float4 *>(bx.w[sl]);
+                const float4 u = wp[0], t = wp[1];
+                o = u.x != 0.f || u.y != 0.f || u.z != 0.f || u.w != 0.f || t.x != 0.f ||
+                    t.y != 0.f || t.z != 0.f || t.w != 0.f;
+            }
             const unsigned long long bal = __ballot(o);
             if (o) bx.list[cnt + __popcll(bal & lower)] = sl;
             cnt += __popcll(bal);
@@ -1160,21 +1190,24 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
         // sixteen texels at a time on the matrix cores: the (texel x ray) weights times
         // the (ray x map) gamma, v_mfma_f64_16x16x4f64 over four rays per step (lane l:
         // A[texel l % 16][ray 4 s + l / 16], B = gb[s][t], D[texel l / 16 + 4 r][map
-        // 16 t + l % 16]); each lane then adds its four texels' values for its map
+        // 16 t + l % 16]); each lane then adds its four texels' values for its map.
+        // Element offsets fit 32 bits: the host keeps the texel-major maps (at least
+        // nx ny 64 floats) below 4 GiB.
         for (int f0 = 0; f0 < cnt; f0 += 16) {
             double wa[kBundle / 4];
             const int fa = f0 + (lane & 15);
 #pragma unroll
             for (int s = 0; s < kBundle / 4; s++)
                 wa[s] = fa < cnt ? (double)bx.w[bx.list[fa]][4 * s + (lane >> 4)] : 0.0;
-            size_t base[4];
+            unsigned base[4];
             bool live[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int fd = f0 + (lane >> 4) + 4 * r;
                 live[r] = fd < cnt;
                 const int sl = live[r] ? bx.list[fd] : 0;
-                base[r] = (size_t)((x0 + sl % bw) + nx * (y0 + sl / bw)) * kTexelMaps + (lane & 15);
+                base[r] = (unsigned)((x0 + sl % bw) + nx * (y0 + sl / bw)) * (unsigned)kTexelMaps +
+                          (unsigned)(lane & 15);
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -1184,13 +1217,13 @@ __global__ __launch_bounds__(kBundleWaves *kWave) void coeff_scatter_bundle_kern
                     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[s], gb[s][t], acc, 0, 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (live[r] && acc[r] != 0.0) unsafeAtomicAdd(a.grad_tm + base[r] + 16 * t, acc[r]);
+                    if (live[r] && acc[r] != 0.0)
+                        unsafeAtomicAdd(a.grad_tm + (base[r] + 16u * (unsigned)t), acc[r]);
             }
         }
         wave_lds_order();
         for (int f = lane; f < cnt; f += kWave) {
             const int sl = bx.list[f];
-            bx.occ[sl] = 0;
 #pragma unroll
             for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
         }
