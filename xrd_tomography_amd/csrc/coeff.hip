@@ -719,11 +719,21 @@ __global__ __launch_bounds__(256) void coeff_src_tm_kernel(CoeffArgs a) {
 // ld_buf[((view - v_base) nv + voxel) n_det + pixel][b].  The same trace as the
 // forward kernel's (trace_chunk_tm), so the VJP sees the forward's bits.
 constexpr int kTraceWaves = 4;
+// Launches over a slab's rays are capped at kRayLaunchBlocks workgroups: a slab of many
+// views has more rays than one dispatch can count in work-items (2^32).  The trace is
+// issued as several launches, each from item a.item_base on (its body stays a single
+// ray per wave and keeps its registers); the scatter kernels stride over their items.
+constexpr unsigned kRayLaunchBlocks = 1u << 20;
+__host__ __device__ inline unsigned ray_blocks(int64_t items, int per_block) {
+    const int64_t b = (items + per_block - 1) / per_block;
+    return (unsigned)(b < (int64_t)kRayLaunchBlocks ? b : (int64_t)kRayLaunchBlocks);
+}
+
 __global__ __launch_bounds__(kTraceWaves *kWave) void coeff_trace_kernel(CoeffArgs a) {
     __shared__ WaveList lists[kTraceWaves];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int P = a.f.n_det, nv = a.n_vox, nx = a.f.nx;
-    const int64_t item = (int64_t)blockIdx.x * kTraceWaves + wid;
+    const int64_t item = a.item_base + (int64_t)blockIdx.x * kTraceWaves + wid;
     if (item >= (int64_t)a.v_count * nv * P) return;  // uniform per wave
     const int p = (int)(item % P);
     const int k = (int)((item / P) % nv);
@@ -735,6 +745,18 @@ __global__ __launch_bounds__(kTraceWaves *kWave) void coeff_trace_kernel(CoeffAr
     const double sum = wave_trace_tm<kFlatForward, kPassesForward, false>(
         a.maps_tm, nx, a.f.ny, a.f.ray_step, ray, lane, lists[wid]);
     a.ld_buf[(size_t)item * kTexelMaps + lane] = (float)(sum * a.f.ray_step);
+}
+
+// The detector legs of a slab (a.v_base, a.v_count) into a.ld_buf, in launches of at
+// most kRayLaunchBlocks workgroups.
+static void launch_trace_slab(CoeffArgs a, hipStream_t s) {
+    const int64_t rays = (int64_t)a.v_count * a.n_vox * a.f.n_det;
+    const int64_t per_launch = (int64_t)kRayLaunchBlocks * kTraceWaves;
+    for (int64_t i0 = 0; i0 < rays; i0 += per_launch) {
+        a.item_base = i0;
+        hipLaunchKernelGGL(coeff_trace_kernel, dim3(ray_blocks(rays - i0, kTraceWaves)),
+                           dim3(kTraceWaves * kWave), 0, s, a);
+    }
 }
 
 // Forward counts: one workgroup per (view, pixel).
@@ -1074,160 +1096,164 @@ __global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_k
         for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
     }
     wave_lds_order();
-    const int64_t item = (int64_t)blockIdx.x * kBundleWaves + wid;
-    if (item >= (int64_t)a.v_count * nv * nb_pix) return;  // uniform per wave
-    const int bnd = (int)(item % nb_pix);
-    const int k = (int)((item / nb_pix) % nv);
-    const int vs = (int)(item / ((int64_t)nb_pix * nv));
-    const int v = a.v_base + vs;
-    const int pix = a.vlist[k];
-    const double vx = (double)(pix % nx) + 0.5, vy = (double)(pix / nx) + 0.5;
-    const float *gam = a.gam_buf + ((size_t)vs * nv + (size_t)k) * P * kTexelMaps;
-    // the rays (n = 0 for pixels past the detector), written by lane i
-    int nmax = 0;
-    {
-        Ray r{};
-        const int p = bnd * kBundle + lane;  // pixel 0 of a bundle always exists
-        if (lane < kBundle) {
-            const PixG pg = pixel_setup(a, v, p < P ? p : bnd * kBundle);
-            r = make_ray(pg.dx, pg.dy, vx, vy, nx, ny, step);
-            if (p >= P) r.n = 0;
-            bx.rays[lane] = r;
-        }
-        int nm = lane < kBundle ? r.n : 0;
-#pragma unroll
-        for (int o = 1; o < kBundle; o <<= 1) nm = max(nm, __shfl_xor(nm, o, kWave));
-        nmax = __builtin_amdgcn_readfirstlane(nm);
-    }
-    wave_lds_order();
-    // the B operands of the emission products: gb[s][t] = gamma * step of ray
-    // 4 s + lane / 16 for map 16 t + lane % 16
-    double gb[kBundle / 4][4];
-#pragma unroll
-    for (int s = 0; s < kBundle / 4; s++) {
-        const int p = bnd * kBundle + 4 * s + (lane >> 4);
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-            gb[s][t] = p < P ? (double)gam[(size_t)p * kTexelMaps + 16 * t + (lane & 15)] * step : 0.0;
-    }
-    const unsigned long long lower = (1ull << lane) - 1ull;
-    for (int q0 = 0; q0 < nmax; q0 += kWave) {
-        // the window's cells are bounded by those of each ray's first and last sample in
-        // it (cell indices are monotone along a ray); corners add one, and only texels
-        // inside the grid are kept.  Lane 2 i takes ray i's first sample in the window
-        // and lane 2 i + 1 its last; the 16 lanes' cells are reduced by shuffles.
-        int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
-        if (lane < 2 * kBundle) {
-            const Ray rr = bx.rays[lane >> 1];
-            if (q0 < rr.n) {
-                int ci, cj;
-                double ax, ay;
-                lane_sample(rr, step, (lane & 1) ? min(q0 + kWave - 1, rr.n - 1) : q0, ci, cj, ax, ay);
-                x0 = ci;
-                x1 = ci + 1;
-                y0 = cj;
-                y1 = cj + 1;
+    // one bundle per wave and pass; the launch is capped (ray_blocks) and the waves
+    // stride over the slab's bundles.  The box is clear again after every window.
+    const int64_t total = (int64_t)a.v_count * nv * nb_pix;
+    for (int64_t item = (int64_t)blockIdx.x * kBundleWaves + wid; item < total;
+         item += (int64_t)gridDim.x * kBundleWaves) {  // uniform per wave
+        const int bnd = (int)(item % nb_pix);
+        const int k = (int)((item / nb_pix) % nv);
+        const int vs = (int)(item / ((int64_t)nb_pix * nv));
+        const int v = a.v_base + vs;
+        const int pix = a.vlist[k];
+        const double vx = (double)(pix % nx) + 0.5, vy = (double)(pix / nx) + 0.5;
+        const float *gam = a.gam_buf + ((size_t)vs * nv + (size_t)k) * P * kTexelMaps;
+        // the rays (n = 0 for pixels past the detector), written by lane i
+        int nmax = 0;
+        {
+            Ray r{};
+            const int p = bnd * kBundle + lane;  // pixel 0 of a bundle always exists
+            if (lane < kBundle) {
+                const PixG pg = pixel_setup(a, v, p < P ? p : bnd * kBundle);
+                r = make_ray(pg.dx, pg.dy, vx, vy, nx, ny, step);
+                if (p >= P) r.n = 0;
+                bx.rays[lane] = r;
             }
-        }
+            int nm = lane < kBundle ? r.n : 0;
 #pragma unroll
-        for (int o = 1; o < 2 * kBundle; o <<= 1) {
-            x0 = min(x0, __shfl_xor(x0, o, kWave));
-            x1 = max(x1, __shfl_xor(x1, o, kWave));
-            y0 = min(y0, __shfl_xor(y0, o, kWave));
-            y1 = max(y1, __shfl_xor(y1, o, kWave));
-        }
-        x0 = max(__builtin_amdgcn_readfirstlane(x0), 0);
-        y0 = max(__builtin_amdgcn_readfirstlane(y0), 0);
-        x1 = min(__builtin_amdgcn_readfirstlane(x1), nx - 1);
-        y1 = min(__builtin_amdgcn_readfirstlane(y1), ny - 1);
-        const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-        const bool boxed = bw > 0 && bh > 0 && bw * bh <= kBoxN;
-#pragma unroll 1
-        for (int i = 0; i < kBundle; i++) {
-            const Ray ray = bx.rays[i];
-            const int n = __builtin_amdgcn_readfirstlane(ray.n);
-            if (q0 >= n) continue;  // uniform
-            if (boxed) {
-                window_runs_to_box(ray, step, n, q0, nx, ny, lane, x0, y0, bw, bx, i);
-            } else {
-                // a window too large for the box: this ray's texel list, one atomic per
-                // texel and map
-                SettledRun sr;
-                cell_runs_settled<false, 1>(ray, step, n, q0, 1, nx, ny, lane, L, sr);
-                int nblk;
-                const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
-                const int p = bnd * kBundle + i;
-                const double g = (double)gam[(size_t)p * kTexelMaps + lane] * step;
-                if (g != 0.0)
-                    for (int e = 0; e < nent; e++)
-                        unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g * L.tex.w[e]);
-            }
-            wave_lds_order();
-        }
-        if (!boxed) {
-            // the run list used the box's weights: clear them for the next window
-            float *wz = &bx.w[0][0];
-            for (int q = lane; q < (int)(sizeof(WaveList) / sizeof(float)); q += kWave) wz[q] = 0.f;
-            wave_lds_order();
-            continue;
-        }
-        // the touched texels in box order
-        int cnt = 0;
-        for (int s0 = 0; s0 < bw * bh; s0 += kWave) {
-            const int sl = s0 + lane;
-            bool o = false;
-            if (sl < bw * bh) {
-                const float4 *wp = reinterpret_cast<const float4 *>(bx.w[sl]);
-                const float4 u = wp[0], t = wp[1];
-                o = u.x != 0.f || u.y != 0.f || u.z != 0.f || u.w != 0.f || t.x != 0.f ||
-                    t.y != 0.f || t.z != 0.f || t.w != 0.f;
-            }
-            const unsigned long long bal = __ballot(o);
-            if (o) bx.list[cnt + __popcll(bal & lower)] = sl;
-            cnt += __popcll(bal);
+            for (int o = 1; o < kBundle; o <<= 1) nm = max(nm, __shfl_xor(nm, o, kWave));
+            nmax = __builtin_amdgcn_readfirstlane(nm);
         }
         wave_lds_order();
-        // sixteen texels at a time on the matrix cores: the (texel x ray) weights times
-        // the (ray x map) gamma, v_mfma_f64_16x16x4f64 over four rays per step (lane l:
-        // A[texel l % 16][ray 4 s + l / 16], B = gb[s][t], D[texel l / 16 + 4 r][map
-        // 16 t + l % 16]); each lane then adds its four texels' values for its map.
-        // Element offsets fit 32 bits: the host keeps the texel-major maps (at least
-        // nx ny 64 floats) below 4 GiB.
-        for (int f0 = 0; f0 < cnt; f0 += 16) {
-            double wa[kBundle / 4];
-            const int fa = f0 + (lane & 15);
+        // the B operands of the emission products: gb[s][t] = gamma * step of ray
+        // 4 s + lane / 16 for map 16 t + lane % 16
+        double gb[kBundle / 4][4];
 #pragma unroll
-            for (int s = 0; s < kBundle / 4; s++)
-                wa[s] = fa < cnt ? (double)bx.w[bx.list[fa]][4 * s + (lane >> 4)] : 0.0;
-            unsigned base[4];
-            bool live[4];
+        for (int s = 0; s < kBundle / 4; s++) {
+            const int p = bnd * kBundle + 4 * s + (lane >> 4);
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int fd = f0 + (lane >> 4) + 4 * r;
-                live[r] = fd < cnt;
-                const int sl = live[r] ? bx.list[fd] : 0;
-                base[r] = (unsigned)((x0 + sl % bw) + nx * (y0 + sl / bw)) * (unsigned)kTexelMaps +
-                          (unsigned)(lane & 15);
+            for (int t = 0; t < 4; t++)
+                gb[s][t] = p < P ? (double)gam[(size_t)p * kTexelMaps + 16 * t + (lane & 15)] * step : 0.0;
+        }
+        const unsigned long long lower = (1ull << lane) - 1ull;
+        for (int q0 = 0; q0 < nmax; q0 += kWave) {
+            // the window's cells are bounded by those of each ray's first and last sample in
+            // it (cell indices are monotone along a ray); corners add one, and only texels
+            // inside the grid are kept.  Lane 2 i takes ray i's first sample in the window
+            // and lane 2 i + 1 its last; the 16 lanes' cells are reduced by shuffles.
+            int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
+            if (lane < 2 * kBundle) {
+                const Ray rr = bx.rays[lane >> 1];
+                if (q0 < rr.n) {
+                    int ci, cj;
+                    double ax, ay;
+                    lane_sample(rr, step, (lane & 1) ? min(q0 + kWave - 1, rr.n - 1) : q0, ci, cj, ax, ay);
+                    x0 = ci;
+                    x1 = ci + 1;
+                    y0 = cj;
+                    y1 = cj + 1;
+                }
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                f64x4v acc = {0.0, 0.0, 0.0, 0.0};
+            for (int o = 1; o < 2 * kBundle; o <<= 1) {
+                x0 = min(x0, __shfl_xor(x0, o, kWave));
+                x1 = max(x1, __shfl_xor(x1, o, kWave));
+                y0 = min(y0, __shfl_xor(y0, o, kWave));
+                y1 = max(y1, __shfl_xor(y1, o, kWave));
+            }
+            x0 = max(__builtin_amdgcn_readfirstlane(x0), 0);
+            y0 = max(__builtin_amdgcn_readfirstlane(y0), 0);
+            x1 = min(__builtin_amdgcn_readfirstlane(x1), nx - 1);
+            y1 = min(__builtin_amdgcn_readfirstlane(y1), ny - 1);
+            const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+            const bool boxed = bw > 0 && bh > 0 && bw * bh <= kBoxN;
+#pragma unroll 1
+            for (int i = 0; i < kBundle; i++) {
+                const Ray ray = bx.rays[i];
+                const int n = __builtin_amdgcn_readfirstlane(ray.n);
+                if (q0 >= n) continue;  // uniform
+                if (boxed) {
+                    window_runs_to_box(ray, step, n, q0, nx, ny, lane, x0, y0, bw, bx, i);
+                } else {
+                    // a window too large for the box: this ray's texel list, one atomic per
+                    // texel and map
+                    SettledRun sr;
+                    cell_runs_settled<false, 1>(ray, step, n, q0, 1, nx, ny, lane, L, sr);
+                    int nblk;
+                    const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
+                    const int p = bnd * kBundle + i;
+                    const double g = (double)gam[(size_t)p * kTexelMaps + lane] * step;
+                    if (g != 0.0)
+                        for (int e = 0; e < nent; e++)
+                            unsafeAtomicAdd(a.grad_tm + L.tex.off[e] + lane, g * L.tex.w[e]);
+                }
+                wave_lds_order();
+            }
+            if (!boxed) {
+                // the run list used the box's weights: clear them for the next window
+                float *wz = &bx.w[0][0];
+                for (int q = lane; q < (int)(sizeof(WaveList) / sizeof(float)); q += kWave) wz[q] = 0.f;
+                wave_lds_order();
+                continue;
+            }
+            // the touched texels in box order
+            int cnt = 0;
+            for (int s0 = 0; s0 < bw * bh; s0 += kWave) {
+                const int sl = s0 + lane;
+                bool o = false;
+                if (sl < bw * bh) {
+                    const float4 *wp = reinterpret_cast<const float4 *>(bx.w[sl]);
+                    const float4 u = wp[0], t = wp[1];
+                    o = u.x != 0.f || u.y != 0.f || u.z != 0.f || u.w != 0.f || t.x != 0.f ||
+                        t.y != 0.f || t.z != 0.f || t.w != 0.f;
+                }
+                const unsigned long long bal = __ballot(o);
+                if (o) bx.list[cnt + __popcll(bal & lower)] = sl;
+                cnt += __popcll(bal);
+            }
+            wave_lds_order();
+            // sixteen texels at a time on the matrix cores: the (texel x ray) weights times
+            // the (ray x map) gamma, v_mfma_f64_16x16x4f64 over four rays per step (lane l:
+            // A[texel l % 16][ray 4 s + l / 16], B = gb[s][t], D[texel l / 16 + 4 r][map
+            // 16 t + l % 16]); each lane then adds its four texels' values for its map.
+            // Element offsets fit 32 bits: the host keeps the texel-major maps (at least
+            // nx ny 64 floats) below 4 GiB.
+            for (int f0 = 0; f0 < cnt; f0 += 16) {
+                double wa[kBundle / 4];
+                const int fa = f0 + (lane & 15);
 #pragma unroll
                 for (int s = 0; s < kBundle / 4; s++)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[s], gb[s][t], acc, 0, 0, 0);
+                    wa[s] = fa < cnt ? (double)bx.w[bx.list[fa]][4 * s + (lane >> 4)] : 0.0;
+                unsigned base[4];
+                bool live[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (live[r] && acc[r] != 0.0)
-                        unsafeAtomicAdd(a.grad_tm + (base[r] + 16u * (unsigned)t), acc[r]);
+                for (int r = 0; r < 4; r++) {
+                    const int fd = f0 + (lane >> 4) + 4 * r;
+                    live[r] = fd < cnt;
+                    const int sl = live[r] ? bx.list[fd] : 0;
+                    base[r] = (unsigned)((x0 + sl % bw) + nx * (y0 + sl / bw)) * (unsigned)kTexelMaps +
+                              (unsigned)(lane & 15);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    f64x4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s < kBundle / 4; s++)
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[s], gb[s][t], acc, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (live[r] && acc[r] != 0.0)
+                            unsafeAtomicAdd(a.grad_tm + (base[r] + 16u * (unsigned)t), acc[r]);
+                }
             }
-        }
-        wave_lds_order();
-        for (int f = lane; f < cnt; f += kWave) {
-            const int sl = bx.list[f];
+            wave_lds_order();
+            for (int f = lane; f < cnt; f += kWave) {
+                const int sl = bx.list[f];
 #pragma unroll
-            for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
+                for (int i = 0; i < kBundle; i++) bx.w[sl][i] = 0.f;
+            }
+            wave_lds_order();
         }
-        wave_lds_order();
     }
 }
 
@@ -1237,14 +1263,16 @@ __global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_k
 __global__ __launch_bounds__(256) void coeff_src_scatter_kernel(CoeffArgs a) {
     __shared__ WaveList lists[4];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
-    const int64_t item = (int64_t)blockIdx.x * 4 + wid;
-    if (item >= (int64_t)a.v_count * a.n_vox) return;  // uniform per wave
-    const int k = (int)(item % a.n_vox), vs = (int)(item / a.n_vox);
-    const int pix = a.vlist[k];
-    const ViewConst vc = a.f.views[a.v_base + vs];
-    const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % a.f.nx) + 0.5,
-                             (double)(pix / a.f.nx) + 0.5, a.f.nx, a.f.ny, a.f.ray_step);
-    scatter_ray_tm(a, ray, a.gsrc_buf + item * kTexelMaps, lane, lists[wid]);
+    const int64_t total = (int64_t)a.v_count * a.n_vox;
+    for (int64_t item = (int64_t)blockIdx.x * 4 + wid; item < total;
+         item += (int64_t)gridDim.x * 4) {  // uniform per wave
+        const int k = (int)(item % a.n_vox), vs = (int)(item / a.n_vox);
+        const int pix = a.vlist[k];
+        const ViewConst vc = a.f.views[a.v_base + vs];
+        const Ray ray = make_ray(vc.src_x, vc.src_y, (double)(pix % a.f.nx) + 0.5,
+                                 (double)(pix / a.f.nx) + 0.5, a.f.nx, a.f.ny, a.f.ray_step);
+        scatter_ray_tm(a, ray, a.gsrc_buf + item * kTexelMaps, lane, lists[wid]);
+    }
 }
 
 // Lane r of sp lanes that share a ray takes a contiguous run of its samples.
@@ -1734,10 +1762,7 @@ hipError_t launch_coeff_forward_slabs(CoeffArgs a, int slab_views, hipStream_t s
     for (int v0 = 0; v0 < V; v0 += slab) {
         a.v_base = v0;
         a.v_count = std::min(slab, V - v0);
-        const int64_t rays = (int64_t)a.v_count * a.n_vox * P;
-        if (rays > 0)
-            hipLaunchKernelGGL(coeff_trace_kernel, dim3((unsigned)((rays + kTraceWaves - 1) / kTraceWaves)),
-                               dim3(kTraceWaves * kWave), 0, s, a);
+        launch_trace_slab(a, s);
         hipLaunchKernelGGL(coeff_forward_kernel<true>, dim3(a.v_count * P), dim3(NT), 0, s, a);
     }
     return hipGetLastError();
@@ -1795,21 +1820,19 @@ hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
         a.v_base = v0;
         a.v_count = std::min(slab, V - v0);
         a.hpart_add = v0 > 0;
-        const int64_t rays = (int64_t)a.v_count * a.n_vox * P;
-        hipLaunchKernelGGL(coeff_trace_kernel, dim3((unsigned)((rays + kTraceWaves - 1) / kTraceWaves)),
-                           dim3(kTraceWaves * kWave), 0, s, a);
+        launch_trace_slab(a, s);
         const int ngs = std::min(ng, a.v_count);
         const int vpg = (a.v_count + ngs - 1) / ngs;
         hipLaunchKernelGGL(coeff_vjp_kernel<true>, dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
         if (a.gam_buf) {
             const int64_t items = (int64_t)a.v_count * a.n_vox * nbp;
-            const int64_t blocks = (items + kBundleWaves - 1) / kBundleWaves;
-            if (blocks > 0)
-                hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3((unsigned)blocks),
+            if (items > 0)
+                hipLaunchKernelGGL(coeff_scatter_bundle_kernel, dim3(ray_blocks(items, kBundleWaves)),
                                    dim3(kBundleWaves * kWave), 0, s, a, nbp);
             const int64_t src_rays = (int64_t)a.v_count * a.n_vox;
-            hipLaunchKernelGGL(coeff_src_scatter_kernel, dim3((unsigned)((src_rays + 3) / 4)),
-                               dim3(256), 0, s, a);
+            if (src_rays > 0)
+                hipLaunchKernelGGL(coeff_src_scatter_kernel, dim3(ray_blocks(src_rays, 4)),
+                                   dim3(256), 0, s, a);
         }
     }
     if (a.grad_coef) hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);

@@ -1185,17 +1185,17 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
 
 // Buffers of the view slabs of the coefficient model with texel-major maps: per view
 // of a slab, the detector legs' line integrals and, for the map VJP, their gamma
-// (FP32, voxels x pixels x 64 each), within kSlabBudget bytes (16 GiB: 20 views of a
-// 256x256 phantom with 256 pixels, 10 with gamma) and within half of the free device
-// memory (counting the buffers the context already holds), so a slab leaves room for
-// the caller.  Sets ca.n_vox (the voxel count, read back: this synchronises the
+// (FP32, voxels x pixels x 64 each), within kSlabBudget bytes (64 GiB: 80 views of a
+// 256x256 phantom with 256 pixels, 37 with gamma; fewer, longer slabs keep the VJP
+// launch's grid fuller) and within half of the free device memory (counting the
+// buffers the context already holds), so a slab leaves room for the caller.  Sets ca.n_vox (the voxel count, read back: this synchronises the
 // stream), ca.ld_buf and ca.gam_buf, and views to the slab length; views = 0 when
 // the maps are planar, no voxel scatters, one view does not fit or an allocation
 // fails (the launches then trace the legs themselves and the map VJP scatters in
 // place: the same results up to the atomic order and the last bits of the traces).
 int coeff_slab_buffers(xrd_context *ctx, const xrd_geometry *g, CoeffArgs &ca, bool gamma,
                        hipStream_t s, size_t &views) {
-    constexpr size_t kSlabBudget = (size_t)16 << 30;
+    constexpr size_t kSlabBudget = (size_t)64 << 30;
     views = 0;
     // XRD_VJP_SCATTER=inplace: the launches trace the legs themselves and the map VJP
     // scatters in place (the path of problems whose buffers do not fit; for testing)

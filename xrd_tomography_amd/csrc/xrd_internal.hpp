@@ -177,6 +177,8 @@ struct CoeffArgs {
     // FP32, the layout of gam_buf), written by coeff_trace_kernel and read by the VJP
     // launch instead of tracing the legs itself (NULL: the VJP launch traces)
     float *ld_buf;
+    // first ray of a coeff_trace_kernel launch within its slab (launches are capped)
+    int64_t item_base;
 };
 
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
