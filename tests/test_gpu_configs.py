@@ -182,6 +182,37 @@ def test_vjp256_dot_product_12_views(xrd):
     assert bool(torch.isfinite(gm).all())
 
 
+def test_vjp256_large_slab_matches_one_view_slabs(xrd, monkeypatch):
+    """24 views of the 256x256 phantom in one slab: 79.6 M detector legs, whose trace
+    is more than one dispatch's 2^32 work-items, so it is issued as several capped
+    launches.  Counts and the coefficient VJP must equal, bit for bit, those of
+    one-view slabs (XRD_VJP_SLAB_VIEWS=1), each far below the limit: the same rays
+    through the same kernels."""
+    import torch
+    from xrd_tomography_amd.synthetic import make_coeff_problem, make_problem
+    cp = make_coeff_problem(make_problem("fan256", n_views=24), K=16, full_mu=True)
+    g, d = cp.base.geometry, cp.base.disc
+    assert g.n_views * g.n_cols * int((cp.base.segmentation > 0).sum()) * 64 > 2**32
+    args = xrd.coeff_problem_tensors(cp)
+    u = torch.from_numpy(np.random.default_rng(6).normal(size=cp.base.n_rows)
+                         .astype(np.float32)).cuda()
+
+    def run():
+        y = xrd.forward_coeffs(*args, geometry=g, disc=d)
+        gc, _ = xrd.vjp_coeffs(*args, u, want_maps=False, geometry=g, disc=d)
+        torch.cuda.synchronize()
+        return y.cpu(), gc.cpu()
+
+    y_big, gc_big = run()
+    monkeypatch.setenv("XRD_VJP_SLAB_VIEWS", "1")
+    y_one, gc_one = run()
+    print(f"\n[vjp256 24 views] counts equal: {bool(torch.equal(y_big, y_one))}, "
+          f"coefficient VJP max |diff| {float((gc_big - gc_one).abs().max()):.3e}")
+    assert torch.equal(y_big, y_one)
+    # the coefficient partials add slab by slab, so the order of the FP64 sums differs
+    assert float((gc_big - gc_one).abs().max()) <= 1e-12 * float(gc_one.abs().max())
+
+
 # ---- the 3-D volume -----------------------------------------------------------------
 
 VOL_VIEWS = [0, 359]
