@@ -635,6 +635,9 @@ def run_volume(args, dev):
     # full-slice MLEM speed with FP32 A
     x = torch.full((ncol,), 10.0, device=dev)
     plan = api.MlemPlan(A, y, c, x)
+    tuned = None if args.no_autotune else plan.autotune(reps=2, cache=args.tune_cache or None)
+    log(f"[vol] MLEM launch: {tuned}")
+    x.fill_(10.0)
     plan.iterate(2, use_graph=True)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -654,7 +657,8 @@ def run_volume(args, dev):
         "data": "synthetic",
         "config": {"workload": f"vol192: slice {sl} of 192x192x{nz} voxels, 720 views, 256 "
                                f"pixels x 61 energies, 128 q-bins x 10 materials; 3-D "
-                               f"trilinear attenuation legs", "rows": nrow, "cols": ncol},
+                               f"trilinear attenuation legs", "rows": nrow, "cols": ncol,
+                   "launch": tuned},
         "a_build_s": round(t_build, 3),
         "a_build_bf16_tables_s": round(t_build_bf16, 3),
         "roofline": {"bound": "hbm", "achieved": round(bytes_it / ms_it / 1e6, 1),
