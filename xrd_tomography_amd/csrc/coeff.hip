@@ -493,8 +493,9 @@ __device__ __forceinline__ double wave_trace_tm(const float *__restrict__ m, int
 
 // Settle form and passes per window per kernel (cell_runs_settled): the flat form and
 // multi-pass windows where registers are to spare; the branch form and one-pass
-// windows in the VJP kernel, whose FP64 gradient totals stay live through every trace.
-constexpr bool kFlatForward = true, kFlatVjp = false, kFlatBundle = true;
+// windows in the VJP kernel, whose FP64 gradient totals stay live through every trace,
+// and in the bundled scatter's fallback for windows too large for its box.
+constexpr bool kFlatForward = true, kFlatVjp = false, kFlatBundle = false;
 constexpr int kPassesForward = kMaxPasses, kPassesVjp = 1;
 
 // Detector legs of a voxel chunk, one wave per ray, one lane per map.
@@ -1178,7 +1179,7 @@ __global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_k
                     // a window too large for the box: this ray's texel list, one atomic per
                     // texel and map
                     SettledRun sr;
-                    cell_runs_settled<false, 1>(ray, step, n, q0, 1, nx, ny, lane, L, sr);
+                    cell_runs_settled<kFlatBundle, 1>(ray, step, n, q0, 1, nx, ny, lane, L, sr);
                     int nblk;
                     const int nent = compact_texels<kListScatter>(L, lane, sr, nx, nblk);
                     const int p = bnd * kBundle + i;
