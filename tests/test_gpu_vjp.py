@@ -48,19 +48,24 @@ def test_forward_coeffs_vs_oracle(xrd, oracle, full_mu, trace, monkeypatch):
 
 
 @pytest.mark.parametrize("full_mu,scatter", [(True, "bundled"), (True, "one-view slabs"),
-                                             (True, "inplace"), (False, "bundled")])
+                                             (True, "per-ray fallback"), (True, "inplace"),
+                                             (False, "bundled")])
 def test_vjp_vs_oracle(xrd, oracle, full_mu, scatter, monkeypatch):
     """scatter: with texel-major maps the detector legs are traced by their own launch
     per slab of views and the map gradient's legs scattered in pixel bundles (the
     default; "one-view slabs" caps a slab at one view with XRD_VJP_SLAB_VIEWS, so the
     small case's coefficient partials, gamma and source-leg sums go through four
-    slabs), or both inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of
-    problems whose slab buffers would not fit)."""
+    slabs; "per-ray fallback" lets no window into the bundle's LDS box,
+    XRD_VJP_BOX_TEXELS=0, so every window takes the path of windows too large for it),
+    or both inside the VJP launch (XRD_VJP_SCATTER=inplace, the path of problems whose
+    slab buffers would not fit)."""
     import torch
     if scatter == "inplace":
         monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
     if scatter == "one-view slabs":
         monkeypatch.setenv("XRD_VJP_SLAB_VIEWS", "1")
+    if scatter == "per-ray fallback":
+        monkeypatch.setenv("XRD_VJP_BOX_TEXELS", "0")
     cp = _case(full_mu=full_mu)
     args = xrd.coeff_problem_tensors(cp)
     rng = np.random.default_rng(3)

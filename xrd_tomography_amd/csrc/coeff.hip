@@ -17,6 +17,7 @@
 // rays with FP64 atomics (order-dependent only at the 1e-16 level).
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 
 #include "xrd_device.hpp"
 #include "xrd_internal.hpp"
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_k
             x1 = min(__builtin_amdgcn_readfirstlane(x1), nx - 1);
             y1 = min(__builtin_amdgcn_readfirstlane(y1), ny - 1);
             const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-            const bool boxed = bw > 0 && bh > 0 && bw * bh <= kBoxN;
+            const bool boxed = bw > 0 && bh > 0 && bw * bh <= a.box_texels;
 #pragma unroll 1
             for (int i = 0; i < kBundle; i++) {
                 const Ray ray = bx.rays[i];
@@ -1817,6 +1818,13 @@ hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
     if (nb <= 0 || a.n_vox <= 0) return hipSuccess;
     // the slab launch hands every leg's gamma on: map gradients need both buffers
     if (a.grad_maps && (!a.gam_buf || !a.gsrc_buf || !a.grad_tm)) return hipErrorInvalidValue;
+    // XRD_VJP_BOX_TEXELS=k accumulates only windows of at most k texels in the box, so
+    // that the fallback for larger windows runs on a small problem (for testing)
+    a.box_texels = kBoxN;
+    if (const char *cap = std::getenv("XRD_VJP_BOX_TEXELS")) {
+        const long k = std::strtol(cap, nullptr, 10);
+        if (k >= 0 && k < kBoxN) a.box_texels = (int)k;
+    }
     for (int v0 = 0; v0 < V; v0 += slab) {
         a.v_base = v0;
         a.v_count = std::min(slab, V - v0);

@@ -179,6 +179,9 @@ struct CoeffArgs {
     float *ld_buf;
     // first ray of a coeff_trace_kernel launch within its slab (launches are capped)
     int64_t item_base;
+    // largest window box (texels) the bundled scatter accumulates in LDS; larger
+    // windows scatter each ray's texel list directly (set by the slab launcher)
+    int box_texels;
 };
 
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
