@@ -275,12 +275,14 @@ def test_batched_mlem_matches_single(xrd, oracle, dtype, wide):
     batch.close()
 
 
+@pytest.mark.parametrize("levels", ["-1", "6"], ids=["static chunks", "guided chunks"])
 @pytest.mark.parametrize("dtype", ["float32", "bf24"])
-def test_batched_graph_matches_eager_every_replay(xrd, dtype):
+def test_batched_graph_matches_eager_every_replay(xrd, monkeypatch, dtype, levels):
     """Two identical batches of five row-span groups, one iterated eagerly and one
     by replaying its captured graph with a synchronization between replays, agree
-    bit for bit after every iteration: each group writes its own partial slots and
-    the update sums them in a fixed order."""
+    bit for bit after every iteration: each chunk writes its own partial slots and
+    the update sums them in a fixed order, whichever workgroup took the chunk."""
+    monkeypatch.setenv("XRD_BATCH_CHUNK_LEVELS", levels)
     import torch
     shapes = [(3001, 48), (20000, 512), (4000, 1024), (12000, 256), (2500, 384), (900, 768)]
 
@@ -306,3 +308,36 @@ def test_batched_graph_matches_eager_every_replay(xrd, dtype):
             assert torch.equal(a[3], b[3])
     be.close()
     bg.close()
+
+
+def test_batched_static_and_guided_chunks_agree(xrd, monkeypatch):
+    """The same problems with one chunk per workgroup and with guided chunks taken from
+    a counter (0, 6 and 8 rounds, the last with far more chunks than 16-row blocks,
+    so many chunks are empty): x after 6 iterations agrees to FP32 rounding, and two
+    guided runs agree bit for bit."""
+    import torch
+    shapes = [(3001, 48), (20000, 512), (777, 1024), (12000, 256), (5, 16)]
+    data = []
+    for i, (r, cc) in enumerate(shapes):
+        A, y, c = _problem(60 + i, rows=r, cols=cc)
+        data.append((torch.from_numpy(A).cuda(), torch.from_numpy(y).cuda(),
+                     torch.from_numpy(c).cuda()))
+
+    def run(levels, blocks_per_cu):
+        monkeypatch.setenv("XRD_BATCH_CHUNK_LEVELS", levels)
+        probs = [(A, y, c, torch.full((A.shape[1],), 10.0, device="cuda")) for A, y, c in data]
+        b = xrd.MlemBatch(probs)
+        b.set_launch(blocks_per_cu, 1)
+        b.iterate(6, use_graph=True)
+        torch.cuda.synchronize()
+        b.close()
+        return [p[3].cpu().numpy() for p in probs]
+
+    ref = run("-1", 2)
+    for levels, bpc in (("0", 2), ("6", 2), ("8", 4)):
+        got = run(levels, bpc)
+        for xr, xg in zip(ref, got):
+            assert np.allclose(xg, xr, rtol=1e-6, atol=1e-6 * max(float(np.abs(xr).max()), 1e-30))
+    again = run("8", 4)
+    for a, b in zip(got, again):
+        assert np.array_equal(a, b)

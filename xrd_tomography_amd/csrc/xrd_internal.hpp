@@ -302,12 +302,16 @@ struct BatchSegment {
 };
 
 // Problems of one row span, streamed by one persistent launch of `grid`
-// workgroups; workgroup b streams segments wg_seg[off + b] .. wg_seg[off + b + 1]-1.
+// workgroups; chunk c is segments wg_seg[off + c] .. wg_seg[off + c + 1]-1.  Static
+// (dynamic == 0): n_chunks == grid and workgroup b streams chunk b.  Dynamic: the
+// workgroups take the n_chunks chunks in order from the group's ticket counter.
 struct BatchGroup {
     int span;
     int grid;
     int wg_seg_offset;
     int rows_per_wave;    // rows per row slot and step (1, 2 or 4)
+    int n_chunks;
+    int dynamic;
 };
 
 struct BatchArgs {
@@ -319,6 +323,7 @@ struct BatchArgs {
     int n_groups;
     BatchGroup groups[kNumSpans];  // the spans that occur, ascending
     double *partial;      // [n_segs x stride]
+    int *tickets;         // [2 x kNumSpans] chunk counters of dynamic groups, zero between launches
 };
 
 // LDS doubles of the batch kernels' block reduction (the widest row shape).
