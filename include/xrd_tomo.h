@@ -426,17 +426,21 @@ int xrd_mlem_batch_set_launch(xrd_mlem_batch *batch, int32_t blocks_per_cu,
                               int32_t rows_per_wave);
 /* For every row-span group (problems are launched per column span 256, 384, 512,
  * 768, 1024, 1536 or 2048), times that group's streaming launch (reps launches, x untouched) over
- * blocks_per_cu in {1,2,3,4} (clamped to what fits on a CU) and rows_per_wave in
- * {1,2,4} and keeps the fastest.  Returns the shape of the group with the largest
- * matrices and the mean time of a whole pass in ms (outputs may be NULL). */
+ * blocks_per_cu in {1,2,3,4} (clamped to what fits on a CU), rows_per_wave in
+ * {1,2,4} and the chunking of its rows (one chunk per workgroup, or guided chunks of
+ * 2 or 6 halving rounds taken from a counter; fixed when the environment variable
+ * XRD_BATCH_CHUNK_LEVELS was set at creation) and keeps the fastest.  Returns the shape
+ * of the group with the largest matrices and the mean time of a whole pass in ms
+ * (outputs may be NULL). */
 int xrd_mlem_batch_autotune(xrd_mlem_batch *batch, int32_t reps, void *stream,
                             int32_t *out_blocks_per_cu, int32_t *out_rows_per_wave,
                             float *out_ms);
 /* The row-span groups of the batch: writes up to max_groups entries of span,
- * persistent grid (workgroups) and rows_per_wave; returns the number of groups
- * (negative on error). */
+ * persistent grid (workgroups), rows_per_wave and chunks (equal to the grid when each
+ * workgroup streams one fixed chunk, else the number of guided chunks); returns the
+ * number of groups (negative on error).  Output arrays may be NULL. */
 int xrd_mlem_batch_groups(const xrd_mlem_batch *batch, int32_t max_groups, int32_t *out_span,
-                          int32_t *out_grid, int32_t *out_rows_per_wave);
+                          int32_t *out_grid, int32_t *out_rows_per_wave, int32_t *out_chunks);
 /* Mean duration in ms of reps streaming passes at the current launch shape,
  * measured with HIP events on stream (x untouched). */
 int xrd_mlem_batch_time_pass(xrd_mlem_batch *batch, int32_t reps, void *stream, float *out_ms);

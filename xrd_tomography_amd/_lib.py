@@ -103,7 +103,8 @@ def _declare(L):
         "xrd_mlem_batch_time_pass": (ctypes.c_int, [_vp, _i32, _vp,
                                                     ctypes.POINTER(ctypes.c_float)]),
         "xrd_mlem_batch_groups": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32),
-                                                 ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+                                                 ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                                 ctypes.POINTER(_i32)]),
         "xrd_convert_matrix_scaled": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.c_int, _vp,
                                                      _vp, _vp]),
         "xrd_mlem_plan_iterate": (ctypes.c_int, [_vp, _i32, ctypes.c_int, _vp]),

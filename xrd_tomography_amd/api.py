@@ -793,13 +793,15 @@ class MlemBatch:
         return {"groups": self.groups(), "ms_pass": ms.value}
 
     def groups(self) -> list:
-        """[{span, grid, rows_per_wave}] of the row-span groups, one launch each."""
+        """[{span, grid, rows_per_wave, chunks}] of the row-span groups, one launch each
+        (chunks == grid: one fixed chunk per workgroup; more: guided chunks)."""
         n = 8
-        sp, gr, rw = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
-        k = lib().xrd_mlem_batch_groups(self.handle, n, sp, gr, rw)
+        sp, gr, rw, ch = ((ctypes.c_int32 * n)() for _ in range(4))
+        k = lib().xrd_mlem_batch_groups(self.handle, n, sp, gr, rw, ch)
         if k < 0:
             check(k, "xrd_mlem_batch_groups")
-        return [{"span": sp[i], "grid": gr[i], "rows_per_wave": rw[i]} for i in range(min(k, n))]
+        return [{"span": sp[i], "grid": gr[i], "rows_per_wave": rw[i], "chunks": ch[i]}
+                for i in range(min(k, n))]
 
     def time_pass(self, reps: int = 10, stream=None) -> float:
         """Mean ms of one streaming pass over all problems, HIP events on the stream."""

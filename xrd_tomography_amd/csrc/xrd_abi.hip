@@ -1417,18 +1417,22 @@ extern "C" int xrd_poisson_nll(xrd_context *ctx, int64_t n, const float *counts,
 
 // ---- batched MLEM -----------------------------------------------------------------
 
+// Chunking of a group: levels < 0 one chunk per workgroup (static); L >= 0 guided
+// chunks taken from a counter, L rounds of halving sizes then a final round.  Six
+// rounds measured fastest on MI355X over a whole batch (48 BF24 phantoms, one process:
+// static 14.15 ms, L = 4: 13.98, L = 6: 13.91, L = 8: 13.97 ms per pass); groups whose
+// launch is short can do better with fewer, and the autotuner picks per group among
+// kBatchLevelChoices.  The environment variable XRD_BATCH_CHUNK_LEVELS (-1 .. 8) fixes
+// every group's levels for batches made afterwards, the autotuner's choice included.
+constexpr int kBatchLevelsDefault = 6;
+constexpr int kBatchLevelChoices[] = {-1, 2, 6};
+
 // Launch shape of one row-span group (index span_index(span)).
 struct BatchShape {
     int blocks_per_cu = 2;
     int rows_per_wave = 2;
+    int levels = kBatchLevelsDefault;
 };
-
-// Chunking of every group: < 0 one chunk per workgroup (static); L >= 0 guided
-// chunks taken from a counter, L rounds of halving sizes then a final round.  Six
-// rounds measured fastest on MI355X (batch of 48 BF24 phantoms, one process: static
-// 14.15 ms, L = 4: 13.98, L = 6: 13.91, L = 8: 13.97 ms per pass).  The environment
-// variable XRD_BATCH_CHUNK_LEVELS (-1 .. 8) overrides it for plans made afterwards.
-constexpr int kBatchLevelsDefault = 6;
 
 struct xrd_mlem_batch {
     xrd_context *ctx = nullptr;
@@ -1437,7 +1441,7 @@ struct xrd_mlem_batch {
     int esize = 4;
     std::vector<BatchProblem> host_probs;
     DevBuf probs, segs, wg_seg, partial, sens, tickets;
-    int levels = kBatchLevelsDefault;
+    bool levels_fixed = false;  // set by XRD_BATCH_CHUNK_LEVELS
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1570,8 +1574,8 @@ hipError_t batch_set_grid(xrd_mlem_batch *b) {
         gr.rows_per_wave = sh.rows_per_wave;
         gr.grid = cus * bpc;
         gr.wg_seg_offset = (int)wg.size();
-        gr.dynamic = b->levels >= 0 ? 1 : 0;
-        const std::vector<double> bounds = chunk_bounds(gr.grid, b->levels);
+        gr.dynamic = sh.levels >= 0 ? 1 : 0;
+        const std::vector<double> bounds = chunk_bounds(gr.grid, sh.levels);
         gr.n_chunks = (int)bounds.size() - 1;
         partition_group(probs, members, b->esize, bounds, segs, wg);
     }
@@ -1658,7 +1662,8 @@ extern "C" int xrd_mlem_batch_create(xrd_context *ctx, int32_t n_problems, int a
     }
     if (const char *lv = std::getenv("XRD_BATCH_CHUNK_LEVELS")) {
         const int v = std::atoi(lv);
-        b->levels = v < -1 ? -1 : (v > 8 ? 8 : v);
+        for (auto &sh : b->shape) sh.levels = v < -1 ? -1 : (v > 8 ? 8 : v);
+        b->levels_fixed = true;
     }
     hipError_t e = b->sens.ensure(sens_total * sizeof(double));
     if (e == hipSuccess) e = b->probs.ensure(probs.size() * sizeof(BatchProblem));
@@ -1781,16 +1786,20 @@ extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *st
         const int si = span_index(b->args.groups[g].span);
         BatchShape best_shape = b->shape[si];
         float best = 1e30f;
-        for (int r : rows) {
-            for (int bpc : bpcs) {
-                b->shape[si].rows_per_wave = r;
-                b->shape[si].blocks_per_cu = bpc;
-                HIP_TRY(batch_set_grid(b), "batch partition");
-                float ms = 0.f;
-                HIP_TRY(batch_time_stream(b, reps, s, &ms, g), "timed pass");
-                if (ms < best) {
-                    best = ms;
-                    best_shape = b->shape[si];
+        const int n_lv = b->levels_fixed ? 1 : (int)(sizeof(kBatchLevelChoices) / sizeof(int));
+        for (int li = 0; li < n_lv; li++) {
+            if (!b->levels_fixed) b->shape[si].levels = kBatchLevelChoices[li];
+            for (int r : rows) {
+                for (int bpc : bpcs) {
+                    b->shape[si].rows_per_wave = r;
+                    b->shape[si].blocks_per_cu = bpc;
+                    HIP_TRY(batch_set_grid(b), "batch partition");
+                    float ms = 0.f;
+                    HIP_TRY(batch_time_stream(b, reps, s, &ms, g), "timed pass");
+                    if (ms < best) {
+                        best = ms;
+                        best_shape = b->shape[si];
+                    }
                 }
             }
         }
@@ -1821,13 +1830,14 @@ extern "C" int xrd_mlem_batch_autotune(xrd_mlem_batch *b, int32_t reps, void *st
 
 extern "C" int xrd_mlem_batch_groups(const xrd_mlem_batch *b, int32_t max_groups,
                                      int32_t *out_span, int32_t *out_grid,
-                                     int32_t *out_rows_per_wave) {
+                                     int32_t *out_rows_per_wave, int32_t *out_chunks) {
     if (!b) return fail(XRD_ERR_NULL_POINTER, "xrd_mlem_batch_groups: batch is NULL");
     const int n = b->args.n_groups;
     for (int g = 0; g < n && g < max_groups; g++) {
         if (out_span) out_span[g] = b->args.groups[g].span;
         if (out_grid) out_grid[g] = b->args.groups[g].grid;
         if (out_rows_per_wave) out_rows_per_wave[g] = b->args.groups[g].rows_per_wave;
+        if (out_chunks) out_chunks[g] = b->args.groups[g].n_chunks;
     }
     return n;
 }
