@@ -1023,15 +1023,17 @@ __device__ __forceinline__ void scatter_ray_tm(const CoeffArgs &a, const Ray &ra
 // texels at a time by FP64 matrix products) is added to grad_tm[texel][b].  That is one
 // atomic per texel, map and bundle instead of one per texel, map and ray.  A window
 // whose box would exceed kBoxN texels scatters each ray's list directly.
-constexpr int kBundle = 8, kBoxSide = 17, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
+constexpr int kBundle = 16, kBoxSide = 15, kBoxN = kBoxSide * kBoxSide, kBundleWaves = 2;
 static_assert(kBundle % 4 == 0, "the emission products take the rays four at a time");
 
 // A wave's LDS: the box (per texel, one weight per ray of the bundle; a texel counts as
 // touched when one of its weights is nonzero, a corner of weight 0 adding nothing), the
 // touched texels of a window in box order, and the bundle's rays.  A window too large
 // for the box uses the start of the weight array as the run list of cell_runs_settled
-// and clears it again, so 10.7 KB per wave leave room for seven two-wave workgroups
-// per CU.
+// and clears it again, so 16.0 KB per wave leave room for five two-wave workgroups
+// per CU.  Sixteen rays per bundle and a 15 x 15 box measured fastest on MI355X (map
+// VJP of 12 views of a 256 x 256 phantom: 8 rays and a 17 x 17 box 0.82 s, 16 rays 0.73
+// s with that box and 0.71-0.72 s with 15 x 15, 24 rays 0.82 s, 32 rays 0.90 s).
 struct alignas(16) BundleBox {
     float w[kBoxN][kBundle];
     int list[kBoxN];
@@ -1205,9 +1207,11 @@ __global__ __launch_bounds__(kBundleWaves *kWave, 3) void coeff_scatter_bundle_k
                 bool o = false;
                 if (sl < bw * bh) {
                     const float4 *wp = reinterpret_cast<const float4 *>(bx.w[sl]);
-                    const float4 u = wp[0], t = wp[1];
-                    o = u.x != 0.f || u.y != 0.f || u.z != 0.f || u.w != 0.f || t.x != 0.f ||
-                        t.y != 0.f || t.z != 0.f || t.w != 0.f;
+#pragma unroll
+                    for (int h = 0; h < kBundle / 4; h++) {
+                        const float4 u = wp[h];
+                        o = o || u.x != 0.f || u.y != 0.f || u.z != 0.f || u.w != 0.f;
+                    }
                 }
                 const unsigned long long bal = __ballot(o);
                 if (o) bx.list[cnt + __popcll(bal & lower)] = sl;
