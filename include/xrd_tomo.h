@@ -357,8 +357,9 @@ int xrd_mlem_plan_create(xrd_context *ctx, const void *A, int a_dtype, int64_t n
                          int32_t n_cols, const float *col_scale, const float *y,
                          const float *c, float *x, int mode, void *stream,
                          xrd_mlem_plan **out);
-/* Run n_iter updates of x in place.  use_graph != 0 replays a captured hipGraph of
- * one iteration (captured on first use). */
+/* Run n_iter updates of x in place.  use_graph != 0 replays captured hipGraphs of one
+ * and of eight iterations (both captured on first use; n_iter / 8 launches of the
+ * second, then the rest of the first), bit for bit equal to n_iter eager updates. */
 int xrd_mlem_plan_iterate(xrd_mlem_plan *plan, int32_t n_iter, int use_graph, void *stream);
 /* Poisson log-likelihood sum_r (y log f - f), f = A x + c + 1e-20, FP64, written to
  * *out_host after synchronising `stream`. */

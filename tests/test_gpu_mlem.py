@@ -59,6 +59,8 @@ def test_mlem_vs_oracle(xrd, oracle, fused):
 
 
 def test_mlem_graph_matches_eager(xrd):
+    """Graph replays equal eager iterations bit for bit, for iteration counts served by
+    the one-iteration graph (4), the eight-iteration graph (16) and both (11)."""
     import torch
     A, y, c = _problem(4, rows=5000, cols=256)
     Ad, yd, cd = (torch.from_numpy(v).cuda() for v in (A, y, c))
@@ -66,9 +68,9 @@ def test_mlem_graph_matches_eager(xrd):
     x2 = x1.clone()
     p1 = xrd.MlemPlan(Ad, yd, cd, x1)
     p2 = xrd.MlemPlan(Ad, yd, cd, x2)
-    for _ in range(3):  # replays with eager work and syncs in between
-        p1.iterate(4, use_graph=False)
-        p2.iterate(4, use_graph=True)
+    for n in (4, 11, 16, 4):  # replays with eager work and syncs in between
+        p1.iterate(n, use_graph=False)
+        p2.iterate(n, use_graph=True)
         torch.cuda.synchronize()
         assert torch.equal(x1, x2)
     p1.close()

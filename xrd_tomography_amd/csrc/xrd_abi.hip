@@ -130,10 +130,24 @@ struct xrd_mlem_plan {
     int mode = XRD_MLEM_FUSED;
     int nblocks = 0;
     DevBuf sens, partial, f, r, llp;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // captured iterations: one, and kPlanGraphIters in a row (one graph launch per
+    // kPlanGraphIters iterations leaves no launch gap between them)
+    hipGraph_t graph = nullptr, graph_multi = nullptr;
+    hipGraphExec_t exec = nullptr, exec_multi = nullptr;
     hipStream_t graph_stream = nullptr;
 };
+
+constexpr int kPlanGraphIters = 8;
+
+// Drops the captured iterations (they hold the launch shape they were captured with).
+static void plan_drop_graphs(xrd_mlem_plan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->exec_multi) (void)hipGraphExecDestroy(p->exec_multi);
+    if (p->graph_multi) (void)hipGraphDestroy(p->graph_multi);
+    p->exec = p->exec_multi = nullptr;
+    p->graph = p->graph_multi = nullptr;
+}
 
 extern "C" {
 
@@ -924,20 +938,33 @@ extern "C" int xrd_mlem_plan_iterate(xrd_mlem_plan *p, int32_t n_iter, int use_g
         return XRD_OK;
     }
     if (!p->exec) {
-        // capture one iteration on a private stream, replay it n_iter times
+        // capture one iteration, and kPlanGraphIters iterations, on a private stream
         hipStream_t cs;
         HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            hipError_t e2 = mlem_one_iteration(p, cs);
-            e = hipStreamEndCapture(cs, &p->graph);
-            if (e2 != hipSuccess) e = e2;
+        hipError_t e = hipSuccess;
+        for (int g = 0; g < 2 && e == hipSuccess; g++) {
+            const int iters = g == 0 ? 1 : kPlanGraphIters;
+            hipGraph_t &graph = g == 0 ? p->graph : p->graph_multi;
+            hipGraphExec_t &exec = g == 0 ? p->exec : p->exec_multi;
+            e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t e2 = hipSuccess;
+                for (int i = 0; i < iters && e2 == hipSuccess; i++) e2 = mlem_one_iteration(p, cs);
+                e = hipStreamEndCapture(cs, &graph);
+                if (e2 != hipSuccess) e = e2;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
         (void)hipStreamDestroy(cs);
-        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        if (e != hipSuccess) {
+            plan_drop_graphs(p);
+            return hip_fail(e, "graph capture");
+        }
     }
-    for (int i = 0; i < n_iter; i++) HIP_TRY(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
+    int left = n_iter;
+    for (; left >= kPlanGraphIters; left -= kPlanGraphIters)
+        HIP_TRY(hipGraphLaunch(p->exec_multi, s), "hipGraphLaunch");
+    for (; left > 0; left--) HIP_TRY(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
     return XRD_OK;
 }
 
@@ -1004,14 +1031,7 @@ extern "C" int xrd_mlem_plan_autotune(xrd_mlem_plan *p, int32_t reps, void *stre
     p->m.blocks_per_cu = best_b;
     p->m.rows_per_wave = best_r;
     p->nblocks = mat_num_blocks(p->m);
-    if (p->exec) {  // the captured iteration used the old launch shape
-        (void)hipGraphExecDestroy(p->exec);
-        p->exec = nullptr;
-    }
-    if (p->graph) {
-        (void)hipGraphDestroy(p->graph);
-        p->graph = nullptr;
-    }
+    plan_drop_graphs(p);  // the captured iterations used the old launch shape
     if (out_blocks_per_cu) *out_blocks_per_cu = best_b;
     if (out_rows_per_wave) *out_rows_per_wave = best_r;
     if (out_ms) *out_ms = best;
@@ -1063,14 +1083,7 @@ extern "C" int xrd_mlem_plan_set_launch(xrd_mlem_plan *p, int32_t blocks_per_cu,
     p->m.blocks_per_cu = blocks_per_cu;
     p->m.rows_per_wave = rows_per_wave;
     p->nblocks = mat_num_blocks(p->m);
-    if (p->exec) {
-        (void)hipGraphExecDestroy(p->exec);
-        p->exec = nullptr;
-    }
-    if (p->graph) {
-        (void)hipGraphDestroy(p->graph);
-        p->graph = nullptr;
-    }
+    plan_drop_graphs(p);
     return XRD_OK;
 }
 
@@ -1123,8 +1136,7 @@ extern "C" const double *xrd_mlem_plan_sensitivity(const xrd_mlem_plan *p) {
 extern "C" int xrd_mlem_plan_destroy(xrd_mlem_plan *p) {
     if (!p) return XRD_OK;
     (void)hipSetDevice(p->ctx->device);
-    if (p->exec) (void)hipGraphExecDestroy(p->exec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
+    plan_drop_graphs(p);
     p->sens.release();
     p->partial.release();
     p->f.release();
