@@ -524,8 +524,24 @@ __global__ __launch_bounds__(G * TQ) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const double Ed1 = a.e_min + (double)ced * a.e_width;
                 const double Ed2 = a.e_min + (double)(ced + 1) * a.e_width;
                 for (int j = cgrp; j < nk; j += Gc) {
-                    for (int e = wlo; e <= whi; e++) {
-                        const double2 eo = teo[j * E + e];
+                    // The shifted source bins [es1, es2] / er(e) rise strictly with e (the
+                    // edges by about a bin width, far above FP64 rounding), so the bins that
+                    // overlap [Ed1, Ed2] are those from the first with eo2 > Ed1 up to the
+                    // last with eo1 < Ed2.  Both ends are found by a four-step search
+                    // (windows hold at most 2 kMaxEnergyHalfwidth + 1 = 15 bins), the same
+                    // number of steps on every lane, and the terms are added in the same
+                    // ascending order, with the same arithmetic, as the full window.
+                    const double2 *te = teo + j * E;
+                    int e_lo = wlo, e_end = wlo;
+#pragma unroll
+                    for (int step = 8; step > 0; step >>= 1) {
+                        const int c = e_lo + step - 1;
+                        if (c <= whi && !(te[c].y > Ed1)) e_lo += step;
+                        const int d = e_end + step - 1;
+                        if (d <= whi && te[d].x < Ed2) e_end += step;
+                    }
+                    for (int e = e_lo; e < e_end; e++) {
+                        const double2 eo = te[e];
                         const double lo_e = eo.x > Ed1 ? eo.x : Ed1;
                         const double hi_e = eo.y < Ed2 ? eo.y : Ed2;
                         const double cm = hi_e - lo_e;

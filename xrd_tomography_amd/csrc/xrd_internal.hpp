@@ -61,6 +61,7 @@ struct ViewConst {
 // Largest detector-response half width (DNRG) the forward kernels support: the
 // window of 2 DNRG + 1 source bins is unrolled.
 constexpr int kMaxEnergyHalfwidth = 7;
+static_assert(2 * kMaxEnergyHalfwidth + 1 <= 15, "the Compton window search takes four steps");
 
 struct ForwardArgs {
     // sizes
