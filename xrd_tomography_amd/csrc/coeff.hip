@@ -731,32 +731,52 @@ __host__ __device__ inline unsigned ray_blocks(int64_t items, int per_block) {
     return (unsigned)(b < (int64_t)kRayLaunchBlocks ? b : (int64_t)kRayLaunchBlocks);
 }
 
+// One wave traces kWave consecutive rays: lane l first sets up ray base + l (pixel
+// geometry and the FP64 intersection with the grid, the same arithmetic as one wave per
+// ray, in parallel over the lanes) into LDS, then the wave traces the rays one after
+// another.  Each ray's line integrals are the same bits as when a wave computed the ray
+// itself.
 __global__ __launch_bounds__(kTraceWaves *kWave) void coeff_trace_kernel(CoeffArgs a) {
     __shared__ WaveList lists[kTraceWaves];
+    __shared__ Ray rays[kTraceWaves][kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
     const int P = a.f.n_det, nv = a.n_vox, nx = a.f.nx;
-    const int64_t item = a.item_base + (int64_t)blockIdx.x * kTraceWaves + wid;
-    if (item >= (int64_t)a.v_count * nv * P) return;  // uniform per wave
-    const int p = (int)(item % P);
-    const int k = (int)((item / P) % nv);
-    const int vs = (int)(item / ((int64_t)P * nv));
-    const PixG pg = pixel_setup(a, a.v_base + vs, p);
-    const int pix = a.vlist[k];
-    const Ray ray = make_ray(pg.dx, pg.dy, (double)(pix % nx) + 0.5, (double)(pix / nx) + 0.5, nx,
-                             a.f.ny, a.f.ray_step);
-    const double sum = wave_trace_tm<kFlatForward, kPassesForward, false>(
-        a.maps_tm, nx, a.f.ny, a.f.ray_step, ray, lane, lists[wid]);
-    a.ld_buf[(size_t)item * kTexelMaps + lane] = (float)(sum * a.f.ray_step);
+    const int64_t total = (int64_t)a.v_count * nv * P;
+    const int64_t base = a.item_base + ((int64_t)blockIdx.x * kTraceWaves + wid) * kWave;
+    if (base >= total) return;  // uniform per wave
+    {
+        const int64_t item = base + lane;
+        Ray r{};
+        if (item < total) {
+            const int p = (int)(item % P);
+            const int k = (int)((item / P) % nv);
+            const int vs = (int)(item / ((int64_t)P * nv));
+            const PixG pg = pixel_setup(a, a.v_base + vs, p);
+            const int pix = a.vlist[k];
+            r = make_ray(pg.dx, pg.dy, (double)(pix % nx) + 0.5, (double)(pix / nx) + 0.5, nx,
+                         a.f.ny, a.f.ray_step);
+        }
+        rays[wid][lane] = r;
+    }
+    wave_lds_order();
+    const int cnt = (int)min((int64_t)kWave, total - base);
+    for (int i = 0; i < cnt; i++) {
+        const Ray ray = rays[wid][i];
+        const double sum = wave_trace_tm<kFlatForward, kPassesForward, false>(
+            a.maps_tm, nx, a.f.ny, a.f.ray_step, ray, lane, lists[wid]);
+        a.ld_buf[(size_t)(base + i) * kTexelMaps + lane] = (float)(sum * a.f.ray_step);
+    }
 }
 
 // The detector legs of a slab (a.v_base, a.v_count) into a.ld_buf, in launches of at
-// most kRayLaunchBlocks workgroups.
+// most kRayLaunchBlocks workgroups of kTraceWaves waves of kWave rays.
 static void launch_trace_slab(CoeffArgs a, hipStream_t s) {
     const int64_t rays = (int64_t)a.v_count * a.n_vox * a.f.n_det;
-    const int64_t per_launch = (int64_t)kRayLaunchBlocks * kTraceWaves;
+    const int64_t per_block = (int64_t)kTraceWaves * kWave;
+    const int64_t per_launch = (int64_t)kRayLaunchBlocks * per_block;
     for (int64_t i0 = 0; i0 < rays; i0 += per_launch) {
         a.item_base = i0;
-        hipLaunchKernelGGL(coeff_trace_kernel, dim3(ray_blocks(rays - i0, kTraceWaves)),
+        hipLaunchKernelGGL(coeff_trace_kernel, dim3(ray_blocks(rays - i0, (int)per_block)),
                            dim3(kTraceWaves * kWave), 0, s, a);
     }
 }
