@@ -29,6 +29,7 @@ namespace {
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kRs2pi = 0.39894228040143267794;
 constexpr int TQ = 128, G = 4, NT = TQ * G, EPT = 16, KC = 16;
+static_assert(TQ == kCoeffFqCols, "fq_buf rows hold the kernels' q lanes");
 constexpr int MAXB = 64, MAXK = 32;
 
 struct VoxC {
@@ -867,16 +868,8 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         } else {
             trace_chunk(a, pg, vox, nk, part, Ld);
         }
-        // F_vox(q_n) for the chunk
-        for (int idx = t; idx < nk * TQ; idx += NT) {
-            const int j = idx / TQ, n = idx % TQ;
-            float s = 0.f;
-            if (n < Q) {
-                const float *cf = a.coef + (size_t)vox[j].pix * a.n_coef;
-                for (int kk = 0; kk < a.n_coef; kk++) s = fmaf(cf[kk], a.basis[kk * Q + n], s);
-            }
-            Fq[j * TQ + n] = s;
-        }
+        // F_vox(q_n) of the chunk, formed once per call (coeff_fq_kernel)
+        for (int idx = t; idx < nk * TQ; idx += NT) Fq[idx] = a.fq_buf[(size_t)k0 * TQ + idx];
         __syncthreads();
         chunk_mu(Lsrc, Ld, bet, a.n_basis, E, nk, mu);
         __syncthreads();
@@ -1375,17 +1368,8 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
         lo = fminf(lo, __shfl_xor(lo, o, kWave));
         hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
     }
-    // coefficients of the chunk are fixed: F_vox(q_n) once
-    for (int idx = t; idx < nk * TQ; idx += NT) {
-        const int j = idx / TQ, n = idx % TQ;
-        float sv = 0.f;
-        if (n < Q) {
-            const int pix = a.vlist[k0 + j];
-            const float *cf = a.coef + (size_t)pix * a.n_coef;
-            for (int kk = 0; kk < a.n_coef; kk++) sv = fmaf(cf[kk], a.basis[kk * Q + n], sv);
-        }
-        Fq[j * TQ + n] = sv;
-    }
+    // F_vox(q_n) of the chunk, formed once per call (coeff_fq_kernel)
+    for (int idx = t; idx < nk * TQ; idx += NT) Fq[idx] = a.fq_buf[(size_t)k0 * TQ + idx];
     double ht[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) ht[j] = 0.0;
@@ -1758,8 +1742,27 @@ hipError_t launch_poisson_nll(int64_t n, const float *cnt, const float *bg, cons
     return hipGetLastError();
 }
 
+// F_vox(q_n) = sum_k coef[vox, k] basis[k, n] of every voxel of the list, once per call
+// (one FP32 fma chain over the coefficients in order), for the forward and VJP kernels,
+// which would otherwise form it again in every (view, pixel) or view-group workgroup.
+__global__ __launch_bounds__(256) void coeff_fq_kernel(CoeffArgs a) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)a.nvox_cap * TQ) return;
+    const int k = (int)(idx / TQ), n = (int)(idx % TQ);
+    if (k >= a.nvox[0]) return;
+    const int Q = a.f.n_q;
+    float s = 0.f;
+    if (n < Q) {
+        const float *cf = a.coef + (size_t)a.vlist[k] * a.n_coef;
+        for (int kk = 0; kk < a.n_coef; kk++) s = fmaf(cf[kk], a.basis[kk * Q + n], s);
+    }
+    a.fq_buf[idx] = s;
+}
+
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(coeff_list_kernel, dim3(1), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(coeff_fq_kernel, dim3((unsigned)(((int64_t)a.nvox_cap * TQ + 255) / 256)),
+                       dim3(256), 0, s, a);
     const size_t n = (size_t)(a.f.nx + 2) * (a.f.ny + 2) * a.n_basis;
     unsigned blocks = (unsigned)((n + 255) / 256);
     if (blocks > 8192) blocks = 8192;

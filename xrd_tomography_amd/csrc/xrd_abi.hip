@@ -110,7 +110,7 @@ struct xrd_context {
     // explicit-matrix partials
     DevBuf partial, small;
     // coefficient-model workspace
-    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm, cm_gam, cm_ld;
+    DevBuf cm_maps, cm_src, cm_vlist, cm_h, cm_tm, cm_gtm, cm_gam, cm_ld, cm_fq;
     // pinned staging for host-computed per-view / per-energy tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -251,7 +251,7 @@ int xrd_context_destroy(xrd_context *c) {
     for (DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels, &c->offsets,
                       &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial, &c->small, &c->cm_maps,
                       &c->cm_src, &c->cm_vlist, &c->cm_h, &c->cm_tm, &c->cm_gtm, &c->cm_gam,
-                      &c->cm_ld})
+                      &c->cm_ld, &c->cm_fq})
         b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
@@ -266,7 +266,7 @@ int64_t xrd_context_workspace_bytes(const xrd_context *c) {
     for (const DevBuf *b : {&c->mu_padded, &c->s2u8, &c->cpx, &c->ref_list, &c->voxels,
                             &c->offsets, &c->src_int, &c->tables, &c->pv, &c->mu_vol, &c->partial,
                             &c->small, &c->cm_maps, &c->cm_src, &c->cm_vlist, &c->cm_h,
-                            &c->cm_tm, &c->cm_gtm, &c->cm_gam, &c->cm_ld})
+                            &c->cm_tm, &c->cm_gtm, &c->cm_gam, &c->cm_ld, &c->cm_fq})
         s += (int64_t)b->bytes;
     return s;
 }
@@ -1171,6 +1171,7 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     HIP_TRY(ctx->cm_maps.ensure(npad * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_src.ensure((size_t)g->n_views * npix * n_basis * sizeof(float)), "workspace alloc");
     HIP_TRY(ctx->cm_vlist.ensure((npix + 4) * sizeof(int)), "workspace alloc");
+    HIP_TRY(ctx->cm_fq.ensure(npix * kCoeffFqCols * sizeof(float)), "workspace alloc");
     const bool tm = coeff_texel_major(n_basis);
     if (tm) HIP_TRY(ctx->cm_tm.ensure(coeff_texel_floats(g->nx, g->ny) * sizeof(float)), "workspace alloc");
     ForwardArgs fa;
@@ -1191,6 +1192,7 @@ int coeff_setup(xrd_context *ctx, const xrd_geometry *g, const xrd_discretizatio
     ca.nvox = (int *)ctx->cm_vlist.ptr + npix;
     ca.nvox_cap = (int)npix;
     ca.maps_tm = tm ? (float *)ctx->cm_tm.ptr : nullptr;
+    ca.fq_buf = (float *)ctx->cm_fq.ptr;
     HIP_TRY(launch_coeff_prepare(ca, s), "coefficient-model preparation");
     return XRD_OK;
 }

@@ -183,8 +183,13 @@ struct CoeffArgs {
     // largest window box (texels) the bundled scatter accumulates in LDS; larger
     // windows scatter each ray's texel list directly (set by the slab launcher)
     int box_texels;
+    // F_vox(q_n) = sum_k coef[vox,k] basis[k,n] per voxel-list index, [nx*ny x 128]
+    // (zero for n >= n_q), formed once per call by coeff_fq_kernel for the forward
+    float *fq_buf;
 };
 
+// Columns of CoeffArgs::fq_buf (the coefficient kernels' q-lane count).
+constexpr int kCoeffFqCols = 128;
 hipError_t launch_coeff_prepare(const CoeffArgs &a, hipStream_t s);
 hipError_t launch_coeff_forward(const CoeffArgs &a, hipStream_t s);
 // Forward in view slabs of slab_views views: per slab the detector legs are traced
