@@ -1323,14 +1323,17 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
     __shared__ double gsrc[KC * MAXB];
     __shared__ double wt[64];
     __shared__ float Lsrc[KC * MAXB];
+    // table rows of kTS entries: the ballots' lane-indexed reads of one energy across
+    // the chunk's voxels then fall into different banks
+    constexpr int kTS = 65;
     __shared__ union {
         struct {
             WaveList cruns[NT / kWave];
             float2 part[NT];
         } trace;
         struct {
-            float4 tw[KC * 64];
-            float tb[KC * 64];
+            float4 tw[KC * kTS];
+            float tb[KC * kTS];
             double pw[KC * 64];                // W(e) P(e)
             float sp_[NT / kWave][KC][16];     // per-wave partial S
         } gauss;
@@ -1430,10 +1433,24 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 table_entry(a, vox[j], pw[j * 64 + e], e, tv, band, Pe);
                 pw[j * 64 + e] = wt[e] * Pe;
                 tv.w = (float)(wt[e] * Pe);
-                tw[j * 64 + e] = tv;
-                tb[j * 64 + e] = band;
+                tw[j * kTS + e] = tv;
+                tb[j * kTS + e] = band;
             }
             __syncthreads();
+            // per energy slot, one ballot gives the chunk's voxels whose band reaches
+            // this wave's bins (the range and the energy are wave-uniform); the voxel
+            // loop then reads only those entries
+            uint32_t inb[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int e = h + G * i;
+                bool in = false;
+                if (i < EPT && e < E && lane < nk) {
+                    const float x = tw[lane * kTS + e].x, band = tb[lane * kTS + e];
+                    in = x + band >= lo && x - band <= hi;
+                }
+                inb[i] = (uint32_t)__ballot(in);
+            }
             // h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n; S_e partials over this wave's bins
 #pragma unroll
             for (int j = 0; j < KC; j++) {
@@ -1445,10 +1462,8 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 for (int i = 0; i < 16; i++) {
                     sv[i] = 0.f;
                     const int e = h + G * i;
-                    if (i >= EPT || e >= E) continue;
-                    const float4 tv = tw[j * 64 + e];
-                    const float band = tb[j * 64 + e];
-                    if (tv.x + band >= lo && tv.x - band <= hi) {
+                    if ((inb[i] >> j) & 1u) {
+                        const float4 tv = tw[j * kTS + e];
                         const float d = (tv.x - qch) + (tv.y - qcl);
                         const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
                         hp = fmaf(tv.w, g, hp);
