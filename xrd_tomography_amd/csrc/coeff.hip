@@ -1314,7 +1314,10 @@ __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1)
 // coefficient VJP 1.11 -> 0.85 s): the trace's lists and the table/Gaussian arrays
 // share their LDS space (live in different phases of each pixel), and beta is read
 // from global memory, where its 16 KB stay in L1.
-template <bool LDBUF>
+// MAPS false: the coefficient-only form, compiled without the S partials (100 against
+// 188 B/lane of scratch; 12 views 0.303 -> 0.294 s).  MAPS true still tests a.grad_maps
+// at run time: with the test constant the map form spilled more and measured 1% slower.
+template <bool LDBUF, bool MAPS>
 __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
     __shared__ VoxC vox[KC];
     __shared__ float Ld[KC * MAXB];
@@ -1357,7 +1360,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
     // views [v_base, v_base + v_count) of a slab (v_count 0: every view)
     const int vend = a.v_count > 0 ? a.v_base + a.v_count : V;
     const int va = a.v_base + blockIdx.y * views_per_group, vb = min(vend, va + views_per_group);
-    const bool maps = a.grad_maps != nullptr;
+    const bool maps = MAPS && a.grad_maps != nullptr;
     float qch = 1e30f, qcl = 0.f, dqc = 0.f;
     if (tq < Q) {
         const double q = f.q_coarse[tq];
@@ -1835,7 +1838,10 @@ hipError_t launch_coeff_vjp(const CoeffArgs &a, hipStream_t s) {
     const int vpg = (a.f.n_views + ng - 1) / ng;
     const int nb = (a.nvox_cap + KC - 1) / KC;
     if (nb > 0) {
-        hipLaunchKernelGGL(coeff_vjp_kernel<false>, dim3(nb, ng), dim3(NT), 0, s, a, vpg);
+        if (a.grad_maps)
+            hipLaunchKernelGGL((coeff_vjp_kernel<false, true>), dim3(nb, ng), dim3(NT), 0, s, a, vpg);
+        else
+            hipLaunchKernelGGL((coeff_vjp_kernel<false, false>), dim3(nb, ng), dim3(NT), 0, s, a, vpg);
         if (a.grad_coef)
             hipLaunchKernelGGL(coeff_grad_kernel, dim3(a.nvox_cap), dim3(TQ), 0, s, a, ng);
         if (a.grad_maps && a.grad_tm)
@@ -1874,7 +1880,10 @@ hipError_t launch_coeff_vjp_slabs(CoeffArgs a, int slab_views, hipStream_t s) {
         launch_trace_slab(a, s);
         const int ngs = std::min(ng, a.v_count);
         const int vpg = (a.v_count + ngs - 1) / ngs;
-        hipLaunchKernelGGL(coeff_vjp_kernel<true>, dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
+        if (a.grad_maps)
+            hipLaunchKernelGGL((coeff_vjp_kernel<true, true>), dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
+        else
+            hipLaunchKernelGGL((coeff_vjp_kernel<true, false>), dim3(nb, ngs), dim3(NT), 0, s, a, vpg);
         if (a.gam_buf) {
             const int64_t items = (int64_t)a.v_count * a.n_vox * nbp;
             if (items > 0)
