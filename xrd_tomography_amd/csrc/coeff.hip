@@ -31,6 +31,11 @@ constexpr double kRs2pi = 0.39894228040143267794;
 constexpr int TQ = 128, G = 4, NT = TQ * G, EPT = 16, KC = 16;
 static_assert(TQ == kCoeffFqCols, "fq_buf rows hold the kernels' q lanes");
 constexpr int MAXB = 64, MAXK = 32;
+// Row strides in LDS of the per-voxel rows that the matrix-core contractions read with
+// one voxel per lane (lane % 16): FP32 leg integrals [KC][LROW] (68 floats put the 16
+// voxels' rows in different banks; at 64 every lane of a column hit one bank) and the
+// FP64 per-energy rows mu / W P S [KC][XROW].
+constexpr int LROW = MAXB + 4, XROW = 68;
 
 struct VoxC {
     double s, As, Bs, gR;
@@ -118,7 +123,7 @@ __device__ __forceinline__ void trace_chunk(const CoeffArgs &a, const PixG &pg, 
             const float2 pv = part[jj + KC * (bcc + nbc * rr)];
             s += (double)(comp ? pv.y : pv.x);
         }
-        Ld[jj * MAXB + b] = (float)(s * a.f.ray_step);
+        Ld[jj * LROW + b] = (float)(s * a.f.ray_step);
     }
 }
 
@@ -510,14 +515,14 @@ __device__ __forceinline__ void trace_chunk_tm(const CoeffArgs &a, const PixG &p
                                  a.f.nx, a.f.ny, a.f.ray_step);
         const double sum = wave_trace_tm<FLAT, MAXP, false>(a.maps_tm, a.f.nx, a.f.ny, a.f.ray_step,
                                                             ray, lane, lists[wid]);
-        if (lane < a.n_basis) Ld[j * MAXB + lane] = (float)(sum * a.f.ray_step);
+        if (lane < a.n_basis) Ld[j * LROW + lane] = (float)(sum * a.f.ray_step);
     }
 }
 
 // Attenuation exponents of a voxel chunk's rays on the matrix cores:
-//   mu[j * 64 + e] = sum_b beta[b, e] (Ls[j][b] + Ld[j][b]),  j < KC voxels, e < E,
+//   mu[j * XROW + e] = sum_b beta[b, e] (Ls[j][b] + Ld[j][b]),  j < KC voxels, e < E,
 // the basis contraction mu(E) = sum_b beta[b, E] m_b applied to the rays' line
-// integrals (Ls, Ld: [KC x MAXB] FP32; bet: beta [B x E], b-major).  Waves 0..3 of the
+// integrals (Ls, Ld: [KC x LROW] FP32; bet: beta [B x E], b-major).  Waves 0..3 of the
 // workgroup each take 16 energies (one 16 x 16 tile of mu) and step over the maps four
 // at a time with v_mfma_f64_16x16x4f64: lane l holds A[l % 16][l / 16] (voxel l % 16,
 // map b0 + l / 16), B[l / 16][l % 16] (map b0 + l / 16, energy e0 + l % 16) and
@@ -536,16 +541,16 @@ __device__ __forceinline__ void chunk_mu(const float *Ls, const float *Ld, const
     f64x4v acc = {0.0, 0.0, 0.0, 0.0};
     for (int b0 = 0; b0 < B; b0 += 4) {
         const int b = b0 + kq;
-        const double av = (b < B && i < nk) ? (double)Ls[i * MAXB + b] + (double)Ld[i * MAXB + b] : 0.0;
+        const double av = (b < B && i < nk) ? (double)Ls[i * LROW + b] + (double)Ld[i * LROW + b] : 0.0;
         const double bv = (b < B && e < E) ? (double)bet[b * E + e] : 0.0;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) mu[(kq + 4 * r) * 64 + e] = acc[r];
+    for (int r = 0; r < 4; r++) mu[(kq + 4 * r) * XROW + e] = acc[r];
 }
 
 // Ray weights of a voxel chunk's map gradient on the matrix cores:
-//   gamma[j][b] = -sum_e beta[b, e] X[j * 64 + e],  X = W(e) P(e) S_e,
+//   gamma[j][b] = -sum_e beta[b, e] X[j * XROW + e],  X = W(e) P(e) S_e,
 // and gsrc[j][b] += gamma[j][b] (the source legs' sum over the pixels), for j < nk,
 // b < B (both [KC x MAXB]).  Waves 0..3 each take 16 maps and step over the energies
 // four at a time with v_mfma_f64_16x16x4f64 (the lane layout of chunk_mu, with voxels
@@ -561,7 +566,7 @@ __device__ __forceinline__ void chunk_gamma(const double *X, const float *bet, i
     f64x4v acc = {0.0, 0.0, 0.0, 0.0};
     for (int e0 = 0; e0 < E; e0 += 4) {
         const int e = e0 + kq;
-        const double av = (e < E && j < nk) ? X[j * 64 + e] : 0.0;
+        const double av = (e < E && j < nk) ? X[j * XROW + e] : 0.0;
         const double bv = (e < E && b < B) ? (double)bet[b * E + e] : 0.0;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
@@ -791,11 +796,11 @@ template <bool LDBUF>
 __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
     __shared__ VoxC vox[KC];
     __shared__ float2 part[NT];
-    __shared__ float Ld[KC * MAXB];
+    __shared__ float Ld[KC * LROW];
     __shared__ float Fq[KC * TQ];
     __shared__ double red[(NT / kWave) * EPT + 64];
     __shared__ float bet[MAXB * 64];
-    __shared__ float Lsrc[KC * MAXB];
+    __shared__ float Lsrc[KC * LROW];
     // table rows of kTS entries: the ballot's per-lane reads of one energy across
     // the chunk's voxels then fall into different banks
     constexpr int kTS = 65;
@@ -804,7 +809,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         struct {
             float4 tw[KC * kTS];
             float tb[KC * kTS];
-            double mu[KC * 64];
+            double mu[KC * XROW];
         } tab;
     } ov;
     WaveList *cruns = ov.cruns;
@@ -854,13 +859,13 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
         }
         for (int idx = t; idx < nk * a.n_basis; idx += NT) {
             const int j = idx / a.n_basis, b = idx % a.n_basis;
-            Lsrc[j * MAXB + b] = a.src_ints[((size_t)v * npix + k0 + j) * a.n_basis + b];
+            Lsrc[j * LROW + b] = a.src_ints[((size_t)v * npix + k0 + j) * a.n_basis + b];
         }
         __syncthreads();
         if (LDBUF) {
             for (int idx = t; idx < nk * a.n_basis; idx += NT) {
                 const int j = idx / a.n_basis, b = idx % a.n_basis;
-                Ld[j * MAXB + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * f.n_det + p) *
+                Ld[j * LROW + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * f.n_det + p) *
                                                 kTexelMaps + b];
             }
         } else if (a.maps_tm) {
@@ -878,7 +883,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_forward_kernel(CoeffArgs a) {
             float4 tv;
             float band;
             double P;
-            table_entry(a, vox[j], mu[j * 64 + e], e, tv, band, P);
+            table_entry(a, vox[j], mu[j * XROW + e], e, tv, band, P);
             tw[j * kTS + e] = tv;
             tb[j * kTS + e] = band;
         }
@@ -1320,12 +1325,12 @@ __device__ __forceinline__ void lane_run(int n, int r, int sp, int &q0, int &q1)
 template <bool LDBUF, bool MAPS>
 __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views_per_group) {
     __shared__ VoxC vox[KC];
-    __shared__ float Ld[KC * MAXB];
+    __shared__ float Ld[KC * LROW];
     __shared__ float Fq[KC * TQ];
     __shared__ double gam[KC * MAXB];
     __shared__ double gsrc[KC * MAXB];
     __shared__ double wt[64];
-    __shared__ float Lsrc[KC * MAXB];
+    __shared__ float Lsrc[KC * LROW];
     // table rows of kTS entries: the ballots' lane-indexed reads of one energy across
     // the chunk's voxels then fall into different banks
     constexpr int kTS = 65;
@@ -1337,7 +1342,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
         struct {
             float4 tw[KC * kTS];
             float tb[KC * kTS];
-            double pw[KC * 64];                // W(e) P(e)
+            double pw[KC * XROW];              // W(e) P(e)
             float sp_[NT / kWave][KC][16];     // per-wave partial S
         } gauss;
     } ov;
@@ -1383,7 +1388,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
         for (int idx = t; idx < KC * MAXB; idx += NT) gsrc[idx] = 0.0;
         for (int idx = t; idx < nk * B; idx += NT) {
             const int j = idx / B, b = idx % B;
-            Lsrc[j * MAXB + b] = a.src_ints[((size_t)v * npix + k0 + j) * B + b];
+            Lsrc[j * LROW + b] = a.src_ints[((size_t)v * npix + k0 + j) * B + b];
         }
         for (int p = 0; p < P; p++) {
             const PixG pg = pixel_setup(a, v, p);
@@ -1415,7 +1420,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
             if (LDBUF) {
                 for (int idx = t; idx < nk * B; idx += NT) {
                     const int j = idx / B, b = idx % B;
-                    Ld[j * MAXB + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * P + p) *
+                    Ld[j * LROW + b] = a.ld_buf[(((size_t)(v - a.v_base) * a.n_vox + (size_t)(k0 + j)) * P + p) *
                                                     kTexelMaps + b];
                 }
             } else if (a.maps_tm) {
@@ -1433,8 +1438,8 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 float4 tv;
                 float band;
                 double Pe;
-                table_entry(a, vox[j], pw[j * 64 + e], e, tv, band, Pe);
-                pw[j * 64 + e] = wt[e] * Pe;
+                table_entry(a, vox[j], pw[j * XROW + e], e, tv, band, Pe);
+                pw[j * XROW + e] = wt[e] * Pe;
                 tv.w = (float)(wt[e] * Pe);
                 tw[j * kTS + e] = tv;
                 tb[j * kTS + e] = band;
@@ -1491,7 +1496,7 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                     double se = 0.0;
                     for (int w = hh * (TQ / kWave); w < (hh + 1) * (TQ / kWave); w++)
                         se += (double)sp_[w][j][i];
-                    pw[j * 64 + e] *= se;
+                    pw[j * XROW + e] *= se;
                 }
                 __syncthreads();
                 chunk_gamma(pw, bet, B, E, nk, gam, gsrc);
