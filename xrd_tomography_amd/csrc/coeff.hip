@@ -1445,43 +1445,81 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                 tb[j * kTS + e] = band;
             }
             __syncthreads();
-            // per energy slot, one ballot gives the chunk's voxels whose band reaches
-            // this wave's bins (the range and the energy are wave-uniform); the voxel
-            // loop then reads only those entries
-            uint32_t inb[16];
+            if (!MAPS) {
+                // coefficients only: one ballot per voxel gives its energy slots whose
+                // band reaches this wave's bins (lane i: slot i); the wave then adds
+                // those terms in ascending slot order, two per step with both table
+                // reads issued before either is used
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int e = h + G * i;
-                bool in = false;
-                if (i < EPT && e < E && lane < nk) {
-                    const float x = tw[lane * kTS + e].x, band = tb[lane * kTS + e];
-                    in = x + band >= lo && x - band <= hi;
-                }
-                inb[i] = (uint32_t)__ballot(in);
-            }
-            // h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n; S_e partials over this wave's bins
-#pragma unroll
-            for (int j = 0; j < KC; j++) {
-                if (j >= nk) break;
-                const float fq = Fq[j * TQ + (tq < Q ? tq : 0)] * dqc;
-                float hp = 0.f;
-                float sv[16];
-#pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    sv[i] = 0.f;
-                    const int e = h + G * i;
-                    if ((inb[i] >> j) & 1u) {
-                        const float4 tv = tw[j * kTS + e];
-                        const float d = (tv.x - qch) + (tv.y - qcl);
-                        const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
-                        hp = fmaf(tv.w, g, hp);
-                        sv[i] = g * fq;
+                for (int j = 0; j < KC; j++) {
+                    if (j >= nk) break;
+                    bool in = false;
+                    if (lane < EPT && h + G * lane < E) {
+                        const int e = h + G * lane;
+                        const float x = tw[j * kTS + e].x, band = tb[j * kTS + e];
+                        in = x + band >= lo && x - band <= hi;
                     }
+                    uint32_t m = (uint32_t)__ballot(in);
+                    auto term = [&](int i) {
+                        const float4 tv = tw[j * kTS + h + G * i];
+                        const float d = (tv.x - qch) + (tv.y - qcl);
+                        return make_float2(tv.w, __builtin_amdgcn_exp2f(d * d * tv.z));
+                    };
+                    float hp = 0.f;
+                    while (m & (m - 1)) {
+                        const int i0 = __builtin_ctz(m);
+                        m &= m - 1;
+                        const int i1 = __builtin_ctz(m);
+                        m &= m - 1;
+                        const float2 u0 = term(i0), u1 = term(i1);
+                        hp = fmaf(u0.x, u0.y, hp);
+                        hp = fmaf(u1.x, u1.y, hp);
+                    }
+                    if (m) {
+                        const float2 u = term(__builtin_ctz(m));
+                        hp = fmaf(u.x, u.y, hp);
+                    }
+                    ht[j] += (double)(hp * dqc);
                 }
-                ht[j] += (double)(hp * dqc);
-                if (maps) {
-                    const float st = transpose_sum16(sv, lane);
-                    if ((lane & 3) == 0) sp_[wid][j][(lane >> 2) & 15] = st;
+            } else {
+                // per energy slot, one ballot gives the chunk's voxels whose band reaches
+                // this wave's bins (the range and the energy are wave-uniform); the voxel
+                // loop then reads only those entries
+                uint32_t inb[16];
+    #pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int e = h + G * i;
+                    bool in = false;
+                    if (i < EPT && e < E && lane < nk) {
+                        const float x = tw[lane * kTS + e].x, band = tb[lane * kTS + e];
+                        in = x + band >= lo && x - band <= hi;
+                    }
+                    inb[i] = (uint32_t)__ballot(in);
+                }
+                // h_vox(q_n) += W(e) P(e) G_e(q_n) dq_n; S_e partials over this wave's bins
+    #pragma unroll
+                for (int j = 0; j < KC; j++) {
+                    if (j >= nk) break;
+                    const float fq = Fq[j * TQ + (tq < Q ? tq : 0)] * dqc;
+                    float hp = 0.f;
+                    float sv[16];
+    #pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        sv[i] = 0.f;
+                        const int e = h + G * i;
+                        if ((inb[i] >> j) & 1u) {
+                            const float4 tv = tw[j * kTS + e];
+                            const float d = (tv.x - qch) + (tv.y - qcl);
+                            const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
+                            hp = fmaf(tv.w, g, hp);
+                            sv[i] = g * fq;
+                        }
+                    }
+                    ht[j] += (double)(hp * dqc);
+                    if (maps) {
+                        const float st = transpose_sum16(sv, lane);
+                        if ((lane & 3) == 0) sp_[wid][j][(lane >> 2) & 15] = st;
+                    }
                 }
             }
             __syncthreads();
