@@ -105,6 +105,29 @@ def test_vjp_dot_product(xrd):
     assert abs(a - b) <= 1e-6 * max(abs(a), abs(b))
 
 
+@pytest.mark.parametrize("scatter", ["bundled", "inplace"])
+def test_vjp_coef_grad_same_with_and_without_maps(xrd, scatter, monkeypatch):
+    """The coefficient-only VJP and the map VJP run separate instances of the VJP
+    kernel (only the map form forms the S partials, and the two walk the Gaussian terms
+    differently, in the same order); within one slab of views their coefficient
+    gradients are the same bits."""
+    import torch
+    if scatter == "inplace":
+        monkeypatch.setenv("XRD_VJP_SCATTER", "inplace")
+    cp = _case()
+    args = xrd.coeff_problem_tensors(cp)
+    g, d = cp.base.geometry, cp.base.disc
+    rng = np.random.default_rng(6)
+    w = torch.from_numpy(rng.normal(size=cp.base.n_rows).astype(np.float32)).cuda()
+    gc_only, gm_none = xrd.vjp_coeffs(*args, w, geometry=g, disc=d, want_maps=False)
+    gc_maps, gm = xrd.vjp_coeffs(*args, w, geometry=g, disc=d)
+    torch.cuda.synchronize()
+    assert gm_none is None and gm is not None
+    diff = int((gc_only != gc_maps).sum())
+    print(f"\n[vjp {scatter}] coefficient gradient elements differing with/without maps: {diff}")
+    assert torch.equal(gc_only, gc_maps)
+
+
 def test_poisson_nll(xrd):
     import torch
     rng = np.random.default_rng(5)
