@@ -1503,16 +1503,23 @@ __global__ __launch_bounds__(NT, 4) void coeff_vjp_kernel(CoeffArgs a, int views
                     const float fq = Fq[j * TQ + (tq < Q ? tq : 0)] * dqc;
                     float hp = 0.f;
                     float sv[16];
-    #pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        sv[i] = 0.f;
-                        const int e = h + G * i;
-                        if ((inb[i] >> j) & 1u) {
-                            const float4 tv = tw[j * kTS + e];
-                            const float d = (tv.x - qch) + (tv.y - qcl);
-                            const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
-                            hp = fmaf(tv.w, g, hp);
-                            sv[i] = g * fq;
+                        // two slots' entries read before either is tested (every slot's
+                    // index lies inside the table row, G * 16 <= kTS; a slot past E
+                    // reads an unused entry and its bit is clear)
+                    static_assert(G * 16 <= kTS, "slot reads stay in the row");
+#pragma unroll
+                    for (int i = 0; i < 16; i += 2) {
+                        const float4 tp[2] = {tw[j * kTS + h + G * i], tw[j * kTS + h + G * (i + 1)]};
+#pragma unroll
+                        for (int u = 0; u < 2; u++) {
+                            sv[i + u] = 0.f;
+                            if ((inb[i + u] >> j) & 1u) {
+                                const float4 tv = tp[u];
+                                const float d = (tv.x - qch) + (tv.y - qcl);
+                                const float g = __builtin_amdgcn_exp2f(d * d * tv.z);
+                                hp = fmaf(tv.w, g, hp);
+                                sv[i + u] = g * fq;
+                            }
                         }
                     }
                     ht[j] += (double)(hp * dqc);
